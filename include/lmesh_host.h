@@ -1,0 +1,48 @@
+/*
+ * lmesh_host.h -- C entry points of the host-side libMesh API subset
+ * (libmesh_amd/host): mesh generation and DoF numbering as libMesh performs
+ * them, flattened to arrays for callers without C++ (ctypes, tests, bench).
+ *
+ * They have no GPU counterpart: they produce the inputs that lmhip.h's
+ * lmh_mesh_create / lmh_dofmap_create take.
+ */
+#ifndef LMESH_HOST_H
+#define LMESH_HOST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lmesh_problem lmesh_problem;
+
+/* MeshTools::Generation::build_cube on the unit cube (mesh_generation.C:317),
+ * then DofMap::distribute_dofs for n_vars Lagrange variables of the element's
+ * own order (dof_map.C:949) with zero Dirichlet boundaries on the sides whose
+ * boundary id bit is set in dirichlet_mask (DofMap::add_dirichlet_boundary).
+ * NULL on invalid input (message in lmesh_last_error). */
+lmesh_problem *lmesh_build_cube(int nx, int ny, int nz, int elem_type, int n_vars, unsigned dirichlet_mask);
+void lmesh_free(lmesh_problem *p);
+const char *lmesh_last_error(void);
+/* Seeded interior perturbation U[-frac*h, frac*h] per coordinate (h = node
+ * spacing); numbering and constraints are unchanged. */
+void lmesh_perturb(lmesh_problem *p, uint64_t seed, double frac);
+/* kappa_e = exp(N(0,1)) per element from mt19937_64(seed) + Box-Muller. */
+void lmesh_lognormal(int64_t n, uint64_t seed, double *out);
+
+int64_t lmesh_n_nodes(const lmesh_problem *p);
+int64_t lmesh_n_elem(const lmesh_problem *p);
+int64_t lmesh_n_dofs(const lmesh_problem *p);
+int lmesh_nodes_per_elem(const lmesh_problem *p);
+int lmesh_dofs_per_elem(const lmesh_problem *p);
+int lmesh_elem_type(const lmesh_problem *p);
+void lmesh_coords(const lmesh_problem *p, double *xyz);       /* n_nodes*3 */
+void lmesh_conn(const lmesh_problem *p, int32_t *conn);       /* n_elem*npe */
+void lmesh_elem_dofs(const lmesh_problem *p, int32_t *dofs);  /* n_elem*dpe */
+void lmesh_constrained(const lmesh_problem *p, uint8_t *c);   /* n_dofs */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,187 @@
+/*
+ * lmhip.h -- C ABI of the MI355X-native libMesh assembly + Jacobi-CG hot path.
+ *
+ * Every entry point takes plain pointers, sizes and opaque handles and returns
+ * an int status (LMH_OK = 0).  The libMesh-facing C++ classes (HipCsrMatrix,
+ * HipLinearSolver, HipAssembler) call only these functions; the comment on each
+ * names the reference interface it replaces (paths relative to the libMesh
+ * source tree).
+ *
+ * Pointer conventions: arguments suffixed _h are host pointers, _d are device
+ * pointers (allocated with lmh_malloc or by any HIP allocator on the context's
+ * device).  All work is enqueued on the context's stream; results in device
+ * memory are ready after lmh_synchronize or any call that returns host data.
+ */
+#ifndef LMHIP_H
+#define LMHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (mapped by the C++ shim to libMesh's exceptions:
+ * LMH_ERR_INVALID -> LogicError (libmesh_common.h:441),
+ * LMH_ERR_DEGENERATE -> DegenerateMap (fe_map.C:467-505),
+ * LMH_ERR_UNSUPPORTED -> libmesh_not_implemented) ---- */
+enum {
+  LMH_OK = 0,
+  LMH_ERR_INVALID = 1,
+  LMH_ERR_HIP = 2,
+  LMH_ERR_DEGENERATE = 3,
+  LMH_ERR_ALLOC = 4,
+  LMH_ERR_UNSUPPORTED = 5
+};
+
+/* libMesh ElemType values (include/enums/enum_elem_type.h). */
+enum { LMH_TET4 = 8, LMH_TET10 = 9, LMH_HEX8 = 10, LMH_HEX27 = 12 };
+
+/* Arithmetic of a call. */
+enum { LMH_FP64 = 0, LMH_FP32 = 1 };
+
+/* Scatter strategy for Ke/Fe -> CSR. */
+enum { LMH_SCATTER_COLORED = 0, LMH_SCATTER_GATHER = 1 };
+
+/* Converged reasons (subset of LinearConvergenceReason,
+ * include/enums/enum_convergence_flags.h:33). */
+enum { LMH_CONVERGED_RTOL = 2, LMH_DIVERGED_ITS = -3, LMH_DIVERGED_NANORINF = -9, LMH_DIVERGED_BREAKDOWN = -5 };
+
+typedef struct lmh_context lmh_context;
+typedef struct lmh_mesh lmh_mesh;
+typedef struct lmh_dofmap lmh_dofmap;
+typedef struct lmh_matrix lmh_matrix;
+
+/* Last error message of the calling thread ("" if none). */
+const char *lmh_last_error(void);
+
+/* ---- context: one device + one stream (LibMeshInit / SolverPackage selection,
+ * src/base/libmesh.C:1064-1122) ---- */
+int lmh_context_create(int device, lmh_context **out);
+int lmh_context_destroy(lmh_context *ctx);
+int lmh_synchronize(lmh_context *ctx);
+/* hipStream_t of the context, as void* */
+void *lmh_context_stream(lmh_context *ctx);
+
+/* ---- device memory (NumericVector storage, numeric_vector.h) ---- */
+int lmh_malloc(lmh_context *ctx, size_t bytes, void **d_out);
+int lmh_free(lmh_context *ctx, void *d);
+int lmh_memcpy_h2d(lmh_context *ctx, void *d_dst, const void *h_src, size_t bytes);
+int lmh_memcpy_d2h(lmh_context *ctx, void *h_dst, const void *d_src, size_t bytes);
+int lmh_memcpy_d2d(lmh_context *ctx, void *d_dst, const void *d_src, size_t bytes);
+int lmh_memset(lmh_context *ctx, void *d, int value, size_t bytes);
+
+/* ---- mesh: element connectivity + nodal coordinates, the data FE::reinit reads
+ * through Elem::point (src/fe/fe.C:197-368, fe_map.C:958-1090).
+ * coords_h: n_nodes*3 FP64; conn_h: n_elem*nodes_per_elem (libMesh local node
+ * order, 0-based node ids). ---- */
+int lmh_mesh_create(lmh_context *ctx, int elem_type, int64_t n_nodes, const double *coords_h, int64_t n_elem,
+                    const int32_t *conn_h, lmh_mesh **out);
+int lmh_mesh_destroy(lmh_mesh *m);
+/* Replace the nodal coordinates (e.g. after MeshTools::Modification). */
+int lmh_mesh_set_coords(lmh_mesh *m, const double *coords_h);
+
+/* ---- DoF map: DofMap::dof_indices per element (dof_map.C:2201-2316) and the
+ * Dirichlet-constrained set (DofMap::is_constrained_dof after
+ * create_dof_constraints, dof_map_constraints.C:592-740).
+ * elem_dofs_h: n_elem * (n_vars*nodes_per_elem), variable-major per element;
+ * constrained_h: n_dofs bytes, nonzero = constrained (may be NULL). ---- */
+int lmh_dofmap_create(lmh_context *ctx, const lmh_mesh *m, int n_vars, int64_t n_dofs, const int32_t *elem_dofs_h,
+                      const uint8_t *constrained_h, lmh_dofmap **out);
+int lmh_dofmap_destroy(lmh_dofmap *dm);
+
+/* ---- sparsity + matrix: DofMap::compute_sparsity / SparsityPattern::Build
+ * (dof_map.C:62-133, 1960-1978; sparsity_pattern.C:96-600) built on the GPU,
+ * handed to SparseMatrix::init/update_sparsity_pattern (sparse_matrix.h,
+ * eigen_sparse_matrix.C:64-115).  Also builds the element scatter map and the
+ * element colouring used by lmh_assemble_*. ---- */
+int lmh_matrix_create(lmh_context *ctx, const lmh_dofmap *dm, lmh_matrix **out);
+int lmh_matrix_destroy(lmh_matrix *A);
+int64_t lmh_matrix_rows(const lmh_matrix *A);
+int64_t lmh_matrix_nnz(const lmh_matrix *A);
+int lmh_matrix_n_colors(const lmh_matrix *A);
+/* Download the pattern (row_ptr_h: rows+1, col_idx_h: nnz). */
+int lmh_matrix_get_pattern(const lmh_matrix *A, int32_t *row_ptr_h, int32_t *col_idx_h);
+/* Device pointers of the CSR arrays (row_ptr int32, col_idx int32, FP64 values). */
+int lmh_matrix_device_arrays(lmh_matrix *A, const int32_t **row_ptr_d, const int32_t **col_idx_d, double **vals_d);
+/* SparseMatrix::zero (eigen_sparse_matrix.C:186-198). */
+int lmh_matrix_zero(lmh_matrix *A);
+/* Values in/out, FP64, nnz entries in pattern order. */
+int lmh_matrix_get_values(const lmh_matrix *A, double *vals_h);
+int lmh_matrix_set_values(lmh_matrix *A, const double *vals_h);
+/* SparseMatrix::add_matrix(DenseMatrix, dofs) for a batch of dense element
+ * matrices held on the device (eigen_sparse_matrix.C:121-135; laspack_matrix.C:626-640):
+ * Ke_d: n_elem*n*n row-major, dofs_d: n_elem*n.  Entries must exist in the
+ * pattern.  Deterministic: each entry accumulates in ascending batch order. */
+int lmh_matrix_add_element_matrices(lmh_matrix *A, int64_t n_elem, int n, const double *Ke_d, const int32_t *dofs_d);
+/* NumericVector::add_vector(values, dofs) for a batch (numeric_vector.C:690-697). */
+int lmh_vector_add_element_vectors(lmh_context *ctx, double *v_d, int64_t n_elem, int n, const double *Fe_d,
+                                   const int32_t *dofs_d);
+
+/* ---- assembly: System::assemble for -div(kappa grad u) = f with QGauss(npts^3)
+ * and Lagrange FE of the element's own order (introduction_ex4.C:313-560;
+ * kappa-weighted form adjoints_ex2/L-shaped.C:101), element constraints
+ * (DofMap::heterogeneously_constrain_element_matrix_and_vector,
+ * dof_map_constraints.C:2518-2616) and the CSR/rhs scatter (eigen_sparse_matrix.C:121-135,
+ * numeric_vector.C:690-697).  Overwrites the matrix values and rhs_d.
+ * kappa_d: per-element coefficient (NULL -> 1); f_kind 0: f = 1, 1: f =
+ * 3 pi^2 sin(pi x) sin(pi y) sin(pi z).  precision LMH_FP64 builds the FP64
+ * CSR; LMH_FP32 runs the element contraction in FP32 on MFMA and writes the
+ * FP32 operator used by the mixed-precision inner solver (geometry stays FP64).
+ * scatter: LMH_SCATTER_COLORED or LMH_SCATTER_GATHER. */
+int lmh_assemble_poisson(lmh_matrix *A, const lmh_mesh *m, const lmh_dofmap *dm, int npts, const double *kappa_d,
+                         int f_kind, double *rhs_d, int precision, int scatter);
+/* Element matrices/vectors only (no scatter), constrained if constrain != 0:
+ * Ke_d n_elem*dpe*dpe, Fe_d n_elem*dpe. */
+int lmh_element_poisson(lmh_context *ctx, const lmh_mesh *m, const lmh_dofmap *dm, int npts, const double *kappa_d,
+                        int f_kind, int constrain, double *Ke_d, double *Fe_d);
+
+/* ---- linear algebra ---- */
+/* y = A x (SparseMatrix::vector_mult, sparse_matrix.C:237-253). */
+int lmh_spmv(lmh_matrix *A, const double *x_d, double *y_d);
+/* r = b - A x in FP64 (LinearImplicitSystem::assembly residual, linear_implicit_system.C:367-381). */
+int lmh_residual(lmh_matrix *A, const double *b_d, const double *x_d, double *r_d);
+double lmh_norm2(lmh_context *ctx, const double *x_d, int64_t n);
+
+/* Jacobi-preconditioned CG, FP64 (EigenSparseLinearSolver::solve with CG +
+ * JACOBI_PRECOND -> Eigen::internal::conjugate_gradient,
+ * eigen_sparse_linear_solver.C:82-170, ConjugateGradient.h:28-90).  x_d holds
+ * the initial guess.  Stops when |r|^2 < max(rtol^2 |b|^2, DBL_MIN); returns
+ * the iteration count and sqrt(|r|^2/|b|^2) exactly as Eigen reports them. */
+int lmh_cg_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int64_t maxits, int64_t *iters,
+                 double *rel_res, int *reason);
+
+/* Mixed precision: FP32 Jacobi-CG on the FP32 operator inside FP64 refinement
+ * of the FP64 residual r = b - A x.  mode 0: reliable updates (the FP32
+ * recurrence continues; r and x are refreshed in FP64 whenever the FP32
+ * residual falls below inner_rtol times the last true residual); mode 1:
+ * classic iterative refinement (each correction solve restarts from zero and
+ * runs to inner_rtol).  Stops when |b - A x| / |b| <= rtol.  Requires both the
+ * FP64 and FP32 operators (lmh_assemble_poisson with each precision, or
+ * lmh_matrix_make_fp32).  stats_h (may be NULL) receives
+ * {inner iterations, outer refreshes, final true relative residual}. */
+int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, double inner_rtol, int mode,
+                    int64_t maxits, int64_t *iters, double *rel_res, int *reason, double *stats_h);
+/* Build the FP32 operator by rounding the FP64 values. */
+int lmh_matrix_make_fp32(lmh_matrix *A);
+
+/* ---- adjoint sensitivity (ImplicitSystem::adjoint_qoi_parameter_sensitivity,
+ * implicit_system.C:517-623, for per-element kappa parameters) ---- */
+/* dJ/du for J = integral of u over [lo,hi]^3 (per-qp inclusion), constrained
+ * entries zero (System::assemble_qoi_derivative + enforce_adjoint_constraints_exactly). */
+int lmh_qoi_box_derivative(const lmh_mesh *m, const lmh_dofmap *dm, int npts, double lo, double hi, double *dq_d);
+/* dJ/dkappa_e = -(1/kappa_e) lambda_e^T Kc_e u_e for every element, FP64. */
+int lmh_adjoint_kappa_gradient(const lmh_mesh *m, const lmh_dofmap *dm, int npts, const double *kappa_d,
+                               const double *u_d, const double *lambda_d, double *grad_d);
+
+/* ---- per-kernel timing with HIP events on the context stream ---- */
+int lmh_profile_enable(lmh_context *ctx, int on);
+/* Average duration in ms and launch count of a named kernel since enable. */
+int lmh_profile_query(lmh_context *ctx, const char *kernel, double *avg_ms, int64_t *count);
+int lmh_profile_reset(lmh_context *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

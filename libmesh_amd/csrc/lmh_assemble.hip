@@ -1,0 +1,822 @@
+// lmh_assemble.hip -- batched FE::reinit + element contraction + element
+// constraints + deterministic CSR scatter (the body of System::assemble for
+// -div(kappa grad u) = f), plus the QoI derivative and adjoint kernels.
+//
+// Per element e and quadrature point q (fe_map.C:958-1090, h1_fe_transformation.C:64):
+//   J_q[r][c] = sum_a X_a[c] dphi_a/dxi_r(q),  det_q = det J_q,  G_q = J_q^{-1}
+//   JxW_q = det_q w_q,  dphi_a(q) = G_q^T dref_a(q)
+// so Ke_ij = sum_q JxW_q kappa_e dphi_i(q).dphi_j(q)
+//          = sum_q dref_i(q)^T M_q dref_j(q),   M_q = JxW_q kappa_e G_q G_q^T,
+// i.e. Ke = Gref^T H_e with Gref[(q,r)][i] = dref_i(q)[r] constant over elements
+// and H_e[(q,r)][j] = (M_q dref_j(q))[r].  Geometry (J, det, inverse) is FP64.
+// The contraction runs on MFMA for HEX27 (v_mfma_f64_16x16x4f64 / f32), on
+// wavefront FMA for HEX8, and on a generic per-element kernel otherwise.
+#include <cmath>
+
+#include "lmh_internal.h"
+
+namespace lmh {
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__device__ inline double forcing(int f_kind, double x, double y, double z) {
+  if (f_kind == 0) return 1.0;
+  const double pi = 3.14159265358979323846264338327950288;
+  return 3.0 * pi * pi * sin(pi * x) * sin(pi * y) * sin(pi * z);
+}
+
+// Geometry at one quadrature point from LDS coordinates X[a*3+c] and the
+// reference derivative table dref[(a*nq+q)*3+r].  Writes M (9, symmetric,
+// scaled by JxW*kappa) and returns JxW (<= 0 signals a degenerate map).
+__device__ inline double qp_geometry(const double *X, const double *dref, int n, int nq, int q, double w,
+                                     double kap, double *M) {
+  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // J[r][c] = d x_c / d xi_r
+  for (int a = 0; a < n; ++a) {
+    const double *d = &dref[(a * nq + q) * 3];
+    const double x = X[a * 3], y = X[a * 3 + 1], z = X[a * 3 + 2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      J[r][0] += x * d[r];
+      J[r][1] += y * d[r];
+      J[r][2] += z * d[r];
+    }
+  }
+  // cofactor inverse as FEMap::compute_single_point_map (fe_map.C:1046-1068)
+  const double dx_dxi = J[0][0], dy_dxi = J[0][1], dz_dxi = J[0][2];
+  const double dx_deta = J[1][0], dy_deta = J[1][1], dz_deta = J[1][2];
+  const double dx_dzeta = J[2][0], dy_dzeta = J[2][1], dz_dzeta = J[2][2];
+  const double det = (dx_dxi * (dy_deta * dz_dzeta - dz_deta * dy_dzeta) +
+                      dy_dxi * (dz_deta * dx_dzeta - dx_deta * dz_dzeta) +
+                      dz_dxi * (dx_deta * dy_dzeta - dy_deta * dx_dzeta));
+  if (!(det > 0.0)) return -1.0;
+  const double inv = 1.0 / det;
+  double G[3][3];  // G[r][c] = d xi_r / d x_c
+  G[0][0] = (dy_deta * dz_dzeta - dz_deta * dy_dzeta) * inv;
+  G[0][1] = (dz_deta * dx_dzeta - dx_deta * dz_dzeta) * inv;
+  G[0][2] = (dx_deta * dy_dzeta - dy_deta * dx_dzeta) * inv;
+  G[1][0] = (dz_dxi * dy_dzeta - dy_dxi * dz_dzeta) * inv;
+  G[1][1] = (dx_dxi * dz_dzeta - dz_dxi * dx_dzeta) * inv;
+  G[1][2] = (dy_dxi * dx_dzeta - dx_dxi * dy_dzeta) * inv;
+  G[2][0] = (dy_dxi * dz_deta - dz_dxi * dy_deta) * inv;
+  G[2][1] = (dz_dxi * dx_deta - dx_dxi * dz_deta) * inv;
+  G[2][2] = (dx_dxi * dy_deta - dy_dxi * dx_deta) * inv;
+  const double jxw = det * w;
+  const double s = jxw * kap;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int t = r; t < 3; ++t) {
+      const double v = s * (G[r][0] * G[t][0] + G[r][1] * G[t][1] + G[r][2] * G[t][2]);
+      M[r * 3 + t] = v;
+      M[t * 3 + r] = v;
+    }
+  return jxw;
+}
+
+// ---------------------------------------------------------------------------
+// Generic Poisson element kernel: one wave per element (any type, nq <= 64).
+// ---------------------------------------------------------------------------
+template <typename CT>
+__global__ __launch_bounds__(64) void k_poisson_generic(const double *__restrict__ tab, int n, int nq,
+                                                        const double *__restrict__ xyz, const int32_t *__restrict__ conn,
+                                                        const int32_t *__restrict__ edofs,
+                                                        const uint8_t *__restrict__ cons,
+                                                        const double *__restrict__ kappa, int f_kind, int constrain,
+                                                        int64_t ne, CT *__restrict__ Ke, double *__restrict__ Fe,
+                                                        int *__restrict__ err) {
+  extern __shared__ double sm[];
+  const int lane = threadIdx.x;
+  double *w = sm;
+  double *phi = w + nq;
+  double *dref = phi + n * nq;
+  double *X = dref + n * nq * 3;
+  double *M = X + 81;
+  double *Jf = M + 9 * nq;
+  for (int t = lane; t < nq + n * nq * 4; t += 64) sm[t] = tab[t];
+  __syncthreads();
+  for (int64_t e = blockIdx.x; e < ne; e += gridDim.x) {
+    if (lane < n) {
+      const int32_t nd = conn[e * n + lane];
+      X[lane * 3] = xyz[3 * (int64_t)nd];
+      X[lane * 3 + 1] = xyz[3 * (int64_t)nd + 1];
+      X[lane * 3 + 2] = xyz[3 * (int64_t)nd + 2];
+    }
+    const bool ci = (lane < n) ? (cons[edofs[e * n + lane]] != 0) : false;
+    const unsigned long long cmask = constrain ? __ballot(ci) : 0ull;
+    const double kap = kappa ? kappa[e] : 1.0;
+    __syncthreads();
+    if (lane < nq) {
+      double Mq[9];
+      const double jxw = qp_geometry(X, dref, n, nq, lane, w[lane], kap, Mq);
+      if (jxw <= 0.0) atomicOr(err, 1);
+      for (int t = 0; t < 9; ++t) M[lane * 9 + t] = Mq[t];
+      double f = 1.0;
+      if (f_kind != 0) {
+        double x = 0, y = 0, z = 0;
+        for (int a = 0; a < n; ++a) {
+          const double p = phi[a * nq + lane];
+          x += X[a * 3] * p;
+          y += X[a * 3 + 1] * p;
+          z += X[a * 3 + 2] * p;
+        }
+        f = forcing(f_kind, x, y, z);
+      }
+      Jf[lane] = jxw * f;
+    }
+    __syncthreads();
+    if (lane < n) {
+      double row[27];
+      for (int j = 0; j < n; ++j) row[j] = 0.0;
+      for (int q = 0; q < nq; ++q) {
+        const double *di = &dref[(lane * nq + q) * 3];
+        const double *Mq = &M[q * 9];
+        double v[3];
+        for (int s = 0; s < 3; ++s) v[s] = di[0] * Mq[s] + di[1] * Mq[3 + s] + di[2] * Mq[6 + s];
+        for (int j = 0; j < n; ++j) {
+          const double *dj = &dref[(j * nq + q) * 3];
+          row[j] += (CT)v[0] * (CT)dj[0] + (CT)v[1] * (CT)dj[1] + (CT)v[2] * (CT)dj[2];
+        }
+      }
+      double fe = 0.0;
+      for (int q = 0; q < nq; ++q) fe += Jf[q] * phi[lane * nq + q];
+      const bool myc = (cmask >> lane) & 1ull;
+      for (int j = 0; j < n; ++j) {
+        double v = row[j];
+        if (myc || ((cmask >> j) & 1ull)) v = (j == lane && myc) ? 1.0 : 0.0;
+        Ke[(e * n + lane) * n + j] = (CT)v;
+      }
+      Fe[e * n + lane] = myc ? 0.0 : fe;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Q1 HEX8, 2x2x2 Gauss: wavefront-level FMA, 8 elements per wave, lane = 8*el + t
+// (t = quadrature point in the geometry phase and matrix row in the contraction).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_poisson_hex8(const double *__restrict__ tab, const double *__restrict__ xyz,
+                                                      const int32_t *__restrict__ conn,
+                                                      const int32_t *__restrict__ edofs,
+                                                      const uint8_t *__restrict__ cons,
+                                                      const double *__restrict__ kappa, int f_kind, int constrain,
+                                                      int64_t ne, double *__restrict__ Ke, double *__restrict__ Fe,
+                                                      int *__restrict__ err) {
+  __shared__ double s_tab[8 + 64 + 192];
+  __shared__ double s_X[4][8][24];
+  __shared__ double s_M[4][8][8 * 9];
+  __shared__ double s_Jf[4][8][8];
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, el = lane >> 3, t = lane & 7;
+  for (int i = tid; i < 8 + 64 + 192; i += 256) s_tab[i] = tab[i];
+  __syncthreads();
+  const double *w = s_tab, *phi = s_tab + 8, *dref = s_tab + 72;
+  const int64_t groups = (ne + 31) / 32;
+  for (int64_t g = blockIdx.x; g < groups; g += gridDim.x) {
+    const int64_t e = g * 32 + wv * 8 + el;
+    const bool valid = e < ne;
+    double *X = s_X[wv][el];
+    bool ci = false;
+    double kap = 1.0;
+    if (valid) {
+      const int32_t nd = conn[e * 8 + t];
+      X[t * 3] = xyz[3 * (int64_t)nd];
+      X[t * 3 + 1] = xyz[3 * (int64_t)nd + 1];
+      X[t * 3 + 2] = xyz[3 * (int64_t)nd + 2];
+      ci = cons[edofs[e * 8 + t]] != 0;
+      if (kappa) kap = kappa[e];
+    }
+    const unsigned long long bal = __ballot(ci);
+    const unsigned cm = constrain ? (unsigned)((bal >> (8 * el)) & 0xffull) : 0u;
+    __syncthreads();
+    if (valid) {
+      double Mq[9];
+      const double jxw = qp_geometry(X, dref, 8, 8, t, w[t], kap, Mq);
+      if (jxw <= 0.0) atomicOr(err, 1);
+      for (int k = 0; k < 9; ++k) s_M[wv][el][t * 9 + k] = Mq[k];
+      double f = 1.0;
+      if (f_kind != 0) {
+        double x = 0, y = 0, z = 0;
+        for (int a = 0; a < 8; ++a) {
+          const double p = phi[a * 8 + t];
+          x += X[a * 3] * p;
+          y += X[a * 3 + 1] * p;
+          z += X[a * 3 + 2] * p;
+        }
+        f = forcing(f_kind, x, y, z);
+      }
+      s_Jf[wv][el][t] = jxw * f;
+    }
+    __syncthreads();
+    if (valid) {
+      double row[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      const double *Mb = s_M[wv][el];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const double *di = &dref[(t * 8 + q) * 3];
+        const double *Mq = &Mb[q * 9];
+        double v[3];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) v[s] = di[0] * Mq[s] + di[1] * Mq[3 + s] + di[2] * Mq[6 + s];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const double *dj = &dref[(j * 8 + q) * 3];
+          row[j] += v[0] * dj[0] + v[1] * dj[1] + v[2] * dj[2];
+        }
+      }
+      double fe = 0.0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) fe += s_Jf[wv][el][q] * phi[t * 8 + q];
+      const bool myc = (cm >> t) & 1u;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        double v = row[j];
+        if (myc || ((cm >> j) & 1u)) v = (j == t && myc) ? 1.0 : 0.0;
+        Ke[(e * 8 + t) * 8 + j] = v;
+      }
+      Fe[e * 8 + t] = myc ? 0.0 : fe;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Q2 HEX27, 3x3x3 Gauss, MFMA contraction.  One element per wave, 4 waves per
+// block, persistent.  Ke (27x27, padded 32x32) = Gref^T (32x84) * H (84x32):
+// tiles (0,0), (0,1), (1,1) on v_mfma_*_16x16x4 (21 k-steps each), tile (1,0)
+// is the transpose of (0,1).  MFMA operand layout (16x16x4): lane l supplies
+// A[l%16][l/16] and B[l/16][l%16]; D element r of lane l is D[4*(l/16)+r][l%16].
+// The Gref^T operand is constant and held in registers for the whole kernel.
+// ---------------------------------------------------------------------------
+template <typename CT>
+struct MfmaT;
+template <>
+struct MfmaT<double> {
+  typedef double4_t acc_t;
+  __device__ static inline acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct MfmaT<float> {
+  typedef float4_t acc_t;
+  __device__ static inline acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int Q2N = 27, Q2Q = 27, Q2K = 81, Q2KS = 21;
+
+template <typename CT>
+__global__ __launch_bounds__(256) void k_poisson_hex27_mfma(const double *__restrict__ tab,
+                                                            const double *__restrict__ xyz,
+                                                            const int32_t *__restrict__ conn,
+                                                            const int32_t *__restrict__ edofs,
+                                                            const uint8_t *__restrict__ cons,
+                                                            const double *__restrict__ kappa, int f_kind,
+                                                            int constrain, int64_t ne, CT *__restrict__ Ke,
+                                                            double *__restrict__ Fe, int *__restrict__ err) {
+  typedef typename MfmaT<CT>::acc_t acc_t;
+  __shared__ double s_w[Q2Q];
+  __shared__ double s_phi[Q2N * Q2Q];
+  __shared__ double s_dref[Q2N * Q2Q * 3];
+  __shared__ double s_X[4][Q2N * 3];
+  __shared__ double s_M[4][Q2Q * 9];
+  __shared__ double s_Jf[4][Q2Q];
+  __shared__ CT s_K[4][Q2N * 28];
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  for (int i = tid; i < Q2Q; i += 256) s_w[i] = tab[i];
+  for (int i = tid; i < Q2N * Q2Q; i += 256) s_phi[i] = tab[Q2Q + i];
+  for (int i = tid; i < Q2N * Q2Q * 3; i += 256) s_dref[i] = tab[Q2Q + Q2N * Q2Q + i];
+  __syncthreads();
+
+  // constant A operand: A[I][ks] = Gref^T[16I + l%16][4ks + l/16]
+  CT areg[2][Q2KS];
+  {
+    const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int I = 0; I < 2; ++I)
+#pragma unroll
+      for (int ks = 0; ks < Q2KS; ++ks) {
+        const int i = 16 * I + li, k = 4 * ks + lk;
+        double v = 0.0;
+        if (i < Q2N && k < Q2K) {
+          const int q = k / 3, r = k - 3 * (k / 3);
+          v = s_dref[(i * Q2Q + q) * 3 + r];
+        }
+        areg[I][ks] = (CT)v;
+      }
+  }
+
+  double *X = s_X[wv];
+  double *M = s_M[wv];
+  double *Jf = s_Jf[wv];
+  CT *Ks = s_K[wv];
+  const int64_t groups = (ne + 3) / 4;
+  for (int64_t g = blockIdx.x; g < groups; g += gridDim.x) {
+    const int64_t e = g * 4 + wv;
+    const bool valid = e < ne;
+    bool ci = false;
+    double kap = 1.0;
+    if (valid) {
+      if (lane < Q2N) {
+        const int32_t nd = conn[e * Q2N + lane];
+        X[lane * 3] = xyz[3 * (int64_t)nd];
+        X[lane * 3 + 1] = xyz[3 * (int64_t)nd + 1];
+        X[lane * 3 + 2] = xyz[3 * (int64_t)nd + 2];
+        ci = cons[edofs[e * Q2N + lane]] != 0;
+      }
+      if (kappa) kap = kappa[e];
+    }
+    const unsigned long long cmask = constrain ? __ballot(ci) : 0ull;
+    __syncthreads();
+    if (valid && lane < Q2Q) {
+      double Mq[9];
+      const double jxw = qp_geometry(X, s_dref, Q2N, Q2Q, lane, s_w[lane], kap, Mq);
+      if (jxw <= 0.0) atomicOr(err, 1);
+#pragma unroll
+      for (int t = 0; t < 9; ++t) M[lane * 9 + t] = Mq[t];
+      double f = 1.0;
+      if (f_kind != 0) {
+        double x = 0, y = 0, z = 0;
+        for (int a = 0; a < Q2N; ++a) {
+          const double p = s_phi[a * Q2Q + lane];
+          x += X[a * 3] * p;
+          y += X[a * 3 + 1] * p;
+          z += X[a * 3 + 2] * p;
+        }
+        f = forcing(f_kind, x, y, z);
+      }
+      Jf[lane] = jxw * f;
+    }
+    __syncthreads();
+    if (valid) {
+      acc_t c00 = {0, 0, 0, 0}, c01 = {0, 0, 0, 0}, c11 = {0, 0, 0, 0};
+      const int li = lane & 15, lk = lane >> 4;
+      const int j0 = li, j1 = 16 + li;
+#pragma unroll
+      for (int ks = 0; ks < Q2KS; ++ks) {
+        const int k = 4 * ks + lk;
+        double b0 = 0.0, b1 = 0.0;
+        if (k < Q2K) {
+          const int q = k / 3, r = k - 3 * (k / 3);
+          const double m0 = M[q * 9 + r * 3], m1 = M[q * 9 + r * 3 + 1], m2 = M[q * 9 + r * 3 + 2];
+          const double *d0 = &s_dref[(j0 * Q2Q + q) * 3];
+          b0 = m0 * d0[0] + m1 * d0[1] + m2 * d0[2];
+          if (j1 < Q2N) {
+            const double *d1 = &s_dref[(j1 * Q2Q + q) * 3];
+            b1 = m0 * d1[0] + m1 * d1[1] + m2 * d1[2];
+          }
+        }
+        c00 = MfmaT<CT>::mma(areg[0][ks], (CT)b0, c00);
+        c01 = MfmaT<CT>::mma(areg[0][ks], (CT)b1, c01);
+        c11 = MfmaT<CT>::mma(areg[1][ks], (CT)b1, c11);
+      }
+      // stage in LDS (row stride 28) with the mirrored (1,0) tile
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i0 = 4 * lk + r, i1 = 16 + 4 * lk + r;
+        Ks[i0 * 28 + j0] = c00[r];
+        if (j1 < Q2N) {
+          Ks[i0 * 28 + j1] = c01[r];
+          Ks[j1 * 28 + i0] = c01[r];
+          if (i1 < Q2N) Ks[i1 * 28 + j1] = c11[r];
+        }
+      }
+    }
+    __syncthreads();
+    if (valid) {
+      for (int t = lane; t < Q2N * Q2N; t += 64) {
+        const int i = t / Q2N, j = t - (t / Q2N) * Q2N;
+        CT v = Ks[i * 28 + j];
+        const bool ri = (cmask >> i) & 1ull, cj = (cmask >> j) & 1ull;
+        if (ri || cj) v = (i == j && ri) ? (CT)1 : (CT)0;
+        Ke[e * (Q2N * Q2N) + t] = v;
+      }
+      if (lane < Q2N) {
+        double fe = 0.0;
+        for (int q = 0; q < Q2Q; ++q) fe += Jf[q] * s_phi[lane * Q2Q + q];
+        Fe[e * Q2N + lane] = ((cmask >> lane) & 1ull) ? 0.0 : fe;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Scatter Ke/Fe -> CSR values / rhs.
+// Gather form: one wave per row; the row's slots accumulate in LDS over the
+// row's incidences in ascending (element, local index) order -- the same
+// summation order as the element loop of the reference -- then the row is
+// written once.  Coloured form: one launch per colour, every (element, i, j)
+// adds into its slot; elements of one colour share no DoF.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(64) void k_scatter_gather(const int32_t *__restrict__ inc_ptr,
+                                                       const int32_t *__restrict__ inc, int dpe, int64_t n_rows,
+                                                       const int32_t *__restrict__ rp, const uint16_t *__restrict__ off,
+                                                       const T *__restrict__ Ke, const double *__restrict__ Fe,
+                                                       T *__restrict__ vals, double *__restrict__ rhs) {
+  extern __shared__ unsigned char smraw[];
+  T *acc = (T *)smraw;
+  const int lane = threadIdx.x;
+  for (int64_t r = blockIdx.x; r < n_rows; r += gridDim.x) {
+    const int32_t b = rp[r], len = rp[r + 1] - b;
+    for (int t = lane; t < len; t += 64) acc[t] = (T)0;
+    __syncthreads();
+    const int32_t k0 = inc_ptr[r], k1 = inc_ptr[r + 1];
+    double f = 0.0;
+    for (int32_t k = k0; k < k1; ++k) {
+      const int64_t ei = inc[k];  // e*dpe + i
+      const int64_t base = ei * dpe;
+      for (int j = lane; j < dpe; j += 64) acc[off[base + j]] += Ke[base + j];
+      if (rhs) f += Fe[ei];
+      __syncthreads();
+    }
+    for (int t = lane; t < len; t += 64) vals[b + t] = acc[t];
+    if (rhs && lane == 0) rhs[r] = f;
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__global__ void k_scatter_color(const int32_t *__restrict__ elems, int64_t n, int dpe,
+                                const int32_t *__restrict__ edofs, const int32_t *__restrict__ rp,
+                                const uint16_t *__restrict__ off, const T *__restrict__ Ke,
+                                const double *__restrict__ Fe, T *__restrict__ vals, double *__restrict__ rhs) {
+  const int64_t dd = (int64_t)dpe * dpe;
+  const int64_t total = n * dd;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = t / dd;
+    const int rem = (int)(t - k * dd);
+    const int i = rem / dpe, j = rem - i * dpe;
+    const int64_t e = elems[k];
+    const int32_t r = edofs[e * dpe + i];
+    const int64_t ij = e * dd + rem;
+    vals[rp[r] + off[ij]] += Ke[ij];
+    if (rhs && j == 0) rhs[r] += Fe[e * dpe + i];
+  }
+}
+
+template <typename T>
+__global__ void k_fill(T *__restrict__ p, int64_t n, T v) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) p[t] = v;
+}
+
+// generic add of a batch of element matrices (binary search for the slot),
+// gather form over the batch's own incidence lists; accumulates onto vals
+__global__ __launch_bounds__(64) void k_add_gather_bs(const int32_t *__restrict__ inc_ptr,
+                                                      const int32_t *__restrict__ inc, int n, int64_t n_rows,
+                                                      const int32_t *__restrict__ dofs, const int32_t *__restrict__ rp,
+                                                      const int32_t *__restrict__ ci, const double *__restrict__ Ke,
+                                                      double *__restrict__ vals, int *__restrict__ err) {
+  extern __shared__ double acc[];
+  const int lane = threadIdx.x;
+  for (int64_t r = blockIdx.x; r < n_rows; r += gridDim.x) {
+    const int32_t k0 = inc_ptr[r], k1 = inc_ptr[r + 1];
+    if (k0 == k1) continue;
+    const int32_t b = rp[r], len = rp[r + 1] - b;
+    for (int t = lane; t < len; t += 64) acc[t] = vals[b + t];
+    __syncthreads();
+    for (int32_t k = k0; k < k1; ++k) {
+      const int64_t ei = inc[k];
+      const int64_t e = ei / n;
+      for (int j = lane; j < n; j += 64) {
+        const int32_t c = dofs[e * n + j];
+        int32_t lo = b, hi = b + len;
+        while (lo < hi) {
+          const int32_t mid = (lo + hi) >> 1;
+          if (ci[mid] < c)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        if (lo < b + len && ci[lo] == c)
+          acc[lo - b] += Ke[ei * n + j];
+        else
+          atomicOr(err, 1);
+      }
+      __syncthreads();
+    }
+    for (int t = lane; t < len; t += 64) vals[b + t] = acc[t];
+    __syncthreads();
+  }
+}
+
+__global__ void k_check_range(const int32_t *__restrict__ v, int64_t n, int64_t hi, int *__restrict__ err) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    if (v[t] < 0 || v[t] >= hi) atomicOr(err, 2);
+}
+
+__global__ void k_add_vec_gather(const int32_t *__restrict__ inc_ptr, const int32_t *__restrict__ inc, int64_t n_rows,
+                                 const double *__restrict__ Fe, double *__restrict__ v) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_rows; r += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t k0 = inc_ptr[r], k1 = inc_ptr[r + 1];
+    if (k0 == k1) continue;
+    double s = v[r];
+    for (int32_t k = k0; k < k1; ++k) s += Fe[inc[k]];
+    v[r] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// QoI derivative element vectors: Q_i = sum_{q: x_q in box} JxW_q phi_i(q)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_qoi_box(const double *__restrict__ tab, int n, int nq,
+                                                const double *__restrict__ xyz, const int32_t *__restrict__ conn,
+                                                int64_t ne, double lo, double hi, double *__restrict__ Q,
+                                                int *__restrict__ err) {
+  extern __shared__ double sm[];
+  const int lane = threadIdx.x;
+  double *w = sm, *phi = w + nq, *dref = phi + n * nq, *X = dref + n * nq * 3, *Jc = X + 81;
+  for (int t = lane; t < nq + n * nq * 4; t += 64) sm[t] = tab[t];
+  __syncthreads();
+  for (int64_t e = blockIdx.x; e < ne; e += gridDim.x) {
+    if (lane < n) {
+      const int32_t nd = conn[e * n + lane];
+      X[lane * 3] = xyz[3 * (int64_t)nd];
+      X[lane * 3 + 1] = xyz[3 * (int64_t)nd + 1];
+      X[lane * 3 + 2] = xyz[3 * (int64_t)nd + 2];
+    }
+    __syncthreads();
+    if (lane < nq) {
+      double Mq[9];
+      const double jxw = qp_geometry(X, dref, n, nq, lane, w[lane], 1.0, Mq);
+      if (jxw <= 0.0) atomicOr(err, 1);
+      double x = 0, y = 0, z = 0;
+      for (int a = 0; a < n; ++a) {
+        const double p = phi[a * nq + lane];
+        x += X[a * 3] * p;
+        y += X[a * 3 + 1] * p;
+        z += X[a * 3 + 2] * p;
+      }
+      const bool in = x >= lo && x <= hi && y >= lo && y <= hi && z >= lo && z <= hi;
+      Jc[lane] = in ? jxw : 0.0;
+    }
+    __syncthreads();
+    if (lane < n) {
+      double s = 0.0;
+      for (int q = 0; q < nq; ++q) s += Jc[q] * phi[lane * nq + q];
+      Q[e * n + lane] = s;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_zero_constrained(double *__restrict__ v, const uint8_t *__restrict__ cons, int64_t n) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+    if (cons[r]) v[r] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint backward kernel: dJ/dkappa_e = -lambda_e^T (dKc_e/dkappa_e) u_e.
+// Kc_e is linear in kappa_e on free rows/columns and kappa-independent on
+// constrained ones, and u, lambda vanish on constrained DoFs, so
+// dJ/dkappa_e = -(1/kappa_e) lambda_e^T Ke_e u_e = -sum_q JxW_q grad(lambda).grad(u)
+// with the nodal values masked to the free DoFs.  One wave per element.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_adjoint_kappa(const double *__restrict__ tab, int n, int nq,
+                                                      const double *__restrict__ xyz, const int32_t *__restrict__ conn,
+                                                      const int32_t *__restrict__ edofs,
+                                                      const uint8_t *__restrict__ cons, const double *__restrict__ u,
+                                                      const double *__restrict__ lam, int64_t ne,
+                                                      double *__restrict__ grad, int *__restrict__ err) {
+  extern __shared__ double sm[];
+  const int lane = threadIdx.x;
+  double *w = sm, *phi = w + nq, *dref = phi + n * nq, *X = dref + n * nq * 3, *U = X + 81, *L = U + 27,
+         *part = L + 27;
+  for (int t = lane; t < nq + n * nq * 4; t += 64) sm[t] = tab[t];
+  __syncthreads();
+  for (int64_t e = blockIdx.x; e < ne; e += gridDim.x) {
+    if (lane < n) {
+      const int32_t nd = conn[e * n + lane];
+      X[lane * 3] = xyz[3 * (int64_t)nd];
+      X[lane * 3 + 1] = xyz[3 * (int64_t)nd + 1];
+      X[lane * 3 + 2] = xyz[3 * (int64_t)nd + 2];
+      const int32_t d = edofs[e * n + lane];
+      const bool c = cons[d] != 0;
+      U[lane] = c ? 0.0 : u[d];
+      L[lane] = c ? 0.0 : lam[d];
+    }
+    __syncthreads();
+    double s = 0.0;
+    if (lane < nq) {
+      double Mq[9];
+      const double jxw = qp_geometry(X, dref, n, nq, lane, w[lane], 1.0, Mq);
+      if (jxw <= 0.0) atomicOr(err, 1);
+      double gu[3] = {0, 0, 0}, gl[3] = {0, 0, 0};
+      for (int a = 0; a < n; ++a) {
+        const double *d = &dref[(a * nq + lane) * 3];
+        for (int r = 0; r < 3; ++r) {
+          gu[r] += U[a] * d[r];
+          gl[r] += L[a] * d[r];
+        }
+      }
+      for (int r = 0; r < 3; ++r) s += gl[r] * (Mq[r * 3] * gu[0] + Mq[r * 3 + 1] * gu[1] + Mq[r * 3 + 2] * gu[2]);
+    }
+    part[lane] = s;
+    __syncthreads();
+    if (lane == 0) {
+      double t = 0.0;
+      for (int q = 0; q < nq; ++q) t += part[q];
+      grad[e] = -t;
+    }
+    __syncthreads();
+  }
+}
+
+static int read_flag(lmh_context *ctx, const char *what) {
+  int flag = 0;
+  LMH_HIP(hipMemcpyAsync(&flag, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  LMH_HIP(hipStreamSynchronize(ctx->stream));
+  if (flag) return set_error(LMH_ERR_DEGENERATE, "%s: Jacobian determinant <= 0 (degenerate map)", what);
+  return LMH_OK;
+}
+
+// Element matrices for a scalar Poisson problem into Ke (CT) and Fe (FP64).
+template <typename CT>
+static int element_poisson_impl(lmh_context *ctx, const lmh_mesh *m, const lmh_dofmap *dm, int npts,
+                                const double *kappa, int f_kind, int constrain, CT *Ke, double *Fe) {
+  LMH_CHECK(dm->nv == 1, LMH_ERR_UNSUPPORTED, "Poisson assembly needs one variable");
+  const FETable *t = nullptr;
+  LMH_TRY(fe_table(ctx, m->type, npts, &t));
+  LMH_CHECK(t->nq <= 64 && t->n <= 27, LMH_ERR_UNSUPPORTED, "element too large for the generic kernel");
+  const int64_t ne = m->n_elem;
+  const int maxg = ctx->cu_count * 8;
+  if (m->type == LMH_HEX27 && npts == 3) {
+    const int g = grid_for((ne + 3) / 4, 1, ctx->cu_count * 2);
+    LMH_LAUNCH(ctx, sizeof(CT) == 8 ? "element_hex27_mfma_fp64" : "element_hex27_mfma_fp32",
+               k_poisson_hex27_mfma<CT>, dim3(g), dim3(256), 0, t->d_data, m->xyz, m->conn, dm->edofs, dm->cons, kappa,
+               f_kind, constrain, ne, Ke, Fe, ctx->d_flags);
+    return check_launch("k_poisson_hex27_mfma");
+  }
+  if (m->type == LMH_HEX8 && npts == 2 && sizeof(CT) == 8) {
+    const int g = grid_for((ne + 31) / 32, 1, ctx->cu_count * 4);
+    LMH_LAUNCH(ctx, "element_hex8_fma", k_poisson_hex8, dim3(g), dim3(256), 0, t->d_data, m->xyz, m->conn, dm->edofs,
+               dm->cons, kappa, f_kind, constrain, ne, (double *)Ke, Fe, ctx->d_flags);
+    return check_launch("k_poisson_hex8");
+  }
+  const size_t shm = sizeof(double) * (t->nq + (size_t)t->n * t->nq * 4 + 81 + 9 * t->nq + t->nq);
+  const int g = grid_for(ne, 1, maxg);
+  LMH_LAUNCH(ctx, "element_generic", k_poisson_generic<CT>, dim3(g), dim3(64), shm, t->d_data, t->n, t->nq, m->xyz,
+             m->conn, dm->edofs, dm->cons, kappa, f_kind, constrain, ne, Ke, Fe, ctx->d_flags);
+  return check_launch("k_poisson_generic");
+}
+
+template <typename T>
+static int scatter_impl(lmh_matrix *A, const T *Ke, const double *Fe, T *vals, double *rhs, int scatter) {
+  lmh_context *ctx = A->ctx;
+  const lmh_dofmap *dm = A->dm;
+  if (scatter == LMH_SCATTER_GATHER) {
+    const int g = (int)std::min<int64_t>(A->rows, (int64_t)ctx->cu_count * 64);
+    LMH_LAUNCH(ctx, sizeof(T) == 8 ? "scatter_gather_fp64" : "scatter_gather_fp32", k_scatter_gather<T>, dim3(g),
+               dim3(64), sizeof(T) * A->max_row, dm->inc_ptr, dm->inc, dm->dpe, A->rows, A->rp, A->off, Ke, Fe, vals,
+               rhs);
+    return check_launch("k_scatter_gather");
+  }
+  LMH_CHECK(scatter == LMH_SCATTER_COLORED, LMH_ERR_INVALID, "unknown scatter mode %d", scatter);
+  LMH_LAUNCH(ctx, "fill", k_fill<T>, dim3(grid_for(A->nnz, 256, 8192)), dim3(256), 0, vals, A->nnz, (T)0);
+  if (rhs) LMH_LAUNCH(ctx, "fill", k_fill<double>, dim3(grid_for(A->rows, 256, 8192)), dim3(256), 0, rhs, A->rows, 0.0);
+  for (int c = 0; c < A->ncolors; ++c) {
+    const int64_t n = A->color_ptr[c + 1] - A->color_ptr[c];
+    const int64_t work = n * dm->dpe * dm->dpe;
+    LMH_LAUNCH(ctx, sizeof(T) == 8 ? "scatter_colored_fp64" : "scatter_colored_fp32", k_scatter_color<T>,
+               dim3(grid_for(work, 256, 65536)), dim3(256), 0, A->color_elems + A->color_ptr[c], n, dm->dpe,
+               dm->edofs, A->rp, A->off, Ke, Fe, vals, rhs);
+  }
+  return check_launch("k_scatter_color");
+}
+
+}  // namespace lmh
+
+using namespace lmh;
+
+extern "C" {
+
+int lmh_element_poisson(lmh_context *ctx, const lmh_mesh *m, const lmh_dofmap *dm, int npts, const double *kappa_d,
+                        int f_kind, int constrain, double *Ke_d, double *Fe_d) {
+  LMH_CHECK(ctx && m && dm && Ke_d && Fe_d, LMH_ERR_INVALID, "null argument");
+  LMH_HIP(hipMemsetAsync(ctx->d_flags, 0, sizeof(int), ctx->stream));
+  LMH_TRY(element_poisson_impl<double>(ctx, m, dm, npts, kappa_d, f_kind, constrain, Ke_d, Fe_d));
+  return read_flag(ctx, "lmh_element_poisson");
+}
+
+int lmh_assemble_poisson(lmh_matrix *A, const lmh_mesh *m, const lmh_dofmap *dm, int npts, const double *kappa_d,
+                         int f_kind, double *rhs_d, int precision, int scatter) {
+  LMH_CHECK(A && m && dm && rhs_d, LMH_ERR_INVALID, "null argument");
+  LMH_CHECK(A->dm == dm && dm->mesh == m, LMH_ERR_INVALID, "matrix, dofmap and mesh do not belong together");
+  lmh_context *ctx = A->ctx;
+  const int64_t ne = m->n_elem, dd = (int64_t)dm->dpe * dm->dpe;
+  if (!A->Fe) LMH_HIP(hipMalloc((void **)&A->Fe, sizeof(double) * ne * dm->dpe));
+  LMH_HIP(hipMemsetAsync(ctx->d_flags, 0, sizeof(int), ctx->stream));
+  if (precision == LMH_FP64) {
+    if (!A->Ke) LMH_HIP(hipMalloc((void **)&A->Ke, sizeof(double) * ne * dd));
+    LMH_TRY(element_poisson_impl<double>(ctx, m, dm, npts, kappa_d, f_kind, 1, A->Ke, A->Fe));
+    LMH_TRY(scatter_impl<double>(A, A->Ke, A->Fe, A->vals, rhs_d, scatter));
+    A->have32 = false;
+  } else {
+    LMH_CHECK(precision == LMH_FP32, LMH_ERR_INVALID, "unknown precision %d", precision);
+    if (!A->Ke32) LMH_HIP(hipMalloc((void **)&A->Ke32, sizeof(float) * ne * dd));
+    if (!A->vals32) LMH_HIP(hipMalloc((void **)&A->vals32, sizeof(float) * A->nnz));
+    LMH_TRY(element_poisson_impl<float>(ctx, m, dm, npts, kappa_d, f_kind, 1, A->Ke32, A->Fe));
+    LMH_TRY(scatter_impl<float>(A, A->Ke32, A->Fe, A->vals32, rhs_d, scatter));
+    A->have32 = true;
+  }
+  return read_flag(ctx, "lmh_assemble_poisson");
+}
+
+int lmh_matrix_add_element_matrices(lmh_matrix *A, int64_t n_elem, int n, const double *Ke_d, const int32_t *dofs_d) {
+  LMH_CHECK(A && Ke_d && dofs_d && n > 0, LMH_ERR_INVALID, "bad argument");
+  if (n_elem == 0) return LMH_OK;
+  lmh_context *ctx = A->ctx;
+  LMH_HIP(hipMemsetAsync(ctx->d_flags, 0, sizeof(int), ctx->stream));
+  LMH_LAUNCH(ctx, "check_range", k_check_range, dim3(grid_for(n_elem * n, 256, 8192)), dim3(256), 0, dofs_d,
+             n_elem * n, A->rows, ctx->d_flags);
+  {
+    int bad = 0;
+    LMH_HIP(hipMemcpyAsync(&bad, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    LMH_HIP(hipStreamSynchronize(ctx->stream));
+    LMH_CHECK(!bad, LMH_ERR_INVALID, "add_matrix: dof index out of range");
+  }
+  int32_t *ip = nullptr, *lst = nullptr;
+  LMH_TRY(build_incidence(ctx, dofs_d, n_elem, n, A->rows, &ip, &lst));
+  const int g = (int)std::min<int64_t>(A->rows, (int64_t)ctx->cu_count * 64);
+  LMH_LAUNCH(ctx, "add_element_matrices", k_add_gather_bs, dim3(g), dim3(64), sizeof(double) * A->max_row, ip, lst, n,
+             A->rows, dofs_d, A->rp, A->ci, Ke_d, A->vals, ctx->d_flags);
+  LMH_TRY(check_launch("k_add_gather_bs"));
+  int bad = 0;
+  LMH_HIP(hipMemcpyAsync(&bad, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  LMH_HIP(hipStreamSynchronize(ctx->stream));
+  LMH_HIP(hipFree(ip));
+  LMH_HIP(hipFree(lst));
+  A->have32 = false;
+  LMH_CHECK(!bad, LMH_ERR_INVALID, "add_matrix: entry outside the sparsity pattern");
+  return LMH_OK;
+}
+
+int lmh_vector_add_element_vectors(lmh_context *ctx, double *v_d, int64_t n_elem, int n, const double *Fe_d,
+                                   const int32_t *dofs_d) {
+  LMH_CHECK(ctx && v_d && Fe_d && dofs_d && n > 0, LMH_ERR_INVALID, "bad argument");
+  if (n_elem == 0) return LMH_OK;
+  // rows = max dof + 1 is unknown here; use a host scan of the dof range
+  std::vector<int32_t> d((size_t)n_elem * n);
+  LMH_HIP(hipMemcpyAsync(d.data(), dofs_d, sizeof(int32_t) * d.size(), hipMemcpyDeviceToHost, ctx->stream));
+  LMH_HIP(hipStreamSynchronize(ctx->stream));
+  int64_t rows = 0;
+  for (int32_t x : d) {
+    LMH_CHECK(x >= 0, LMH_ERR_INVALID, "negative dof index");
+    rows = std::max<int64_t>(rows, (int64_t)x + 1);
+  }
+  int32_t *ip = nullptr, *lst = nullptr;
+  LMH_TRY(build_incidence(ctx, dofs_d, n_elem, n, rows, &ip, &lst));
+  LMH_LAUNCH(ctx, "add_element_vectors", k_add_vec_gather, dim3(grid_for(rows, 256, 8192)), dim3(256), 0, ip, lst,
+             rows, Fe_d, v_d);
+  LMH_TRY(check_launch("k_add_vec_gather"));
+  LMH_HIP(hipStreamSynchronize(ctx->stream));
+  LMH_HIP(hipFree(ip));
+  LMH_HIP(hipFree(lst));
+  return LMH_OK;
+}
+
+int lmh_qoi_box_derivative(const lmh_mesh *m, const lmh_dofmap *dm, int npts, double lo, double hi, double *dq_d) {
+  LMH_CHECK(m && dm && dq_d, LMH_ERR_INVALID, "null argument");
+  LMH_CHECK(dm->nv == 1, LMH_ERR_UNSUPPORTED, "QoI derivative needs one variable");
+  lmh_context *ctx = m->ctx;
+  const FETable *t = nullptr;
+  LMH_TRY(fe_table(ctx, m->type, npts, &t));
+  double *Q = nullptr;
+  LMH_HIP(hipMallocAsync((void **)&Q, sizeof(double) * m->n_elem * t->n, ctx->stream));
+  LMH_HIP(hipMemsetAsync(ctx->d_flags, 0, sizeof(int), ctx->stream));
+  const size_t shm = sizeof(double) * (t->nq + (size_t)t->n * t->nq * 4 + 81 + t->nq);
+  LMH_LAUNCH(ctx, "qoi_box", k_qoi_box, dim3(grid_for(m->n_elem, 1, ctx->cu_count * 8)), dim3(64), shm, t->d_data,
+             t->n, t->nq, m->xyz, m->conn, m->n_elem, lo, hi, Q, ctx->d_flags);
+  LMH_TRY(check_launch("k_qoi_box"));
+  LMH_LAUNCH(ctx, "fill", k_fill<double>, dim3(grid_for(dm->n_dofs, 256, 8192)), dim3(256), 0, dq_d, dm->n_dofs, 0.0);
+  LMH_LAUNCH(ctx, "add_element_vectors", k_add_vec_gather, dim3(grid_for(dm->n_dofs, 256, 8192)), dim3(256), 0,
+             dm->inc_ptr, dm->inc, dm->n_dofs, (const double *)Q, dq_d);
+  LMH_LAUNCH(ctx, "zero_constrained", k_zero_constrained, dim3(grid_for(dm->n_dofs, 256, 8192)), dim3(256), 0, dq_d,
+             dm->cons, dm->n_dofs);
+  LMH_TRY(check_launch("qoi"));
+  LMH_HIP(hipFreeAsync(Q, ctx->stream));
+  return read_flag(ctx, "lmh_qoi_box_derivative");
+}
+
+int lmh_adjoint_kappa_gradient(const lmh_mesh *m, const lmh_dofmap *dm, int npts, const double *kappa_d,
+                               const double *u_d, const double *lambda_d, double *grad_d) {
+  (void)kappa_d;  // dJ/dkappa_e does not depend on kappa_e for a per-element constant kappa
+  LMH_CHECK(m && dm && u_d && lambda_d && grad_d, LMH_ERR_INVALID, "null argument");
+  LMH_CHECK(dm->nv == 1, LMH_ERR_UNSUPPORTED, "adjoint gradient needs one variable");
+  lmh_context *ctx = m->ctx;
+  const FETable *t = nullptr;
+  LMH_TRY(fe_table(ctx, m->type, npts, &t));
+  LMH_CHECK(t->nq <= 64 && t->n <= 27, LMH_ERR_UNSUPPORTED, "element too large");
+  LMH_HIP(hipMemsetAsync(ctx->d_flags, 0, sizeof(int), ctx->stream));
+  const size_t shm = sizeof(double) * (t->nq + (size_t)t->n * t->nq * 4 + 81 + 27 + 27 + 64);
+  LMH_LAUNCH(ctx, "adjoint_kappa", k_adjoint_kappa, dim3(grid_for(m->n_elem, 1, ctx->cu_count * 8)), dim3(64), shm,
+             t->d_data, t->n, t->nq, m->xyz, m->conn, dm->edofs, dm->cons, u_d, lambda_d, m->n_elem, grad_d,
+             ctx->d_flags);
+  LMH_TRY(check_launch("k_adjoint_kappa"));
+  return read_flag(ctx, "lmh_adjoint_kappa_gradient");
+}
+
+}  // extern "C"

@@ -1,0 +1,157 @@
+// lmh_internal.h -- shared internals of liblmhip (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/lmhip.h"
+
+namespace lmh {
+
+// ---- errors ---------------------------------------------------------------
+int set_error(int code, const char *fmt, ...);
+
+#define LMH_HIP(call)                                                                                  \
+  do {                                                                                                 \
+    hipError_t _e = (call);                                                                            \
+    if (_e != hipSuccess)                                                                              \
+      return ::lmh::set_error(LMH_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(_e),     \
+                              __FILE__, __LINE__);                                                     \
+  } while (0)
+
+#define LMH_CHECK(cond, code, ...)                                                                     \
+  do {                                                                                                 \
+    if (!(cond)) return ::lmh::set_error(code, __VA_ARGS__);                                           \
+  } while (0)
+
+#define LMH_TRY(call)                                                                                  \
+  do {                                                                                                 \
+    int _rc = (call);                                                                                  \
+    if (_rc != LMH_OK) return _rc;                                                                     \
+  } while (0)
+
+// ---- FE reference tables (QGauss + Lagrange shapes) -------------------------
+// Layout (all FP64): w[nq], phi[n*nq] (phi[a*nq+q]), dref[n*nq*3]
+// (dref[(a*nq+q)*3+r]).
+struct FETable {
+  int type = 0, n = 0, nq = 0, arg = 0;
+  std::vector<double> w, phi, dref, qp;
+  double *d_data = nullptr;  // device copy: w | phi | dref
+  const double *d_w() const { return d_data; }
+  const double *d_phi() const { return d_data + nq; }
+  const double *d_dref() const { return d_data + nq + (size_t)n * nq; }
+};
+// Build the table for an element type; arg = points per direction (hex) or
+// QGauss order (tet).  Returns false for unsupported combinations.
+bool build_fe_table(int type, int arg, FETable &t);
+int nodes_per_elem(int type);
+
+// ---- profiling: HIP events around each tracked launch on the context stream ---
+struct ProfEntry {
+  double total_ms = 0;
+  int64_t count = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+
+}  // namespace lmh
+
+struct lmh_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::map<std::pair<int, int>, std::unique_ptr<lmh::FETable>> tables;
+  bool prof_on = false;
+  std::map<std::string, lmh::ProfEntry> prof;
+  std::vector<hipEvent_t> event_pool;
+  // small device scratch for reductions / solver scalars
+  double *d_scal = nullptr;     // 64 doubles
+  double *d_part = nullptr;     // reduction partials
+  unsigned *d_count = nullptr;  // last-block counters (zeroed, self-resetting)
+  int *d_flags = nullptr;       // error flags
+  double *h_pinned = nullptr;   // pinned host scalars
+  int n_part = 0;
+  int cu_count = 256;
+};
+
+struct lmh_mesh {
+  lmh_context *ctx = nullptr;
+  int type = 0, npe = 0;
+  int64_t n_nodes = 0, n_elem = 0;
+  double *xyz = nullptr;     // n_nodes*3
+  int32_t *conn = nullptr;   // n_elem*npe
+};
+
+struct lmh_dofmap {
+  lmh_context *ctx = nullptr;
+  const lmh_mesh *mesh = nullptr;
+  int nv = 1, dpe = 0;
+  int64_t n_dofs = 0;
+  int32_t *edofs = nullptr;   // n_elem*dpe
+  uint8_t *cons = nullptr;    // n_dofs
+  // incidence: dof -> sorted list of (e*dpe + i)
+  int32_t *inc_ptr = nullptr;  // n_dofs+1
+  int32_t *inc = nullptr;      // n_elem*dpe
+};
+
+struct lmh_matrix {
+  lmh_context *ctx = nullptr;
+  const lmh_dofmap *dm = nullptr;
+  int64_t rows = 0, nnz = 0;
+  int max_row = 0;
+  int32_t *rp = nullptr, *ci = nullptr;
+  double *vals = nullptr;
+  float *vals32 = nullptr;
+  bool have32 = false;
+  uint16_t *off = nullptr;  // n_elem*dpe*dpe slot offsets within rows
+  int ncolors = 0;
+  std::vector<int64_t> color_ptr;  // host, ncolors+1
+  int32_t *color_elems = nullptr;  // device, n_elem (grouped by colour, ascending id)
+  // element buffers
+  double *Ke = nullptr, *Fe = nullptr;
+  float *Ke32 = nullptr;
+  // solver work vectors (allocated on first solve)
+  double *w64 = nullptr;  // 6*rows
+  float *w32 = nullptr;   // 6*rows
+};
+
+namespace lmh {
+
+struct ProfScope {
+  lmh_context *ctx;
+  const char *name;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(lmh_context *c, const char *n);
+  ~ProfScope();
+};
+
+hipEvent_t get_event(lmh_context *ctx);
+
+// launch helper: records HIP events on the context stream when profiling is on
+#define LMH_LAUNCH(ctx, name, kernel, grid, block, shmem, ...)                                          \
+  do {                                                                                                 \
+    ::lmh::ProfScope _ps((ctx), (name));                                                               \
+    hipLaunchKernelGGL(kernel, grid, block, shmem, (ctx)->stream, __VA_ARGS__);                        \
+  } while (0)
+
+int check_launch(const char *what);
+
+// device helpers shared across translation units
+int exclusive_scan_i32(lmh_context *ctx, const int32_t *in, int32_t *out, int64_t n, int64_t *total);
+int build_incidence(lmh_context *ctx, const int32_t *edofs, int64_t n_elem, int dpe, int64_t n_dofs,
+                    int32_t **inc_ptr, int32_t **inc);
+int fe_table(lmh_context *ctx, int type, int arg, const FETable **out);
+
+// grid for an XCD-aware 1-D launch
+inline int grid_for(int64_t work, int per_block, int max_blocks) {
+  int64_t g = (work + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > max_blocks) g = max_blocks;
+  return (int)g;
+}
+
+}  // namespace lmh

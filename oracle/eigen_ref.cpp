@@ -1,0 +1,40 @@
+// eigen_ref.cpp -- drives the reference's own vendored Eigen 3.3.9 CG with
+// DiagonalPreconditioner, exactly as EigenSparseLinearSolver::solve does for
+// CG + JACOBI_PRECOND (src/solvers/eigen_sparse_linear_solver.C:106-160:
+// ConjugateGradient<EigenSM, Lower|Upper, DiagonalPreconditioner<Number>>,
+// setMaxIterations, setTolerance, solveWithGuess).  EigenSM is a RowMajor
+// SparseMatrix<Number, RowMajor, eigen_idx_type> (include/numerics/eigen_core_support.h).
+//
+// TEST INFRASTRUCTURE ONLY: built by oracle/Makefile against the headers under
+// the reference tree into oracle/_ref/, and used to pin the oracle's CG
+// restatement and as the "reference" CPU baseline.
+#include <Eigen/IterativeLinearSolvers>
+#include <Eigen/Sparse>
+#include <cstdint>
+#include <cstring>
+
+typedef Eigen::SparseMatrix<double, Eigen::RowMajor, int> EigenSM;
+
+extern "C" int ref_eigen_cg(int64_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *b,
+                            double *x, double tol, int64_t maxits, int64_t *iters, double *err) {
+  Eigen::Map<const EigenSM> view((Eigen::Index)n, (Eigen::Index)n, (Eigen::Index)rp[n], rp, ci, v);
+  EigenSM A(view);
+  Eigen::Map<const Eigen::VectorXd> bb(b, n);
+  Eigen::VectorXd xx = Eigen::Map<const Eigen::VectorXd>(x, n);
+  Eigen::ConjugateGradient<EigenSM, Eigen::Lower | Eigen::Upper, Eigen::DiagonalPreconditioner<double>> solver(A);
+  solver.setMaxIterations((Eigen::Index)maxits);
+  solver.setTolerance(tol);
+  xx = solver.solveWithGuess(bb, xx);
+  std::memcpy(x, xx.data(), sizeof(double) * n);
+  *iters = solver.iterations();
+  *err = solver.error();
+  return (int)solver.info();
+}
+
+extern "C" void ref_eigen_spmv(int64_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *x,
+                               double *y) {
+  Eigen::Map<const EigenSM> A((Eigen::Index)n, (Eigen::Index)n, (Eigen::Index)rp[n], rp, ci, v);
+  Eigen::Map<const Eigen::VectorXd> xx(x, n);
+  Eigen::Map<Eigen::VectorXd> yy(y, n);
+  yy.noalias() = A * xx;
+}

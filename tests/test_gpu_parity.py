@@ -1,0 +1,226 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bars: integer/index work (sparsity pattern, colour classes) bit-exact; FP64
+matrix entries and right-hand sides within 1e-12 normwise relative
+(max|A - A_ref| / max|A_ref|); constrained diagonals exact integers; CG to the
+requested relative residual with the oracle's iteration count +-2.
+"""
+import numpy as np
+import pytest
+
+import libmesh_amd as L
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SEED_PERTURB = 0x5EED0001
+SEED_KAPPA = 0x5EED0002
+NPTS = {L.HEX8: 2, L.HEX27: 3, L.TET4: 3, L.TET10: 5}  # hex: points/direction, tet: QGauss order
+
+
+def make(n, t, nv=1, mask=L.ALL_FACES, perturb=False):
+    p = L.Problem(n, t, nv, mask)
+    m = O.Mesh(n, t)
+    if perturb:
+        p.perturb(SEED_PERTURB, 0.2)
+        m.perturb(SEED_PERTURB, 0.2)
+    s = O.System(m, nv, mask)
+    return p, m, s
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("t,n,nv", [(L.HEX8, 4, 1), (L.HEX27, 3, 1), (L.HEX27, 5, 1), (L.TET4, 2, 1),
+                                    (L.TET10, 2, 1), (L.HEX8, 3, 3), (L.HEX27, 2, 3), (L.TET10, 2, 3)])
+def test_sparsity_bit_exact(ctx, t, n, nv):
+    p, m, s = make(n, t, nv)
+    g = L.GpuProblem.from_problem(ctx, p)
+    rp, ci = s.pattern()
+    grp, gci = g.pattern()
+    assert g.nnz == s.nnz
+    assert np.array_equal(grp, rp)
+    assert np.array_equal(gci, ci)
+
+
+def test_sparsity_q2_closed_form(ctx):
+    n = 8
+    p = L.Problem(n, L.HEX27)
+    g = L.GpuProblem.from_problem(ctx, p)
+    assert g.nnz == (8 * n + 1) ** 3
+    assert g.n_colors == 8  # structured hex lattice -> parity colouring
+
+
+@pytest.mark.parametrize("t,n", [(L.HEX8, 4), (L.HEX27, 3), (L.TET4, 2), (L.TET10, 2)])
+@pytest.mark.parametrize("perturb", [False, True])
+def test_element_matrices(ctx, t, n, perturb):
+    if perturb and t in (L.TET4, L.TET10):
+        pytest.skip("tet meshes are not perturbed in any configuration")
+    p, m, s = make(n, t, perturb=perturb)
+    kappa = L.lognormal(p.n_elem, SEED_KAPPA)
+    npts = NPTS[t]
+    oK, oF = s.element_poisson(npts, kappa, 1, constrain=True)
+    g = L.GpuProblem.from_problem(ctx, p, build_matrix=False)
+    kd = L.DeviceArray(ctx, p.n_elem, data=kappa)
+    Kd = L.DeviceArray(ctx, p.n_elem * p.dpe * p.dpe)
+    Fd = L.DeviceArray(ctx, p.n_elem * p.dpe)
+    g.element_poisson(npts, kd, 1, Kd, Fd, constrain=True)
+    K = Kd.download().reshape(oK.shape)
+    F = Fd.download().reshape(oF.shape)
+    assert rel(K, oK) < 1e-12
+    assert rel(F, oF) < 1e-12
+
+
+@pytest.mark.parametrize("scatter", [L.SCATTER_GATHER, L.SCATTER_COLORED])
+@pytest.mark.parametrize("t,n,perturb,f_kind", [(L.HEX8, 16, False, 1), (L.HEX27, 6, True, 0), (L.TET10, 2, False, 1)])
+def test_assembly(ctx, scatter, t, n, perturb, f_kind):
+    p, m, s = make(n, t, perturb=perturb)
+    kappa = L.lognormal(p.n_elem, SEED_KAPPA)
+    ov, orhs = s.assemble_poisson(NPTS[t], kappa, f_kind)
+    g = L.GpuProblem.from_problem(ctx, p)
+    kd = L.DeviceArray(ctx, p.n_elem, data=kappa)
+    b = L.DeviceArray(ctx, p.n_dofs)
+    g.assemble_poisson(NPTS[t], kd, f_kind, b, L.FP64, scatter)
+    v = g.values()
+    rhs = b.download()
+    assert rel(v, ov) < 1e-12
+    assert rel(rhs, orhs) < 1e-12
+    # constrained rows: exact integer diagonal (number of incident elements), zero off-diagonals
+    rp, ci = s.pattern()
+    cons = np.flatnonzero(p.constrained)
+    diag_pos = np.array([rp[r] + np.searchsorted(ci[rp[r]:rp[r + 1]], r) for r in cons])
+    assert np.array_equal(v[diag_pos], ov[diag_pos])
+    assert np.all(v[diag_pos] == np.round(v[diag_pos]))
+    assert np.all(rhs[cons] == 0.0)
+
+
+def test_scatter_bit_exact_given_same_element_matrices(ctx):
+    """With the oracle's own element matrices, the gather scatter sums every
+    entry in the reference's element order, so the CSR values are bit-identical."""
+    p, m, s = make(4, L.HEX27, perturb=True)
+    kappa = L.lognormal(p.n_elem, SEED_KAPPA)
+    oK, oF = s.element_poisson(3, kappa, 0, constrain=True)
+    ov, orhs = s.assemble_poisson(3, kappa, 0)
+    g = L.GpuProblem.from_problem(ctx, p)
+    Kd = L.DeviceArray(ctx, oK.size, data=oK.ravel())
+    Dd = L.DeviceArray(ctx, p.elem_dofs.size, np.int32, data=p.elem_dofs.ravel())
+    L.check(L.hip().lmh_matrix_zero(g.matrix))
+    L.check(L.hip().lmh_matrix_add_element_matrices(g.matrix, p.n_elem, p.dpe, Kd.ptr, Dd.ptr))
+    assert np.array_equal(g.values(), ov)
+    Fd = L.DeviceArray(ctx, oF.size, data=oF.ravel())
+    r = L.DeviceArray(ctx, p.n_dofs)
+    r.zero()
+    L.check(L.hip().lmh_vector_add_element_vectors(ctx.h, r.ptr, p.n_elem, p.dpe, Fd.ptr, Dd.ptr))
+    assert np.array_equal(r.download(), orhs)
+
+
+def test_spmv(ctx):
+    p, m, s = make(5, L.HEX27, perturb=True)
+    kappa = L.lognormal(p.n_elem, SEED_KAPPA)
+    ov, orhs = s.assemble_poisson(3, kappa, 0)
+    rp, ci = s.pattern()
+    g = L.GpuProblem.from_problem(ctx, p)
+    g.set_values(ov)
+    xh = np.random.default_rng(1).standard_normal(p.n_dofs)
+    x = L.DeviceArray(ctx, p.n_dofs, data=xh)
+    y = L.DeviceArray(ctx, p.n_dofs)
+    g.spmv(x, y)
+    yo = O.spmv(rp, ci, ov, xh)
+    assert rel(y.download(), yo) < 1e-13
+
+
+@pytest.mark.parametrize("n", [4, 8])
+def test_cg_c1_recipe(ctx, n):
+    p, m, s = make(n, L.HEX27, perturb=True)
+    kappa = L.lognormal(p.n_elem, SEED_KAPPA)
+    g = L.GpuProblem.from_problem(ctx, p)
+    kd = L.DeviceArray(ctx, p.n_elem, data=kappa)
+    b = L.DeviceArray(ctx, p.n_dofs)
+    x = L.DeviceArray(ctx, p.n_dofs)
+    x.zero()
+    g.assemble_poisson(3, kd, 0, b)
+    its, res, reason = g.cg(b, x, 1e-10, 5000)
+    ov, orhs = s.assemble_poisson(3, kappa, 0)
+    rp, ci = s.pattern()
+    ox, oits, oerr = O.cg_eigen(rp, ci, ov, orhs, 1e-10, 5000)
+    xs = x.download()
+    true_res = np.linalg.norm(orhs - O.spmv(rp, ci, ov, xs)) / np.linalg.norm(orhs)
+    assert reason == L.CONVERGED_RTOL
+    assert abs(its - oits) <= 2
+    assert res < 1e-10 and true_res < 1.05e-10
+    tight, _, _ = O.cg_eigen(rp, ci, ov, orhs, 1e-14, 20000)
+    assert np.linalg.norm(xs - tight) / np.linalg.norm(tight) < 1e-7
+
+
+def test_cg_c0_manufactured(ctx):
+    """16^3 HEX8 Q1, f = 3 pi^2 sin sin sin: the rhs is a discrete eigenvector, so
+    Jacobi-CG converges in one step (Eigen reports 0 iterations) and the nodal
+    error against sin(pi x) sin(pi y) sin(pi z) is the O(h^2) discretisation error."""
+    p, m, s = make(16, L.HEX8)
+    g = L.GpuProblem.from_problem(ctx, p)
+    b = L.DeviceArray(ctx, p.n_dofs)
+    x = L.DeviceArray(ctx, p.n_dofs)
+    x.zero()
+    g.assemble_poisson(2, None, 1, b)
+    its, res, reason = g.cg(b, x, 1e-12, 5000)
+    ov, orhs = s.assemble_poisson(2, None, 1)
+    rp, ci = s.pattern()
+    ox, oits, _ = O.cg_eigen(rp, ci, ov, orhs, 1e-12, 5000)
+    xs = x.download()
+    assert its == oits
+    assert np.abs(xs - ox).max() < 1e-12
+    X = p.coords
+    ue = np.sin(np.pi * X[:, 0]) * np.sin(np.pi * X[:, 1]) * np.sin(np.pi * X[:, 2])
+    assert np.abs(xs - ue).max() < 0.0066
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_mixed_precision(ctx, mode):
+    p, m, s = make(8, L.HEX27, perturb=True)
+    kappa = L.lognormal(p.n_elem, SEED_KAPPA)
+    g = L.GpuProblem.from_problem(ctx, p)
+    kd = L.DeviceArray(ctx, p.n_elem, data=kappa)
+    b = L.DeviceArray(ctx, p.n_dofs)
+    g.assemble_poisson(3, kd, 0, b)
+    g.make_fp32()
+    x = L.DeviceArray(ctx, p.n_dofs)
+    x.zero()
+    inner = 0.1 if mode == 0 else 1e-5
+    its, res, reason, stats = g.mixed(b, x, 1e-10, inner, mode)
+    assert reason == L.CONVERGED_RTOL, (its, res, reason, stats)
+    ov, orhs = s.assemble_poisson(3, kappa, 0)
+    rp, ci = s.pattern()
+    xs = x.download()
+    true_res = np.linalg.norm(orhs - O.spmv(rp, ci, ov, xs)) / np.linalg.norm(orhs)
+    assert true_res <= 1e-10
+    tight, _, _ = O.cg_eigen(rp, ci, ov, orhs, 1e-14, 20000)
+    assert np.linalg.norm(xs - tight) / np.linalg.norm(tight) < 1e-7
+
+
+def test_adjoint_gradient(ctx):
+    n = 4
+    p, m, s = make(n, L.HEX27, perturb=True)
+    kappa = L.lognormal(p.n_elem, SEED_KAPPA)
+    g = L.GpuProblem.from_problem(ctx, p)
+    kd = L.DeviceArray(ctx, p.n_elem, data=kappa)
+    b = L.DeviceArray(ctx, p.n_dofs)
+    u = L.DeviceArray(ctx, p.n_dofs)
+    lam = L.DeviceArray(ctx, p.n_dofs)
+    dq = L.DeviceArray(ctx, p.n_dofs)
+    grad = L.DeviceArray(ctx, p.n_elem)
+    u.zero()
+    lam.zero()
+    g.assemble_poisson(3, kd, 0, b)
+    g.cg(b, u, 1e-13, 5000)
+    g.qoi_box_derivative(3, 0.25, 0.75, dq)
+    assert rel(dq.download(), s.qoi_box_derivative(3)) < 1e-12
+    g.cg(dq, lam, 1e-13, 5000)
+    g.adjoint_kappa_gradient(3, kd, u, lam, grad)
+    gr = grad.download()
+    uh, lh = u.download(), lam.download()
+    ex = s.adjoint_exact(3, kappa, uh, lh)
+    assert rel(gr, ex) < 1e-10
+    elems = np.arange(0, p.n_elem, 7)
+    fd = s.adjoint_sensitivity(3, kappa, uh, lh, elems)
+    assert np.abs(gr[elems] - fd).max() / np.abs(fd).max() < 1e-6
