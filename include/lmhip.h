@@ -110,6 +110,8 @@ int lmh_matrix_zero(lmh_matrix *A);
 /* Values in/out, FP64, nnz entries in pattern order. */
 int lmh_matrix_get_values(const lmh_matrix *A, double *vals_h);
 int lmh_matrix_set_values(lmh_matrix *A, const double *vals_h);
+/* FP32 operator values (after FP32 assembly or lmh_matrix_make_fp32). */
+int lmh_matrix_get_values_fp32(const lmh_matrix *A, float *vals_h);
 /* SparseMatrix::add_matrix(DenseMatrix, dofs) for a batch of dense element
  * matrices held on the device (eigen_sparse_matrix.C:121-135; laspack_matrix.C:626-640):
  * Ke_d: n_elem*n*n row-major, dofs_d: n_elem*n.  Entries must exist in the

@@ -69,6 +69,7 @@ HIP_API = {
     "lmh_matrix_zero": (_I, [_P]),
     "lmh_matrix_get_values": (_I, [_P, _P]),
     "lmh_matrix_set_values": (_I, [_P, _P]),
+    "lmh_matrix_get_values_fp32": (_I, [_P, _P]),
     "lmh_matrix_add_element_matrices": (_I, [_P, _I64, _I, _P, _P]),
     "lmh_vector_add_element_vectors": (_I, [_P, _P, _I64, _I, _P, _P]),
     "lmh_assemble_poisson": (_I, [_P, _P, _P, _I, _P, _I, _P, _I, _I]),
@@ -306,6 +307,11 @@ class GpuProblem:
     def values(self):
         v = np.empty(self.nnz)
         check(hip().lmh_matrix_get_values(self.matrix, _ptr(v)))
+        return v
+
+    def values_fp32(self):
+        v = np.empty(self.nnz, dtype=np.float32)
+        check(hip().lmh_matrix_get_values_fp32(self.matrix, _ptr(v)))
         return v
 
     def set_values(self, v):

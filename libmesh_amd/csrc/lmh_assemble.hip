@@ -28,9 +28,10 @@ __device__ inline double forcing(int f_kind, double x, double y, double z) {
 
 // Geometry at one quadrature point from LDS coordinates X[a*3+c] and the
 // reference derivative table dref[(a*nq+q)*3+r].  Writes M (9, symmetric,
-// scaled by JxW*kappa) and returns JxW (<= 0 signals a degenerate map).
+// scaled by JxW*kappa) and JxW (which is negative for the negative-weight tet
+// rule), and returns det J (<= 0 signals a degenerate map, fe_map.C:467-505).
 __device__ inline double qp_geometry(const double *X, const double *dref, int n, int nq, int q, double w,
-                                     double kap, double *M) {
+                                     double kap, double *M, double &jxw_out) {
   double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // J[r][c] = d x_c / d xi_r
   for (int a = 0; a < n; ++a) {
     const double *d = &dref[(a * nq + q) * 3];
@@ -49,6 +50,7 @@ __device__ inline double qp_geometry(const double *X, const double *dref, int n,
   const double det = (dx_dxi * (dy_deta * dz_dzeta - dz_deta * dy_dzeta) +
                       dy_dxi * (dz_deta * dx_dzeta - dx_deta * dz_dzeta) +
                       dz_dxi * (dx_deta * dy_dzeta - dy_deta * dx_dzeta));
+  jxw_out = 0.0;
   if (!(det > 0.0)) return -1.0;
   const double inv = 1.0 / det;
   double G[3][3];  // G[r][c] = d xi_r / d x_c
@@ -71,7 +73,8 @@ __device__ inline double qp_geometry(const double *X, const double *dref, int n,
       M[r * 3 + t] = v;
       M[t * 3 + r] = v;
     }
-  return jxw;
+  jxw_out = jxw;
+  return det;
 }
 
 // ---------------------------------------------------------------------------
@@ -108,8 +111,8 @@ __global__ __launch_bounds__(64) void k_poisson_generic(const double *__restrict
     __syncthreads();
     if (lane < nq) {
       double Mq[9];
-      const double jxw = qp_geometry(X, dref, n, nq, lane, w[lane], kap, Mq);
-      if (jxw <= 0.0) atomicOr(err, 1);
+      double jxw;
+      if (qp_geometry(X, dref, n, nq, lane, w[lane], kap, Mq, jxw) <= 0.0) atomicOr(err, 1);
       for (int t = 0; t < 9; ++t) M[lane * 9 + t] = Mq[t];
       double f = 1.0;
       if (f_kind != 0) {
@@ -191,8 +194,8 @@ __global__ __launch_bounds__(256) void k_poisson_hex8(const double *__restrict__
     __syncthreads();
     if (valid) {
       double Mq[9];
-      const double jxw = qp_geometry(X, dref, 8, 8, t, w[t], kap, Mq);
-      if (jxw <= 0.0) atomicOr(err, 1);
+      double jxw;
+      if (qp_geometry(X, dref, 8, 8, t, w[t], kap, Mq, jxw) <= 0.0) atomicOr(err, 1);
       for (int k = 0; k < 9; ++k) s_M[wv][el][t * 9 + k] = Mq[k];
       double f = 1.0;
       if (f_kind != 0) {
@@ -245,7 +248,8 @@ __global__ __launch_bounds__(256) void k_poisson_hex8(const double *__restrict__
 // block, persistent.  Ke (27x27, padded 32x32) = Gref^T (32x84) * H (84x32):
 // tiles (0,0), (0,1), (1,1) on v_mfma_*_16x16x4 (21 k-steps each), tile (1,0)
 // is the transpose of (0,1).  MFMA operand layout (16x16x4): lane l supplies
-// A[l%16][l/16] and B[l/16][l%16]; D element r of lane l is D[4*(l/16)+r][l%16].
+// A[l%16][l/16] and B[l/16][l%16]; accumulator element r of lane l holds
+// D[drow(l/16, r)][l%16] (see MfmaT::drow).
 // The Gref^T operand is constant and held in registers for the whole kernel.
 // ---------------------------------------------------------------------------
 template <typename CT>
@@ -256,6 +260,8 @@ struct MfmaT<double> {
   __device__ static inline acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // f64 16x16x4: the four accumulator registers of lane group g hold rows g, g+4, g+8, g+12
+  __device__ static inline int drow(int g, int r) { return g + 4 * r; }
 };
 template <>
 struct MfmaT<float> {
@@ -263,6 +269,8 @@ struct MfmaT<float> {
   __device__ static inline acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  // f32 16x16x4: lane group g holds rows 4g .. 4g+3
+  __device__ static inline int drow(int g, int r) { return 4 * g + r; }
 };
 
 constexpr int Q2N = 27, Q2Q = 27, Q2K = 81, Q2KS = 21;
@@ -332,8 +340,8 @@ __global__ __launch_bounds__(256) void k_poisson_hex27_mfma(const double *__rest
     __syncthreads();
     if (valid && lane < Q2Q) {
       double Mq[9];
-      const double jxw = qp_geometry(X, s_dref, Q2N, Q2Q, lane, s_w[lane], kap, Mq);
-      if (jxw <= 0.0) atomicOr(err, 1);
+      double jxw;
+      if (qp_geometry(X, s_dref, Q2N, Q2Q, lane, s_w[lane], kap, Mq, jxw) <= 0.0) atomicOr(err, 1);
 #pragma unroll
       for (int t = 0; t < 9; ++t) M[lane * 9 + t] = Mq[t];
       double f = 1.0;
@@ -375,7 +383,7 @@ __global__ __launch_bounds__(256) void k_poisson_hex27_mfma(const double *__rest
       // stage in LDS (row stride 28) with the mirrored (1,0) tile
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i0 = 4 * lk + r, i1 = 16 + 4 * lk + r;
+        const int i0 = MfmaT<CT>::drow(lk, r), i1 = 16 + i0;
         Ks[i0 * 28 + j0] = c00[r];
         if (j1 < Q2N) {
           Ks[i0 * 28 + j1] = c01[r];
@@ -541,8 +549,8 @@ __global__ __launch_bounds__(64) void k_qoi_box(const double *__restrict__ tab, 
     __syncthreads();
     if (lane < nq) {
       double Mq[9];
-      const double jxw = qp_geometry(X, dref, n, nq, lane, w[lane], 1.0, Mq);
-      if (jxw <= 0.0) atomicOr(err, 1);
+      double jxw;
+      if (qp_geometry(X, dref, n, nq, lane, w[lane], 1.0, Mq, jxw) <= 0.0) atomicOr(err, 1);
       double x = 0, y = 0, z = 0;
       for (int a = 0; a < n; ++a) {
         const double p = phi[a * nq + lane];
@@ -602,8 +610,8 @@ __global__ __launch_bounds__(64) void k_adjoint_kappa(const double *__restrict__
     double s = 0.0;
     if (lane < nq) {
       double Mq[9];
-      const double jxw = qp_geometry(X, dref, n, nq, lane, w[lane], 1.0, Mq);
-      if (jxw <= 0.0) atomicOr(err, 1);
+      double jxw;
+      if (qp_geometry(X, dref, n, nq, lane, w[lane], 1.0, Mq, jxw) <= 0.0) atomicOr(err, 1);
       double gu[3] = {0, 0, 0}, gl[3] = {0, 0, 0};
       for (int a = 0; a < n; ++a) {
         const double *d = &dref[(a * nq + lane) * 3];
@@ -723,6 +731,14 @@ int lmh_assemble_poisson(lmh_matrix *A, const lmh_mesh *m, const lmh_dofmap *dm,
     A->have32 = true;
   }
   return read_flag(ctx, "lmh_assemble_poisson");
+}
+
+int lmh_matrix_get_values_fp32(const lmh_matrix *A, float *vals_h) {
+  LMH_CHECK(A && vals_h, LMH_ERR_INVALID, "null argument");
+  LMH_CHECK(A->have32 && A->vals32, LMH_ERR_INVALID, "matrix has no FP32 operator");
+  LMH_HIP(hipMemcpyAsync(vals_h, A->vals32, sizeof(float) * A->nnz, hipMemcpyDeviceToHost, A->ctx->stream));
+  LMH_HIP(hipStreamSynchronize(A->ctx->stream));
+  return LMH_OK;
 }
 
 int lmh_matrix_add_element_matrices(lmh_matrix *A, int64_t n_elem, int n, const double *Ke_d, const int32_t *dofs_d) {
