@@ -1,0 +1,235 @@
+"""Benchmark: assembly + Jacobi-CG to rtol 1e-10 of the variable-coefficient Q2
+Poisson problem on one MI355X, plain FP64 path and mixed-precision path.
+
+Workload (one "step"): 64^3 HEX27 cube, Q2 Lagrange, 3x3x3 Gauss, 2,146,689
+DoFs, kappa_e = exp(N(0,1)) per element, interior nodes perturbed by +-0.2 h
+(h = node spacing), f = 1, u = 0 on all faces.  A step zeroes the solution,
+assembles the operator and right-hand side on the GPU (geometry, element
+contraction, constraints, CSR scatter) and solves to |b - A x| / |b| <= 1e-10.
+Mesh generation, DoF numbering and the GPU sparsity pattern are setup
+(EquationSystems::init), done once before timing.
+
+value = DoFs / (t_assemble + t_solve) / 1e6 of the mixed-precision path
+(FP32 inner CG with FP64 reliable-update refinement); the FP64 path is timed the
+same way and reported beside it.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mdof/s assembly + CG time to 1e-10 (mixed vs FP64), Q2 hex 2.1M dof, 1 GPU"
+SEED_PERTURB = 0x5EED0001
+SEED_KAPPA = 0x5EED0002
+HBM_PEAK_GBS = 8000.0
+
+
+def spmv_bytes(nnz, n, val_bytes):
+    """Algorithmic bytes of one CSR SpMV: values + int32 columns + row_ptr + x read + y write."""
+    return (val_bytes + 4) * nnz + 4 * (n + 1) + 2 * val_bytes * n
+
+
+def cpu_baseline(n_sample):
+    """The oracle (C++ restatement of libMesh's FP64 path) on host cores, on a
+    smaller instance of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    m = O.Mesh(n_sample, O.HEX27)
+    m.perturb(SEED_PERTURB, 0.2)
+    kappa = O.kappa_lognormal(m.n_elem, SEED_KAPPA)
+    s = O.System(m)
+    t1 = time.perf_counter()
+    vals, rhs = s.assemble_poisson(3, kappa, 0)
+    t2 = time.perf_counter()
+    rp, ci = s.pattern()
+    x, its, err = O.cg_eigen(rp, ci, vals, rhs, 1e-10, 20000)
+    t3 = time.perf_counter()
+    return {
+        "value": s.n_dofs / ((t2 - t1) + (t3 - t2)) / 1e6,
+        "unit": "Mdof/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{n_sample}^3 HEX27 ({s.n_dofs} dofs, same recipe), oracle assembly (serial element loop) "
+                  f"{t2 - t1:.2f}s + Eigen-restated Jacobi-CG {its} its {t3 - t2:.2f}s (OpenMP SpMV), setup "
+                  f"{t1 - t0:.2f}s excluded",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=64, help="elements per cube edge")
+    ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--inner", type=float, default=0.1, help="reliable-update trigger of the mixed solver")
+    ap.add_argument("--scatter", choices=["gather", "colored"], default="gather")
+    ap.add_argument("--cpu-sample", type=int, default=40, help="cube edge of the CPU baseline sample (0: skip)")
+    ap.add_argument("--quiet", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    import libmesh_amd as L
+
+    log = (lambda *a: None) if args.quiet else (lambda *a: print(*a, file=sys.stderr, flush=True))
+    scatter = L.SCATTER_GATHER if args.scatter == "gather" else L.SCATTER_COLORED
+
+    t0 = time.perf_counter()
+    p = L.Problem(args.n, L.HEX27)
+    p.perturb(SEED_PERTURB, 0.2)
+    kappa = L.lognormal(p.n_elem, SEED_KAPPA)
+    ctx = L.Context(0)
+    g = L.GpuProblem.from_problem(ctx, p)
+    kd = L.DeviceArray(ctx, p.n_elem, data=kappa)
+    b = L.DeviceArray(ctx, p.n_dofs)
+    x = L.DeviceArray(ctx, p.n_dofs)
+    ctx.sync()
+    t_setup = time.perf_counter() - t0
+    N, nnz = p.n_dofs, g.nnz
+    log(f"setup {t_setup:.2f}s: dofs={N} nnz={nnz} elements={p.n_elem} colours={g.n_colors}")
+
+    def step(mixed):
+        x.zero()
+        ctx.sync()
+        ta = time.perf_counter()
+        g.assemble_poisson(3, kd, 0, b, L.FP64, scatter)
+        if mixed:
+            g.make_fp32()
+        ctx.sync()
+        tb = time.perf_counter()
+        if mixed:
+            its, res, reason, stats = g.mixed(b, x, args.rtol, args.inner, 0, 20000)
+            nref = int(stats[1])
+        else:
+            its, res, reason = g.cg(b, x, args.rtol, 20000)
+            nref = 0
+        ctx.sync()
+        tc = time.perf_counter()
+        if reason != L.CONVERGED_RTOL:
+            raise RuntimeError(f"solve did not converge: its={its} res={res} reason={reason}")
+        return tb - ta, tc - tb, its, res, nref
+
+    results = {}
+    for mode in ("fp64", "mixed"):
+        mixed = mode == "mixed"
+        for _ in range(args.warmup):
+            step(mixed)
+        ctx.profile_reset()
+        ctx.profile(True)
+        ctx.sync()
+        T0 = time.perf_counter()
+        rs = [step(mixed) for _ in range(args.steps)]
+        ctx.sync()
+        T1 = time.perf_counter()
+        ctx.profile(False)
+        kname = "mx_spmv_fp32" if mixed else "cg_spmv_fp64"
+        kavg, kcnt = ctx.profile_query(kname)
+        ta = float(np.mean([r[0] for r in rs]))
+        ts = float(np.mean([r[1] for r in rs]))
+        results[mode] = {
+            "ms_per_step": (T1 - T0) / args.steps * 1e3,
+            "t_assemble_ms": ta * 1e3,
+            "t_solve_ms": ts * 1e3,
+            "iterations": rs[-1][2],
+            "refreshes": rs[-1][4],
+            "final_rel_residual": rs[-1][3],
+            "mdofs": N / (ta + ts) / 1e6,
+            "spmv_kernel": kname,
+            "spmv_avg_ms": kavg,
+            "spmv_launches": kcnt,
+            "x": x.download() if True else None,
+        }
+        log(f"{mode}: asm {ta*1e3:.2f} ms, solve {ts*1e3:.1f} ms, its {rs[-1][2]}, refreshes {rs[-1][4]}, "
+            f"res {rs[-1][3]:.2e}, {N/(ta+ts)/1e6:.2f} Mdof/s, {kname} avg {kavg*1e3:.1f} us x{kcnt}")
+
+    # assembly kernel timings (FP64 path, one profiled assembly)
+    ctx.profile_reset()
+    ctx.profile(True)
+    g.assemble_poisson(3, kd, 0, b, L.FP64, scatter)
+    ctx.sync()
+    ctx.profile(False)
+    asm_k = {}
+    for k in ("element_hex27_mfma_fp64", "scatter_gather_fp64", "scatter_colored_fp64"):
+        ms, c = ctx.profile_query(k)
+        if c:
+            asm_k[k] = ms * c
+    x64 = results["fp64"].pop("x")
+    xmx = results["mixed"].pop("x")
+    err_mixed_vs_fp64 = float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))
+
+    mx = results["mixed"]
+    alg = spmv_bytes(nnz, N, 4)
+    achieved = alg / (mx["spmv_avg_ms"] * 1e-3) / 1e9
+    alg64 = spmv_bytes(nnz, N, 8)
+    ach64 = alg64 / (results["fp64"]["spmv_avg_ms"] * 1e-3) / 1e9
+    flops_contract = 27 * 27 * 27 * 3 * 2 * p.n_elem
+    out = {
+        "metric": METRIC,
+        "value": round(mx["mdofs"], 4),
+        "unit": "Mdof/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mx["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.n}^3 HEX27 Q2 Poisson -div(kappa grad u)=1, kappa=exp(N(0,1)) per element, "
+                        f"interior nodes perturbed +-0.2h, u=0 on boundary, 3x3x3 Gauss, Jacobi-CG to rtol "
+                        f"{args.rtol:g}; mixed = FP32 inner CG with FP64 reliable-update refinement",
+            "dofs": N,
+            "nnz": nnz,
+            "elements": p.n_elem,
+            "scatter": args.scatter,
+            "inner_trigger": args.inner,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": mx["spmv_kernel"],
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "algorithmic_bytes": alg,
+            "avg_launch_ms": round(mx["spmv_avg_ms"], 5),
+        },
+        "fp64": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in results["fp64"].items()},
+        "mixed": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in results["mixed"].items()},
+        "speedup_mixed_vs_fp64": round(mx["mdofs"] / results["fp64"]["mdofs"], 4),
+        "mixed_rel_diff_vs_fp64_solution": err_mixed_vs_fp64,
+        "fp64_spmv_roofline": {"achieved": round(ach64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": round(ach64 / HBM_PEAK_GBS, 4), "algorithmic_bytes": alg64},
+        "assembly_kernels_ms": {k: round(v, 4) for k, v in asm_k.items()},
+        "contraction_gflop": flops_contract / 1e9,
+        "setup_s": round(t_setup, 3),
+    }
+    if "element_hex27_mfma_fp64" in asm_k:
+        tf = flops_contract / (asm_k["element_hex27_mfma_fp64"] * 1e-3) / 1e12
+        out["contraction_tflops_fp64"] = round(tf, 3)
+        out["contraction_frac_of_fp64_peak"] = round(tf / 78.6, 4)
+    if args.cpu_sample > 0:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_sample)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

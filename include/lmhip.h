@@ -142,6 +142,8 @@ int lmh_element_poisson(lmh_context *ctx, const lmh_mesh *m, const lmh_dofmap *d
 /* ---- linear algebra ---- */
 /* y = A x (SparseMatrix::vector_mult, sparse_matrix.C:237-253). */
 int lmh_spmv(lmh_matrix *A, const double *x_d, double *y_d);
+/* y = A32 x on the FP32 operator (after FP32 assembly or lmh_matrix_make_fp32). */
+int lmh_spmv_fp32(lmh_matrix *A, const float *x_d, float *y_d);
 /* r = b - A x in FP64 (LinearImplicitSystem::assembly residual, linear_implicit_system.C:367-381). */
 int lmh_residual(lmh_matrix *A, const double *b_d, const double *x_d, double *r_d);
 double lmh_norm2(lmh_context *ctx, const double *x_d, int64_t n);

@@ -75,6 +75,7 @@ HIP_API = {
     "lmh_assemble_poisson": (_I, [_P, _P, _P, _I, _P, _I, _P, _I, _I]),
     "lmh_element_poisson": (_I, [_P, _P, _P, _I, _P, _I, _I, _P, _P]),
     "lmh_spmv": (_I, [_P, _P, _P]),
+    "lmh_spmv_fp32": (_I, [_P, _P, _P]),
     "lmh_residual": (_I, [_P, _P, _P, _P]),
     "lmh_norm2": (_D, [_P, _P, _I64]),
     "lmh_cg_solve": (_I, [_P, _P, _P, _D, _I64, C.POINTER(_I64), C.POINTER(_D), C.POINTER(_I)]),

@@ -104,6 +104,8 @@ struct lmh_matrix {
   int64_t rows = 0, nnz = 0;
   int max_row = 0;
   int32_t *rp = nullptr, *ci = nullptr;
+  int32_t *rb = nullptr;  // SpMV row-block boundaries (nrb+1 rows)
+  int64_t nrb = 0;
   double *vals = nullptr;
   float *vals32 = nullptr;
   bool have32 = false;
@@ -115,8 +117,8 @@ struct lmh_matrix {
   double *Ke = nullptr, *Fe = nullptr;
   float *Ke32 = nullptr;
   // solver work vectors (allocated on first solve)
-  double *w64 = nullptr;  // 6*rows
-  float *w32 = nullptr;   // 6*rows
+  double *w64 = nullptr;  // 4*rows
+  float *w32 = nullptr;   // 5*rows
 };
 
 namespace lmh {

@@ -21,11 +21,12 @@ enum {
   S_BB = 0, S_THR, S_ABSNEW, S_ABSOLD, S_PAP, S_ALPHA, S_BETA, S_RR, S_ITS, S_DONE, S_REASON,
   S_MAXRR = 16, S_REFRESH, S_RRTRUE, S_NREF, S_DELTA2, S_MODE, S_RZ
 };
-// counter slots in ctx->d_count
-enum { C_SPMV = 0, C_UPD, C_RES, C_AUX };
+
+
 
 constexpr int BLK = 256;
 constexpr int NXCD = 8;
+constexpr int RB_CAP = 2048;  // stored entries per SpMV row block
 
 // Blocks are dispatched round-robin over the XCDs; remap so each XCD owns one
 // contiguous range of rows (its L2 then holds the x entries its rows gather).
@@ -69,98 +70,135 @@ __device__ inline void block_sum(double (&v)[NV], double (*sh)[BLK / 64]) {
   }
 }
 
-// Publish NV block partials; returns true in every thread of the last block,
-// whose thread 0 then holds the grid totals in v (summed in block order).
+// Publish the block's NV partial sums to part[k*gridDim.x + blockIdx.x].  A
+// one-block finalize kernel launched next on the same stream reduces them in
+// block order (deterministic); the kernel boundary makes the partials visible
+// without any device-scope fence inside the producer.
 template <int NV>
-__device__ inline bool grid_finish(double (&v)[NV], double *part, unsigned *count) {
+__device__ inline void publish(double (&v)[NV], double *__restrict__ part) {
   __shared__ double sh[NV][BLK / 64];
-  __shared__ bool last;
   block_sum<NV>(v, sh);
-  const int G = gridDim.x;
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) part[k * G + blockIdx.x] = v[k];
-    __threadfence();
-    const unsigned prev = atomicAdd(count, 1u);
-    last = (prev == (unsigned)G - 1);
+    for (int k = 0; k < NV; ++k) part[k * gridDim.x + blockIdx.x] = v[k];
   }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  double t[NV];
+}
+
+// In a one-block kernel: totals of G published partials; valid in thread 0.
+template <int NV>
+__device__ inline void reduce_partials(const double *__restrict__ part, int G, double (&t)[NV]) {
+  __shared__ double sh[NV][BLK / 64];
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double s = 0;
     for (int b = threadIdx.x; b < G; b += BLK) s += part[k * G + b];
     t[k] = s;
   }
-  __syncthreads();
   block_sum<NV>(t, sh);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] = t[k];
-    *count = 0;
-  }
-  return true;
 }
 
-// y_r = sum_k v_k x_{c_k} over row r with L lanes per row
-template <int L, typename VT, typename XT, typename AT>
-__device__ inline AT row_dot(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                             const VT *__restrict__ v, const XT *__restrict__ x, int64_t r, int sub) {
-  AT s = 0;
-  const int32_t b = rp[r], e = rp[r + 1];
-  for (int32_t k = b + sub; k < e; k += L) s += (AT)v[k] * (AT)x[ci[k]];
+// Row-block ("CSR stream") SpMV core.  Rows are grouped into blocks holding at
+// most RB_CAP stored entries; a workgroup streams a block's values and column
+// indices with coalesced, independent loads (RB_CAP/BLK per thread), gathers x,
+// writes the products to LDS, then 8 lanes per row sum each row's products in a
+// fixed order.  A single row longer than RB_CAP is one block, reduced directly.
+// epi(row, sum) consumes each row result in the calling thread.
+template <typename VT, typename XT, typename AT, typename Epi>
+__device__ inline void rowblock_spmv(const int32_t *__restrict__ rb, int64_t nrb, const int32_t *__restrict__ rp,
+                                     const int32_t *__restrict__ ci, const VT *__restrict__ v,
+                                     const XT *__restrict__ x, Epi &&epi) {
+  __shared__ AT prod[RB_CAP];
+  __shared__ double red[BLK / 64];
+  constexpr int PER = RB_CAP / BLK;
+  int64_t lo, hi;
+  block_range(nrb, lo, hi);
+  const int tid = threadIdx.x;
+  for (int64_t bk = lo; bk < hi; ++bk) {
+    const int32_t r0 = rb[bk], r1 = rb[bk + 1];
+    const int32_t k0 = rp[r0], cnt = rp[r1] - k0;
+    if (cnt > RB_CAP) {
+      AT s = 0;
+      for (int32_t t = tid; t < cnt; t += BLK)
+        s += (AT)__builtin_nontemporal_load(&v[k0 + t]) * (AT)x[__builtin_nontemporal_load(&ci[k0 + t])];
+      double w = wave_sum((double)s);
+      if ((tid & 63) == 0) red[tid >> 6] = w;
+      __syncthreads();
+      if (tid == 0) {
+        double t2 = 0;
+        for (int k = 0; k < BLK / 64; ++k) t2 += red[k];
+        epi(r0, (AT)t2);
+      }
+      __syncthreads();
+      continue;
+    }
+    // branch-free loads (out-of-range slots re-read entry k0 and are dropped)
+    // so all PER value/index loads, then all PER gathers, are in flight at once
+    VT vv[PER];
+    int32_t cc[PER];
+    AT xx[PER];
 #pragma unroll
-  for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, L);
-  return s;
+    for (int j = 0; j < PER; ++j) {
+      const int32_t t = tid + j * BLK;
+      const int32_t k = k0 + (t < cnt ? t : 0);
+      vv[j] = __builtin_nontemporal_load(&v[k]);
+      cc[j] = __builtin_nontemporal_load(&ci[k]);
+    }
+#pragma unroll
+    for (int j = 0; j < PER; ++j) xx[j] = (AT)x[cc[j]];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int32_t t = tid + j * BLK;
+      if (t < cnt) prod[t] = (AT)vv[j] * xx[j];
+    }
+    __syncthreads();
+    const int g = tid >> 3, sub = tid & 7;
+    for (int32_t r = r0 + g; r < r1; r += BLK / 8) {
+      const int32_t a = rp[r] - k0, e = rp[r + 1] - k0;
+      AT s = 0;
+      for (int32_t t = a + sub; t < e; t += 8) s += prod[t];
+      s += __shfl_xor(s, 4, 8);
+      s += __shfl_xor(s, 2, 8);
+      s += __shfl_xor(s, 1, 8);
+      if (sub == 0) epi(r, s);
+    }
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
 // plain SpMV y = A x
 // ---------------------------------------------------------------------------
-template <int L, typename VT, typename XT, typename YT, typename AT>
-__global__ __launch_bounds__(BLK) void k_spmv(int64_t n, const int32_t *__restrict__ rp,
-                                              const int32_t *__restrict__ ci, const VT *__restrict__ v,
-                                              const XT *__restrict__ x, YT *__restrict__ y) {
-  int64_t lo, hi;
-  block_range(n, lo, hi);
-  const int g = threadIdx.x / L, sub = threadIdx.x % L;
-  for (int64_t r0 = lo; r0 < hi; r0 += BLK / L) {
-    const int64_t r = r0 + g;
-    if (r < hi) {
-      const AT s = row_dot<L, VT, XT, AT>(rp, ci, v, x, r, sub);
-      if (sub == 0) y[r] = (YT)s;
-    }
-  }
+template <typename VT, typename XT, typename YT, typename AT>
+__global__ __launch_bounds__(BLK) void k_spmv(const int32_t *__restrict__ rb, int64_t nrb,
+                                              const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                              const VT *__restrict__ v, const XT *__restrict__ x,
+                                              YT *__restrict__ y) {
+  rowblock_spmv<VT, XT, AT>(rb, nrb, rp, ci, v, x, [&](int32_t r, AT s) { y[r] = (YT)s; });
 }
 
-// r = b - A x (FP64), optionally accumulating |r|^2 and |b|^2
-template <int L>
-__global__ __launch_bounds__(BLK) void k_residual(int64_t n, const int32_t *__restrict__ rp,
-                                                  const int32_t *__restrict__ ci, const double *__restrict__ v,
-                                                  const double *__restrict__ b, const double *__restrict__ x,
-                                                  double *__restrict__ r, double *__restrict__ part,
-                                                  unsigned *__restrict__ count, double *__restrict__ S) {
-  int64_t lo, hi;
-  block_range(n, lo, hi);
-  const int g = threadIdx.x / L, sub = threadIdx.x % L;
+
+// r = b - A x (FP64); partials |r|^2, |b|^2
+__global__ __launch_bounds__(BLK) void k_residual(const int32_t *__restrict__ rb, int64_t nrb,
+                                                  const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                  const double *__restrict__ v, const double *__restrict__ b,
+                                                  const double *__restrict__ x, double *__restrict__ r,
+                                                  double *__restrict__ part) {
   double acc[2] = {0, 0};
-  for (int64_t r0 = lo; r0 < hi; r0 += BLK / L) {
-    const int64_t row = r0 + g;
-    if (row < hi) {
-      const double s = row_dot<L, double, double, double>(rp, ci, v, x, row, sub);
-      if (sub == 0) {
-        const double bv = b[row], rv = bv - s;
-        r[row] = rv;
-        acc[0] += rv * rv;
-        acc[1] += bv * bv;
-      }
-    }
-  }
-  if (grid_finish<2>(acc, part, count) && threadIdx.x == 0) {
-    S[S_RR] = acc[0];
-    S[S_BB] = acc[1];
+  rowblock_spmv<double, double, double>(rb, nrb, rp, ci, v, x, [&](int32_t row, double s) {
+    const double bv = b[row], rv = bv - s;
+    r[row] = rv;
+    acc[0] += rv * rv;
+    acc[1] += bv * bv;
+  });
+  publish<2>(acc, part);
+}
+
+__global__ __launch_bounds__(BLK) void k_fin_residual(const double *__restrict__ part, int G, double *__restrict__ S) {
+  double t[2];
+  reduce_partials<2>(part, G, t);
+  if (threadIdx.x == 0) {
+    S[S_RR] = t[0];
+    S[S_BB] = t[1];
   }
 }
 
@@ -187,12 +225,11 @@ __global__ void k_invdiag(int64_t n, const int32_t *__restrict__ rp, const int32
 // ---------------------------------------------------------------------------
 // FP64 CG
 // ---------------------------------------------------------------------------
-// initial: r = b - A x already computed with |r|^2 (S_RR) and |b|^2 (S_BB).
-// p = dinv r, absNew = r.p ; decide the early exits of ConjugateGradient.h:44-60
-__global__ __launch_bounds__(BLK) void k_cg_init(int64_t n, double tol, const double *__restrict__ dinv,
+// after k_residual + k_fin_residual: p = dinv r (x = p = 0 when b = 0); partial r.p
+__global__ __launch_bounds__(BLK) void k_cg_init(int64_t n, const double *__restrict__ dinv,
                                                  const double *__restrict__ r, double *__restrict__ p,
                                                  double *__restrict__ x, double *__restrict__ part,
-                                                 unsigned *__restrict__ count, double *__restrict__ S) {
+                                                 const double *__restrict__ S) {
   int64_t lo, hi;
   block_range(n, lo, hi);
   const bool zero_rhs = S[S_BB] == 0.0;
@@ -207,11 +244,19 @@ __global__ __launch_bounds__(BLK) void k_cg_init(int64_t n, double tol, const do
       acc[0] += r[i] * z;
     }
   }
-  if (grid_finish<1>(acc, part, count) && threadIdx.x == 0) {
+  publish<1>(acc, part);
+}
+
+// the early exits of ConjugateGradient.h:44-60
+__global__ __launch_bounds__(BLK) void k_fin_cg_init(const double *__restrict__ part, int G, double tol,
+                                                     double *__restrict__ S) {
+  double t[1];
+  reduce_partials<1>(part, G, t);
+  if (threadIdx.x == 0) {
     const double bb = S[S_BB];
     const double thr = fmax(tol * tol * bb, 2.2250738585072014e-308);
     S[S_THR] = thr;
-    S[S_ABSNEW] = acc[0];
+    S[S_ABSNEW] = t[0];
     S[S_ITS] = 0;
     S[S_DONE] = 0;
     S[S_REASON] = 0;
@@ -226,31 +271,32 @@ __global__ __launch_bounds__(BLK) void k_cg_init(int64_t n, double tol, const do
   }
 }
 
-template <int L>
-__global__ __launch_bounds__(BLK) void k_cg_spmv(int64_t n, const int32_t *__restrict__ rp,
-                                                 const int32_t *__restrict__ ci, const double *__restrict__ v,
-                                                 const double *__restrict__ p, double *__restrict__ q,
-                                                 double *__restrict__ part, unsigned *__restrict__ count,
-                                                 double *__restrict__ S) {
+// q = A p; partial p.q  (template VT/AT: FP64 for CG, FP32 for the mixed inner solver)
+template <typename VT, typename AT>
+__global__ __launch_bounds__(BLK) void k_spmv_dot(const int32_t *__restrict__ rb, int64_t nrb,
+                                                  const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                  const VT *__restrict__ v, const VT *__restrict__ p,
+                                                  VT *__restrict__ q, double *__restrict__ part,
+                                                  const double *__restrict__ S) {
   if (S[S_DONE] != 0.0) return;
-  int64_t lo, hi;
-  block_range(n, lo, hi);
-  const int g = threadIdx.x / L, sub = threadIdx.x % L;
   double acc[1] = {0};
-  for (int64_t r0 = lo; r0 < hi; r0 += BLK / L) {
-    const int64_t r = r0 + g;
-    if (r < hi) {
-      const double s = row_dot<L, double, double, double>(rp, ci, v, p, r, sub);
-      if (sub == 0) {
-        q[r] = s;
-        acc[0] += p[r] * s;
-      }
-    }
-  }
-  if (grid_finish<1>(acc, part, count) && threadIdx.x == 0) {
-    const double pap = acc[0];
+  rowblock_spmv<VT, VT, AT>(rb, nrb, rp, ci, v, p, [&](int32_t r, AT s) {
+    q[r] = (VT)s;
+    acc[0] += (double)p[r] * (double)s;
+  });
+  publish<1>(acc, part);
+}
+
+// alpha = (r,z) / (p,Ap); slot holds (r,z): S_ABSNEW for CG, S_RZ for the mixed solver
+__global__ __launch_bounds__(BLK) void k_fin_alpha(const double *__restrict__ part, int G, int rz_slot,
+                                                   double *__restrict__ S) {
+  if (S[S_DONE] != 0.0) return;
+  double t[1];
+  reduce_partials<1>(part, G, t);
+  if (threadIdx.x == 0) {
+    const double pap = t[0];
     S[S_PAP] = pap;
-    const double alpha = S[S_ABSNEW] / pap;
+    const double alpha = S[rz_slot] / pap;
     S[S_ALPHA] = alpha;
     if (!(pap != 0.0) || !isfinite(alpha)) {
       S[S_DONE] = 1;
@@ -262,8 +308,7 @@ __global__ __launch_bounds__(BLK) void k_cg_spmv(int64_t n, const int32_t *__res
 __global__ __launch_bounds__(BLK) void k_cg_update(int64_t n, const double *__restrict__ dinv,
                                                    const double *__restrict__ p, const double *__restrict__ q,
                                                    double *__restrict__ x, double *__restrict__ r,
-                                                   double *__restrict__ part, unsigned *__restrict__ count,
-                                                   double *__restrict__ S) {
+                                                   double *__restrict__ part, const double *__restrict__ S) {
   if (S[S_DONE] != 0.0) return;
   int64_t lo, hi;
   block_range(n, lo, hi);
@@ -276,19 +321,27 @@ __global__ __launch_bounds__(BLK) void k_cg_update(int64_t n, const double *__re
     acc[0] += rv * rv;
     acc[1] += rv * (dinv[i] * rv);
   }
-  if (grid_finish<2>(acc, part, count) && threadIdx.x == 0) {
-    S[S_RR] = acc[0];
-    if (!isfinite(acc[0])) {
+  publish<2>(acc, part);
+}
+
+// residualNorm2 test, then absOld/absNew/beta and i++ (ConjugateGradient.h:76-87)
+__global__ __launch_bounds__(BLK) void k_fin_cg_beta(const double *__restrict__ part, int G, double *__restrict__ S) {
+  if (S[S_DONE] != 0.0) return;
+  double t[2];
+  reduce_partials<2>(part, G, t);
+  if (threadIdx.x == 0) {
+    S[S_RR] = t[0];
+    if (!isfinite(t[0])) {
       S[S_DONE] = 1;
       S[S_REASON] = LMH_DIVERGED_NANORINF;
-    } else if (acc[0] < S[S_THR]) {
+    } else if (t[0] < S[S_THR]) {
       S[S_DONE] = 1;
       S[S_REASON] = LMH_CONVERGED_RTOL;
     } else {
       const double absOld = S[S_ABSNEW];
       S[S_ABSOLD] = absOld;
-      S[S_ABSNEW] = acc[1];
-      S[S_BETA] = acc[1] / absOld;
+      S[S_ABSNEW] = t[1];
+      S[S_BETA] = t[1] / absOld;
       S[S_ITS] += 1;
     }
   }
@@ -308,109 +361,73 @@ __global__ __launch_bounds__(BLK) void k_cg_pupdate(int64_t n, const double *__r
 // Mixed precision.  FP32 recurrence (x_s, r_s, p_s, q_s) on the FP32 operator;
 // FP64 solution x and true residual r = b - A x on the FP64 operator.
 // ---------------------------------------------------------------------------
-// r64 = b - A64 x with |r|^2, |b|^2; r32 = (float) r64; p32 = dinv32 r32 (fresh
-// start) ; rz = r32.p32.  Used for the initial residual and for every refresh.
-template <int L>
-__global__ __launch_bounds__(BLK) void k_mx_residual(int64_t n, int init, const int32_t *__restrict__ rp,
-                                                     const int32_t *__restrict__ ci, const double *__restrict__ v,
-                                                     const double *__restrict__ b, const double *__restrict__ x,
-                                                     const float *__restrict__ dinv32, float *__restrict__ r32,
-                                                     float *__restrict__ p32, double *__restrict__ part,
-                                                     unsigned *__restrict__ count, double *__restrict__ S) {
+// r64 = b - A64 x; r32 = (float) r64; z = dinv32 r32 (p32 = z on the initial
+// call); partials |r|^2, |b|^2, (r32, z).  Used initially and on every refresh.
+__global__ __launch_bounds__(BLK) void k_mx_residual(int init, const int32_t *__restrict__ rb, int64_t nrb,
+                                                     const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                     const double *__restrict__ v, const double *__restrict__ b,
+                                                     const double *__restrict__ x, const float *__restrict__ dinv32,
+                                                     float *__restrict__ r32, float *__restrict__ p32,
+                                                     double *__restrict__ part, const double *__restrict__ S) {
   if (!init && (S[S_DONE] != 0.0 || S[S_REFRESH] == 0.0)) return;
-  int64_t lo, hi;
-  block_range(n, lo, hi);
-  const int g = threadIdx.x / L, sub = threadIdx.x % L;
   double acc[3] = {0, 0, 0};
-  for (int64_t r0 = lo; r0 < hi; r0 += BLK / L) {
-    const int64_t row = r0 + g;
-    if (row < hi) {
-      const double s = row_dot<L, double, double, double>(rp, ci, v, x, row, sub);
-      if (sub == 0) {
-        const double bv = b[row], rv = bv - s;
-        const float rf = (float)rv;
-        r32[row] = rf;
-        const float z = dinv32[row] * rf;
-        if (init) p32[row] = z;
-        acc[0] += rv * rv;
-        acc[1] += bv * bv;
-        acc[2] += (double)rf * (double)z;
-      }
-    }
-  }
-  if (grid_finish<3>(acc, part, count) && threadIdx.x == 0) {
-    const double rr = acc[0];
-    S[S_RRTRUE] = rr;
-    if (init) {
-      const double tol2 = S[S_THR];  // holds rtol^2 on entry
-      S[S_BB] = acc[1];
-      S[S_THR] = tol2 * acc[1];
-      S[S_ITS] = 0;
-      S[S_NREF] = 0;
-      S[S_DONE] = 0;
-      S[S_REASON] = 0;
-      S[S_REFRESH] = 0;
-      S[S_RZ] = acc[2];
-      S[S_MAXRR] = rr;
-      if (acc[1] == 0.0 || rr <= S[S_THR]) {
-        S[S_DONE] = 1;
-        S[S_REASON] = LMH_CONVERGED_RTOL;
-      }
-    } else {
-      S[S_NREF] += 1;
-      if (rr <= S[S_THR]) {
-        S[S_DONE] = 1;
-        S[S_REASON] = LMH_CONVERGED_RTOL;
-      } else if (!isfinite(rr)) {
-        S[S_DONE] = 1;
-        S[S_REASON] = LMH_DIVERGED_NANORINF;
-      } else {
-        S[S_MAXRR] = rr;
-        const double rzn = acc[2];
-        S[S_BETA] = (S[S_MODE] == 0.0) ? rzn / S[S_RZ] : 0.0;
-        S[S_RZ] = rzn;
-      }
-    }
-  }
+  rowblock_spmv<double, double, double>(rb, nrb, rp, ci, v, x, [&](int32_t row, double s) {
+    const double bv = b[row], rv = bv - s;
+    const float rf = (float)rv;
+    r32[row] = rf;
+    const float z = dinv32[row] * rf;
+    if (init) p32[row] = z;
+    acc[0] += rv * rv;
+    acc[1] += bv * bv;
+    acc[2] += (double)rf * (double)z;
+  });
+  publish<3>(acc, part);
 }
 
-template <int L>
-__global__ __launch_bounds__(BLK) void k_mx_spmv(int64_t n, const int32_t *__restrict__ rp,
-                                                 const int32_t *__restrict__ ci, const float *__restrict__ v,
-                                                 const float *__restrict__ p, float *__restrict__ q,
-                                                 double *__restrict__ part, unsigned *__restrict__ count,
-                                                 double *__restrict__ S) {
-  if (S[S_DONE] != 0.0) return;
-  int64_t lo, hi;
-  block_range(n, lo, hi);
-  const int g = threadIdx.x / L, sub = threadIdx.x % L;
-  double acc[1] = {0};
-  for (int64_t r0 = lo; r0 < hi; r0 += BLK / L) {
-    const int64_t r = r0 + g;
-    if (r < hi) {
-      const float s = row_dot<L, float, float, float>(rp, ci, v, p, r, sub);
-      if (sub == 0) {
-        q[r] = s;
-        acc[0] += (double)p[r] * (double)s;
-      }
-    }
-  }
-  if (grid_finish<1>(acc, part, count) && threadIdx.x == 0) {
-    const double pap = acc[0];
-    const double alpha = S[S_RZ] / pap;
-    S[S_ALPHA] = alpha;
-    if (!(pap != 0.0) || !isfinite(alpha)) {
+__global__ __launch_bounds__(BLK) void k_fin_mx_residual(const double *__restrict__ part, int G, int init,
+                                                         double *__restrict__ S) {
+  if (!init && (S[S_DONE] != 0.0 || S[S_REFRESH] == 0.0)) return;
+  double t[3];
+  reduce_partials<3>(part, G, t);
+  if (threadIdx.x != 0) return;
+  const double rr = t[0];
+  S[S_RRTRUE] = rr;
+  if (init) {
+    const double tol2 = S[S_THR];  // rtol^2 on entry
+    S[S_BB] = t[1];
+    S[S_THR] = tol2 * t[1];
+    S[S_ITS] = 0;
+    S[S_NREF] = 0;
+    S[S_DONE] = 0;
+    S[S_REASON] = 0;
+    S[S_REFRESH] = 0;
+    S[S_RZ] = t[2];
+    S[S_MAXRR] = rr;
+    if (t[1] == 0.0 || rr <= S[S_THR]) {
       S[S_DONE] = 1;
-      S[S_REASON] = LMH_DIVERGED_BREAKDOWN;
+      S[S_REASON] = LMH_CONVERGED_RTOL;
     }
+    return;
+  }
+  S[S_NREF] += 1;
+  if (rr <= S[S_THR]) {
+    S[S_DONE] = 1;
+    S[S_REASON] = LMH_CONVERGED_RTOL;
+  } else if (!isfinite(rr)) {
+    S[S_DONE] = 1;
+    S[S_REASON] = LMH_DIVERGED_NANORINF;
+  } else {
+    // keep the search direction (mode 0, reliable update) or restart (mode 1)
+    S[S_MAXRR] = rr;
+    S[S_BETA] = (S[S_MODE] == 0.0) ? t[2] / S[S_RZ] : 0.0;
+    S[S_RZ] = t[2];
   }
 }
 
 __global__ __launch_bounds__(BLK) void k_mx_update(int64_t n, const float *__restrict__ dinv32,
                                                    const float *__restrict__ p, const float *__restrict__ q,
                                                    float *__restrict__ xs, float *__restrict__ r,
-                                                   double *__restrict__ part, unsigned *__restrict__ count,
-                                                   double *__restrict__ S) {
+                                                   double *__restrict__ part, const double *__restrict__ S) {
   if (S[S_DONE] != 0.0) return;
   int64_t lo, hi;
   block_range(n, lo, hi);
@@ -424,23 +441,32 @@ __global__ __launch_bounds__(BLK) void k_mx_update(int64_t n, const float *__res
     acc[0] += (double)rv * (double)rv;
     acc[1] += (double)rv * (double)z;
   }
-  if (grid_finish<2>(acc, part, count) && threadIdx.x == 0) {
-    S[S_ITS] += 1;
-    const double rr = acc[0];
-    if (!isfinite(rr)) {
-      S[S_DONE] = 1;
-      S[S_REASON] = LMH_DIVERGED_NANORINF;
-    } else if (rr < S[S_DELTA2] * S[S_MAXRR] || rr < S[S_THR]) {
-      S[S_REFRESH] = 1;
-    } else {
-      S[S_REFRESH] = 0;
-      S[S_BETA] = acc[1] / S[S_RZ];
-      S[S_RZ] = acc[1];
-    }
+  publish<2>(acc, part);
+}
+
+// refresh when the FP32 residual fell below delta times the last true residual
+// (or below the final threshold); otherwise the ordinary CG beta
+__global__ __launch_bounds__(BLK) void k_fin_mx_update(const double *__restrict__ part, int G,
+                                                       double *__restrict__ S) {
+  if (S[S_DONE] != 0.0) return;
+  double t[2];
+  reduce_partials<2>(part, G, t);
+  if (threadIdx.x != 0) return;
+  S[S_ITS] += 1;
+  const double rr = t[0];
+  if (!isfinite(rr)) {
+    S[S_DONE] = 1;
+    S[S_REASON] = LMH_DIVERGED_NANORINF;
+  } else if (rr < S[S_DELTA2] * S[S_MAXRR] || rr < S[S_THR]) {
+    S[S_REFRESH] = 1;
+  } else {
+    S[S_REFRESH] = 0;
+    S[S_BETA] = t[1] / S[S_RZ];
+    S[S_RZ] = t[1];
   }
 }
 
-// x += xs; xs = 0 (only on a refresh, or when forced)
+// x += xs; xs = 0 (on a refresh, or when forced)
 __global__ __launch_bounds__(BLK) void k_mx_fold(int64_t n, int force, float *__restrict__ xs, double *__restrict__ x,
                                                  const double *__restrict__ S) {
   if (!force && (S[S_DONE] != 0.0 || S[S_REFRESH] == 0.0)) return;
@@ -467,37 +493,25 @@ __global__ void k_round32(int64_t n, const double *__restrict__ a, float *__rest
     b[i] = (float)a[i];
 }
 
-__global__ __launch_bounds__(BLK) void k_norm2(int64_t n, const double *__restrict__ x, double *__restrict__ part,
-                                               unsigned *__restrict__ count, double *__restrict__ out) {
+__global__ __launch_bounds__(BLK) void k_norm2(int64_t n, const double *__restrict__ x, double *__restrict__ part) {
   int64_t lo, hi;
   block_range(n, lo, hi);
   double acc[1] = {0};
   for (int64_t i = lo + threadIdx.x; i < hi; i += BLK) acc[0] += x[i] * x[i];
-  if (grid_finish<1>(acc, part, count) && threadIdx.x == 0) *out = acc[0];
+  publish<1>(acc, part);
 }
 
-// lanes per row from the mean row length
-static int lanes_for(const lmh_matrix *A) {
-  const double avg = A->rows ? (double)A->nnz / (double)A->rows : 0.0;
-  if (avg <= 12) return 4;
-  if (avg <= 24) return 8;
-  if (avg <= 96) return 16;
-  return 32;
+__global__ __launch_bounds__(BLK) void k_fin_sum(const double *__restrict__ part, int G, double *__restrict__ out) {
+  double t[1];
+  reduce_partials<1>(part, G, t);
+  if (threadIdx.x == 0) *out = t[0];
 }
 
 static int vec_grid(lmh_context *ctx) {
-  // partial-sum buffer holds 4 * n_part doubles; grid_finish uses up to 3 * grid
+  // the partial buffer holds 4 * n_part doubles; publish<3> uses 3 * grid
   const int g = ctx->cu_count * 8;
   return g < ctx->n_part ? g : ctx->n_part;
 }
-
-#define LMH_DISPATCH_L(Lv, ...)                                                                        \
-  switch (Lv) {                                                                                        \
-    case 4: { constexpr int L = 4; __VA_ARGS__; } break;                                               \
-    case 8: { constexpr int L = 8; __VA_ARGS__; } break;                                               \
-    case 16: { constexpr int L = 16; __VA_ARGS__; } break;                                             \
-    default: { constexpr int L = 32; __VA_ARGS__; } break;                                             \
-  }
 
 static int ensure_work(lmh_matrix *A, bool mixed) {
   if (!A->w64) LMH_HIP(hipMalloc((void **)&A->w64, sizeof(double) * 4 * A->rows));
@@ -515,8 +529,18 @@ int lmh_spmv(lmh_matrix *A, const double *x_d, double *y_d) {
   LMH_CHECK(A && x_d && y_d, LMH_ERR_INVALID, "null argument");
   lmh_context *ctx = A->ctx;
   const int g = vec_grid(ctx);
-  LMH_DISPATCH_L(lanes_for(A), LMH_LAUNCH(ctx, "spmv_fp64", (k_spmv<L, double, double, double, double>), dim3(g),
-                                          dim3(BLK), 0, A->rows, A->rp, A->ci, A->vals, x_d, y_d));
+  LMH_LAUNCH(ctx, "spmv_fp64", (k_spmv<double, double, double, double>), dim3(g),
+                                          dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals, x_d, y_d);
+  return check_launch("k_spmv");
+}
+
+int lmh_spmv_fp32(lmh_matrix *A, const float *x_d, float *y_d) {
+  LMH_CHECK(A && x_d && y_d, LMH_ERR_INVALID, "null argument");
+  LMH_CHECK(A->have32 && A->vals32, LMH_ERR_INVALID, "matrix has no FP32 operator");
+  lmh_context *ctx = A->ctx;
+  const int g = vec_grid(ctx);
+  LMH_LAUNCH(ctx, "spmv_fp32", (k_spmv<float, float, float, float>), dim3(g),
+                                          dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals32, x_d, y_d);
   return check_launch("k_spmv");
 }
 
@@ -524,16 +548,17 @@ int lmh_residual(lmh_matrix *A, const double *b_d, const double *x_d, double *r_
   LMH_CHECK(A && b_d && x_d && r_d, LMH_ERR_INVALID, "null argument");
   lmh_context *ctx = A->ctx;
   const int g = vec_grid(ctx);
-  LMH_DISPATCH_L(lanes_for(A), LMH_LAUNCH(ctx, "residual_fp64", k_residual<L>, dim3(g), dim3(BLK), 0, A->rows, A->rp,
-                                          A->ci, A->vals, b_d, x_d, r_d, ctx->d_part, ctx->d_count + C_RES,
-                                          ctx->d_scal + 32));
+  LMH_LAUNCH(ctx, "residual_fp64", k_residual, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp,
+                                          A->ci, A->vals, b_d, x_d, r_d, ctx->d_part);
+  LMH_LAUNCH(ctx, "fin", k_fin_residual, dim3(1), dim3(BLK), 0, ctx->d_part, g, ctx->d_scal + 32);
   return check_launch("k_residual");
 }
 
 double lmh_norm2(lmh_context *ctx, const double *x_d, int64_t n) {
   if (!ctx || !x_d) return NAN;
-  hipLaunchKernelGGL(k_norm2, dim3(vec_grid(ctx)), dim3(BLK), 0, ctx->stream, n, x_d, ctx->d_part,
-                     ctx->d_count + C_AUX, ctx->d_scal + 48);
+  const int g = vec_grid(ctx);
+  hipLaunchKernelGGL(k_norm2, dim3(g), dim3(BLK), 0, ctx->stream, n, x_d, ctx->d_part);
+  hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(BLK), 0, ctx->stream, ctx->d_part, g, ctx->d_scal + 48);
   double v = NAN;
   if (hipMemcpyAsync(&v, ctx->d_scal + 48, sizeof(double), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
     return NAN;
@@ -586,24 +611,25 @@ int lmh_cg_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int
   LMH_TRY(ensure_work(A, false));
   const int64_t n = A->rows;
   double *r = A->w64, *p = r + n, *q = p + n, *dinv = q + n;
-  double *S = ctx->d_scal;
+  double *S = ctx->d_scal, *part = ctx->d_part;
   const int g = vec_grid(ctx);
-  const int Lv = lanes_for(A);
   LMH_LAUNCH(ctx, "invdiag", (k_invdiag<double, double>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, n, A->rp, A->ci,
              A->vals, dinv);
-  LMH_DISPATCH_L(Lv, LMH_LAUNCH(ctx, "residual_fp64", k_residual<L>, dim3(g), dim3(BLK), 0, n, A->rp, A->ci, A->vals,
-                                b_d, x_d, r, ctx->d_part, ctx->d_count + C_RES, S));
-  LMH_LAUNCH(ctx, "cg_init", k_cg_init, dim3(g), dim3(BLK), 0, n, rtol, dinv, r, p, x_d, ctx->d_part,
-             ctx->d_count + C_AUX, S);
+  LMH_LAUNCH(ctx, "residual_fp64", k_residual, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals,
+                                b_d, x_d, r, part);
+  LMH_LAUNCH(ctx, "fin", k_fin_residual, dim3(1), dim3(BLK), 0, part, g, S);
+  LMH_LAUNCH(ctx, "cg_init", k_cg_init, dim3(g), dim3(BLK), 0, n, dinv, r, p, x_d, part, S);
+  LMH_LAUNCH(ctx, "fin", k_fin_cg_init, dim3(1), dim3(BLK), 0, part, g, rtol, S);
   LMH_TRY(check_launch("cg setup"));
   auto iter = [&]() {
-    LMH_DISPATCH_L(Lv, LMH_LAUNCH(ctx, "cg_spmv_fp64", k_cg_spmv<L>, dim3(g), dim3(BLK), 0, n, A->rp, A->ci, A->vals, p,
-                                  q, ctx->d_part, ctx->d_count + C_SPMV, S));
-    LMH_LAUNCH(ctx, "cg_update_fp64", k_cg_update, dim3(g), dim3(BLK), 0, n, dinv, p, q, x_d, r, ctx->d_part,
-               ctx->d_count + C_UPD, S);
+    LMH_LAUNCH(ctx, "cg_spmv_fp64", (k_spmv_dot<double, double>), dim3(g), dim3(BLK), 0, A->rb,
+                                  A->nrb, A->rp, A->ci, A->vals, p, q, part, S);
+    LMH_LAUNCH(ctx, "fin", k_fin_alpha, dim3(1), dim3(BLK), 0, part, g, (int)S_ABSNEW, S);
+    LMH_LAUNCH(ctx, "cg_update_fp64", k_cg_update, dim3(g), dim3(BLK), 0, n, dinv, p, q, x_d, r, part, S);
+    LMH_LAUNCH(ctx, "fin", k_fin_cg_beta, dim3(1), dim3(BLK), 0, part, g, S);
     LMH_LAUNCH(ctx, "cg_pupdate_fp64", k_cg_pupdate, dim3(g), dim3(BLK), 0, n, dinv, r, p, S);
   };
-  LMH_TRY(run_batches(ctx, maxits, 16, iter));
+  LMH_TRY(run_batches(ctx, maxits, 4, iter));
   double h[16];
   LMH_HIP(hipMemcpy(h, S, sizeof(h), hipMemcpyDeviceToHost));
   const bool done = h[S_DONE] != 0.0;
@@ -623,9 +649,8 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
   LMH_TRY(ensure_work(A, true));
   const int64_t n = A->rows;
   float *xs = A->w32, *r32 = xs + n, *p32 = r32 + n, *q32 = p32 + n, *d32 = q32 + n;
-  double *S = ctx->d_scal;
+  double *S = ctx->d_scal, *part = ctx->d_part;
   const int g = vec_grid(ctx);
-  const int Lv = lanes_for(A);
   double init[32] = {0};
   init[S_THR] = rtol * rtol;
   init[S_DELTA2] = inner_rtol * inner_rtol;
@@ -634,33 +659,34 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
   LMH_HIP(hipMemsetAsync(xs, 0, sizeof(float) * n, ctx->stream));
   LMH_LAUNCH(ctx, "invdiag", (k_invdiag<float, float>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, n, A->rp, A->ci,
              A->vals32, d32);
-  LMH_DISPATCH_L(Lv, LMH_LAUNCH(ctx, "mx_residual_fp64", k_mx_residual<L>, dim3(g), dim3(BLK), 0, n, 1, A->rp, A->ci,
-                                A->vals, b_d, x_d, d32, r32, p32, ctx->d_part, ctx->d_count + C_RES, S));
+  LMH_LAUNCH(ctx, "mx_residual_fp64", k_mx_residual, dim3(g), dim3(BLK), 0, 1, A->rb, A->nrb, A->rp, A->ci,
+                                A->vals, b_d, x_d, d32, r32, p32, part, S);
+  LMH_LAUNCH(ctx, "fin", k_fin_mx_residual, dim3(1), dim3(BLK), 0, part, g, 1, S);
   LMH_TRY(check_launch("mixed setup"));
   auto iter = [&]() {
-    LMH_DISPATCH_L(Lv, LMH_LAUNCH(ctx, "mx_spmv_fp32", k_mx_spmv<L>, dim3(g), dim3(BLK), 0, n, A->rp, A->ci, A->vals32,
-                                  p32, q32, ctx->d_part, ctx->d_count + C_SPMV, S));
-    LMH_LAUNCH(ctx, "mx_update_fp32", k_mx_update, dim3(g), dim3(BLK), 0, n, d32, p32, q32, xs, r32, ctx->d_part,
-               ctx->d_count + C_UPD, S);
+    LMH_LAUNCH(ctx, "mx_spmv_fp32", (k_spmv_dot<float, float>), dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp,
+                                  A->ci, A->vals32, p32, q32, part, S);
+    LMH_LAUNCH(ctx, "fin", k_fin_alpha, dim3(1), dim3(BLK), 0, part, g, (int)S_RZ, S);
+    LMH_LAUNCH(ctx, "mx_update_fp32", k_mx_update, dim3(g), dim3(BLK), 0, n, d32, p32, q32, xs, r32, part, S);
+    LMH_LAUNCH(ctx, "fin", k_fin_mx_update, dim3(1), dim3(BLK), 0, part, g, S);
     LMH_LAUNCH(ctx, "mx_fold", k_mx_fold, dim3(g), dim3(BLK), 0, n, 0, xs, x_d, S);
-    LMH_DISPATCH_L(Lv, LMH_LAUNCH(ctx, "mx_residual_fp64", k_mx_residual<L>, dim3(g), dim3(BLK), 0, n, 0, A->rp, A->ci,
-                                  A->vals, b_d, x_d, d32, r32, p32, ctx->d_part, ctx->d_count + C_RES, S));
+    LMH_LAUNCH(ctx, "mx_residual_fp64", k_mx_residual, dim3(g), dim3(BLK), 0, 0, A->rb, A->nrb, A->rp, A->ci,
+                                  A->vals, b_d, x_d, d32, r32, p32, part, S);
+    LMH_LAUNCH(ctx, "fin", k_fin_mx_residual, dim3(1), dim3(BLK), 0, part, g, 0, S);
     LMH_LAUNCH(ctx, "mx_pupdate_fp32", k_mx_pupdate, dim3(g), dim3(BLK), 0, n, d32, r32, p32, S);
   };
-  LMH_TRY(run_batches(ctx, maxits, 16, iter));
+  LMH_TRY(run_batches(ctx, maxits, 4, iter));
   double h[32];
   LMH_HIP(hipMemcpy(h, S, sizeof(h), hipMemcpyDeviceToHost));
-  bool done = h[S_DONE] != 0.0;
+  const bool done = h[S_DONE] != 0.0;
   if (!done) {
     // iteration limit: fold the pending correction and report the true residual
     LMH_LAUNCH(ctx, "mx_fold", k_mx_fold, dim3(g), dim3(BLK), 0, n, 1, xs, x_d, S);
-    LMH_DISPATCH_L(Lv, LMH_LAUNCH(ctx, "residual_fp64", k_residual<L>, dim3(g), dim3(BLK), 0, n, A->rp, A->ci, A->vals,
-                                  b_d, x_d, A->w64, ctx->d_part, ctx->d_count + C_RES, ctx->d_scal + 32));
+    LMH_LAUNCH(ctx, "residual_fp64", k_residual, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals,
+                                  b_d, x_d, A->w64, part);
+    LMH_LAUNCH(ctx, "fin", k_fin_residual, dim3(1), dim3(BLK), 0, part, g, ctx->d_scal + 32);
     LMH_TRY(check_launch("mixed finish"));
-    double t[2];
-    LMH_HIP(hipMemcpy(&t[0], ctx->d_scal + 32 + S_BB, sizeof(double), hipMemcpyDeviceToHost));
-    LMH_HIP(hipMemcpy(&t[1], ctx->d_scal + 32 + S_RR, sizeof(double), hipMemcpyDeviceToHost));
-    h[S_RRTRUE] = t[1];
+    LMH_HIP(hipMemcpy(&h[S_RRTRUE], ctx->d_scal + 32 + S_RR, sizeof(double), hipMemcpyDeviceToHost));
   }
   if (iters) *iters = (int64_t)h[S_ITS];
   if (rel_res) *rel_res = h[S_BB] > 0 ? std::sqrt(h[S_RRTRUE] / h[S_BB]) : 0.0;

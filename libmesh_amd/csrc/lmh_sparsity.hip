@@ -185,6 +185,18 @@ int lmh_matrix_create(lmh_context *ctx, const lmh_dofmap *dm, lmh_matrix **out) 
   LMH_HIP(hipMemcpyAsync(rp.data(), A->rp, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost, ctx->stream));
   LMH_HIP(hipStreamSynchronize(ctx->stream));
   for (int64_t r = 0; r < n; ++r) A->max_row = std::max<int>(A->max_row, rp[r + 1] - rp[r]);
+  // SpMV row blocks: consecutive rows holding at most kRowBlockCap stored
+  // entries (a longer row forms a block of its own)
+  {
+    const int kRowBlockCap = 2048;
+    std::vector<int32_t> rb{0};
+    for (int64_t r = 0; r < n; ++r)
+      if (r > rb.back() && rp[r + 1] - rp[rb.back()] > kRowBlockCap) rb.push_back((int32_t)r);
+    rb.push_back((int32_t)n);
+    A->nrb = (int64_t)rb.size() - 1;
+    LMH_HIP(hipMalloc((void **)&A->rb, sizeof(int32_t) * rb.size()));
+    LMH_HIP(hipMemcpy(A->rb, rb.data(), sizeof(int32_t) * rb.size(), hipMemcpyHostToDevice));
+  }
   LMH_HIP(hipFree(rowlen));
   LMH_CHECK(A->max_row <= 65535, LMH_ERR_UNSUPPORTED, "row length %d exceeds 16-bit scatter offsets", A->max_row);
 
@@ -218,6 +230,7 @@ int lmh_matrix_destroy(lmh_matrix *A) {
   (void)hipStreamSynchronize(A->ctx->stream);
   (void)hipFree(A->rp);
   (void)hipFree(A->ci);
+  (void)hipFree(A->rb);
   (void)hipFree(A->vals);
   (void)hipFree(A->vals32);
   (void)hipFree(A->off);
