@@ -97,6 +97,11 @@ int lmh_dofmap_destroy(lmh_dofmap *dm);
  * eigen_sparse_matrix.C:64-115).  Also builds the element scatter map and the
  * element colouring used by lmh_assemble_*. ---- */
 int lmh_matrix_create(lmh_context *ctx, const lmh_dofmap *dm, lmh_matrix **out);
+/* A matrix with a caller-given CSR pattern (SparseMatrix::update_sparsity_pattern
+ * from a SparsityPattern::Graph, sparse_matrix.h:175; sorted rows, square, FP64
+ * values zeroed).  It has no DoF map: element assembly calls reject it. */
+int lmh_matrix_create_csr(lmh_context *ctx, int64_t rows, const int32_t *row_ptr_h, const int32_t *col_idx_h,
+                          lmh_matrix **out);
 int lmh_matrix_destroy(lmh_matrix *A);
 int64_t lmh_matrix_rows(const lmh_matrix *A);
 int64_t lmh_matrix_nnz(const lmh_matrix *A);

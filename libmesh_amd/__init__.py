@@ -60,6 +60,7 @@ HIP_API = {
     "lmh_dofmap_create": (_I, [_P, _P, _I, _I64, _P, _P, C.POINTER(_P)]),
     "lmh_dofmap_destroy": (_I, [_P]),
     "lmh_matrix_create": (_I, [_P, _P, C.POINTER(_P)]),
+    "lmh_matrix_create_csr": (_I, [_P, _I64, _P, _P, C.POINTER(_P)]),
     "lmh_matrix_destroy": (_I, [_P]),
     "lmh_matrix_rows": (_I64, [_P]),
     "lmh_matrix_nnz": (_I64, [_P]),

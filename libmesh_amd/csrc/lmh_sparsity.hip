@@ -138,6 +138,21 @@ static int color_elements(const std::vector<int32_t> &edofs, int64_t n_elem, int
   return LMH_OK;
 }
 
+static int build_row_blocks(lmh_matrix *A, const std::vector<int32_t> &rp) {
+  // SpMV row blocks: consecutive rows holding at most kRowBlockCap stored
+  // entries (a longer row forms a block of its own)
+  const int kRowBlockCap = 2048;
+  std::vector<int32_t> rb{0};
+  const int64_t n = A->rows;
+  for (int64_t r = 0; r < n; ++r)
+    if (r > rb.back() && rp[r + 1] - rp[rb.back()] > kRowBlockCap) rb.push_back((int32_t)r);
+  rb.push_back((int32_t)n);
+  A->nrb = (int64_t)rb.size() - 1;
+  LMH_HIP(hipMalloc((void **)&A->rb, sizeof(int32_t) * rb.size()));
+  LMH_HIP(hipMemcpy(A->rb, rb.data(), sizeof(int32_t) * rb.size(), hipMemcpyHostToDevice));
+  return LMH_OK;
+}
+
 }  // namespace lmh
 
 using namespace lmh;
@@ -185,18 +200,7 @@ int lmh_matrix_create(lmh_context *ctx, const lmh_dofmap *dm, lmh_matrix **out) 
   LMH_HIP(hipMemcpyAsync(rp.data(), A->rp, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost, ctx->stream));
   LMH_HIP(hipStreamSynchronize(ctx->stream));
   for (int64_t r = 0; r < n; ++r) A->max_row = std::max<int>(A->max_row, rp[r + 1] - rp[r]);
-  // SpMV row blocks: consecutive rows holding at most kRowBlockCap stored
-  // entries (a longer row forms a block of its own)
-  {
-    const int kRowBlockCap = 2048;
-    std::vector<int32_t> rb{0};
-    for (int64_t r = 0; r < n; ++r)
-      if (r > rb.back() && rp[r + 1] - rp[rb.back()] > kRowBlockCap) rb.push_back((int32_t)r);
-    rb.push_back((int32_t)n);
-    A->nrb = (int64_t)rb.size() - 1;
-    LMH_HIP(hipMalloc((void **)&A->rb, sizeof(int32_t) * rb.size()));
-    LMH_HIP(hipMemcpy(A->rb, rb.data(), sizeof(int32_t) * rb.size(), hipMemcpyHostToDevice));
-  }
+  LMH_TRY(build_row_blocks(A.get(), rp));
   LMH_HIP(hipFree(rowlen));
   LMH_CHECK(A->max_row <= 65535, LMH_ERR_UNSUPPORTED, "row length %d exceeds 16-bit scatter offsets", A->max_row);
 
@@ -221,6 +225,40 @@ int lmh_matrix_create(lmh_context *ctx, const lmh_dofmap *dm, lmh_matrix **out) 
   LMH_HIP(hipMalloc((void **)&A->color_elems, sizeof(int32_t) * ne));
   LMH_HIP(hipMemcpy(A->color_elems, celems.data(), sizeof(int32_t) * ne, hipMemcpyHostToDevice));
   LMH_HIP(hipStreamSynchronize(ctx->stream));
+  *out = A.release();
+  return LMH_OK;
+}
+
+int lmh_matrix_create_csr(lmh_context *ctx, int64_t rows, const int32_t *row_ptr_h, const int32_t *col_idx_h,
+                          lmh_matrix **out) {
+  LMH_CHECK(ctx && row_ptr_h && out && rows > 0, LMH_ERR_INVALID, "bad argument");
+  LMH_CHECK(row_ptr_h[0] == 0, LMH_ERR_INVALID, "row_ptr[0] must be 0");
+  const int64_t nnz = row_ptr_h[rows];
+  LMH_CHECK(nnz >= 0 && (nnz == 0 || col_idx_h), LMH_ERR_INVALID, "bad nnz");
+  std::vector<int32_t> rp(row_ptr_h, row_ptr_h + rows + 1);
+  int max_row = 0;
+  for (int64_t r = 0; r < rows; ++r) {
+    LMH_CHECK(rp[r + 1] >= rp[r], LMH_ERR_INVALID, "row_ptr not monotone at row %lld", (long long)r);
+    max_row = std::max<int>(max_row, rp[r + 1] - rp[r]);
+    for (int32_t k = rp[r]; k < rp[r + 1]; ++k) {
+      LMH_CHECK(col_idx_h[k] >= 0 && col_idx_h[k] < rows, LMH_ERR_INVALID, "column index out of range");
+      LMH_CHECK(k == rp[r] || col_idx_h[k] > col_idx_h[k - 1], LMH_ERR_INVALID,
+                "columns of row %lld not strictly increasing", (long long)r);
+    }
+  }
+  LMH_HIP(hipSetDevice(ctx->device));
+  std::unique_ptr<lmh_matrix> A(new lmh_matrix);
+  A->ctx = ctx;
+  A->rows = rows;
+  A->nnz = nnz;
+  A->max_row = max_row;
+  LMH_HIP(hipMalloc((void **)&A->rp, sizeof(int32_t) * (rows + 1)));
+  LMH_HIP(hipMalloc((void **)&A->ci, sizeof(int32_t) * (nnz ? nnz : 1)));
+  LMH_HIP(hipMalloc((void **)&A->vals, sizeof(double) * (nnz ? nnz : 1)));
+  LMH_HIP(hipMemcpy(A->rp, rp.data(), sizeof(int32_t) * (rows + 1), hipMemcpyHostToDevice));
+  if (nnz) LMH_HIP(hipMemcpy(A->ci, col_idx_h, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+  LMH_HIP(hipMemset(A->vals, 0, sizeof(double) * (nnz ? nnz : 1)));
+  LMH_TRY(build_row_blocks(A.get(), rp));
   *out = A.release();
   return LMH_OK;
 }

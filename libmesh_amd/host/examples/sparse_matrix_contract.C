@@ -1,0 +1,114 @@
+// sparse_matrix_contract.C -- the SparseMatrix contract checks of the
+// reference's tests/numerics/sparse_matrix_test.h (setValues/testValues with
+// (i+1)(j+1), testTransposeNorms l1(A) = linf(A^T), testClone, get_diagonal,
+// add(a, X)) run on HipCsrMatrix, plus add_matrix on a DofMap-built pattern.
+// Prints one line per check and exits non-zero on any failure.
+#include <cmath>
+#include <cstdio>
+#include <numeric>
+
+#include "libmesh/hip_csr_matrix.h"
+
+using namespace libMesh;
+
+static int failures = 0;
+static void check(bool ok, const char *what) {
+  std::printf("%s %s\n", ok ? "PASS" : "FAIL", what);
+  if (!ok) ++failures;
+}
+
+int main() {
+  try {
+    // one rank: global_m = global_n = 1 * 5.75 - 1 = 4, locally dense block
+    const numeric_index_type m = 4;
+    HipCsrMatrix<Number> A;
+    A.init(m, m, m, m, m, 5);
+    std::vector<numeric_index_type> rows(m), cols(m);
+    std::iota(rows.begin(), rows.end(), A.row_start());
+    std::iota(cols.begin(), cols.end(), A.col_start());
+    DenseMatrix<Number> local(m, m);
+    for (unsigned i = 0; i < m; ++i)
+      for (unsigned j = 0; j < m; ++j) local(i, j) = (i + 1.0) * (j + 1.0);
+    A.zero();
+    A.add_matrix(local, rows, cols);
+    A.close();
+
+    bool ok = true;
+    for (numeric_index_type i = A.row_start(); i < A.row_stop(); ++i) {
+      std::vector<numeric_index_type> idx;
+      std::vector<Number> val;
+      A.get_row(i, idx, val);
+      for (size_t j = 0; j < idx.size(); ++j)
+        ok &= idx[j] == A.col_start() + j && std::abs(val[j] - (i + 1.0) * (j + 1.0)) < 1e-12;
+      ok &= idx.size() == m;
+    }
+    check(ok, "testValues: get_row returns (i+1)(j+1)");
+    check(A(2, 3) == 12.0 && A.m() == m && A.n() == m, "operator() and sizes");
+
+    HipCsrMatrix<Number> At;
+    A.get_transpose(At);
+    check(std::abs(A.l1_norm() - At.linfty_norm()) < 1e-12 && std::abs(A.linfty_norm() - At.l1_norm()) < 1e-12,
+          "testTransposeNorms: l1(A) = linf(A^T)");
+    check(std::abs(A.l1_norm() - 40.0) < 1e-12, "l1 norm closed form (max column sum 4*10)");
+
+    auto C = A.clone();
+    bool cl = true;
+    for (numeric_index_type i = 0; i < m; ++i)
+      for (numeric_index_type j = 0; j < m; ++j) cl &= (*C)(i, j) == A(i, j);
+    check(cl, "testClone: clone equals original");
+    auto Z = A.zero_clone();
+    check(Z->l1_norm() == 0.0 && Z->n_nonzeros() == A.n_nonzeros(), "zero_clone keeps the pattern, zero values");
+
+    A.add(2.0, *C);
+    check(A(3, 3) == 48.0, "add(a, X): A + 2A");
+
+    HipVector<Number> d(m);
+    A.get_diagonal(d);
+    check(d(0) == 3.0 && d(3) == 48.0, "get_diagonal");
+
+    HipVector<Number> x(m), y(m);
+    for (numeric_index_type i = 0; i < m; ++i) x.set(i, 1.0);
+    x.close();
+    A.vector_mult(y, x);
+    check(std::abs(y(0) - 3.0 * 10.0) < 1e-12, "vector_mult on the device");
+
+    // DofMap-built pattern: element matrices in call order
+    Mesh mesh;
+    MeshTools::Generation::build_cube(mesh, 3, 3, 3, 0., 1., 0., 1., 0., 1., HEX8);
+    DofMap dm;
+    dm.add_variable("u", FEType(FIRST, LAGRANGE));
+    dm.distribute_dofs(mesh);
+    HipCsrMatrix<Number> B;
+    B.attach_dof_map(dm);
+    B.init();
+    check(B.n_nonzeros() == 10 * 10 * 10, "DofMap pattern nnz = (3n+1)^3");
+    B.zero();
+    std::vector<dof_id_type> di;
+    DenseMatrix<Number> Ke(8, 8);
+    for (unsigned a = 0; a < 8; ++a)
+      for (unsigned b = 0; b < 8; ++b) Ke(a, b) = (a == b) ? 1.0 : 0.0;
+    for (dof_id_type e = 0; e < mesh.n_elem(); ++e) {
+      dm.dof_indices(e, di);
+      std::vector<numeric_index_type> dd(di.begin(), di.end());
+      B.add_matrix(Ke, dd);
+    }
+    B.close();
+    // diagonal = number of elements touching each node: corner 1, interior 8
+    check(B(0, 0) == 1.0, "add_matrix batch: corner diagonal 1");
+    double mx = 0;
+    for (numeric_index_type i = 0; i < B.m(); ++i) mx = std::max(mx, (double)B(i, i));
+    check(mx == 8.0, "add_matrix batch: interior diagonal 8");
+    bool threw = false;
+    try {
+      B.set(0, B.m() - 1, 1.0);
+    } catch (const LogicError &) {
+      threw = true;
+    }
+    check(threw, "set outside the pattern throws LogicError");
+  } catch (const std::exception &e) {
+    std::printf("FAIL exception: %s\n", e.what());
+    return 1;
+  }
+  std::printf("%d failures\n", failures);
+  return failures ? 1 : 0;
+}

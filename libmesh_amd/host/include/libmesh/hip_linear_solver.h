@@ -1,0 +1,42 @@
+// hip_linear_solver.h -- HipLinearSolver<T>: Jacobi-preconditioned CG on the
+// GPU for a HipCsrMatrix (EigenSparseLinearSolver CG + JACOBI_PRECOND,
+// src/solvers/eigen_sparse_linear_solver.C:82-170), optionally as FP32 inner CG
+// with FP64 reliable-update refinement to the same FP64 tolerance.
+#pragma once
+
+#include "libmesh/hip_csr_matrix.h"
+#include "libmesh/linear_solver.h"
+
+namespace libMesh {
+
+template <typename T>
+class HipLinearSolver : public LinearSolver<T> {
+ public:
+  HipLinearSolver() = default;
+  void init(const char *name = nullptr) override;
+  std::pair<unsigned int, Real> solve(SparseMatrix<T> &matrix, NumericVector<T> &solution, NumericVector<T> &rhs,
+                                      std::optional<double> tol = std::nullopt,
+                                      std::optional<unsigned int> m_its = std::nullopt) override;
+  // The assembled operator is symmetric (C^T K C with unit diagonals on
+  // constrained rows), so the adjoint solve is the same solve.
+  std::pair<unsigned int, Real> adjoint_solve(SparseMatrix<T> &matrix, NumericVector<T> &solution,
+                                              NumericVector<T> &rhs, std::optional<double> tol = std::nullopt,
+                                              std::optional<unsigned int> m_its = std::nullopt) override;
+  LinearConvergenceReason get_converged_reason() const override { return _reason; }
+
+  // mixed precision: FP32 inner CG, FP64 true residual refreshed when the FP32
+  // residual drops by `inner` relative to the last refresh
+  void set_mixed_precision(bool on, double inner = 0.1) {
+    _mixed = on;
+    _inner = inner;
+  }
+  unsigned int n_refreshes() const { return _nref; }
+
+ private:
+  LinearConvergenceReason _reason = UNKNOWN_FLAG;
+  bool _mixed = false;
+  double _inner = 0.1;
+  unsigned int _nref = 0;
+};
+
+}  // namespace libMesh

@@ -1,0 +1,130 @@
+// system.h -- the System / ImplicitSystem / LinearImplicitSystem /
+// EquationSystems surface the assembly + solve path goes through
+// (include/systems/*.h).  Injection points are libMesh's own:
+//   System::add_matrix("System Matrix", unique_ptr<SparseMatrix>)  (system.C:1041-1061)
+//   ImplicitSystem::linear_solver, a public unique_ptr            (implicit_system.h:328)
+//   System::attach_assemble_object / attach_assemble_function     (system.C:1959-1987)
+#pragma once
+
+#include <functional>
+#include <map>
+#include <memory>
+#include <string>
+
+#include "libmesh/dof_map.h"
+#include "libmesh/linear_solver.h"
+#include "libmesh/replicated_mesh.h"
+
+namespace libMesh {
+
+class EquationSystems;
+
+class System {
+ public:
+  // System::Assembly (include/systems/system.h:170)
+  class Assembly {
+   public:
+    virtual ~Assembly() = default;
+    virtual void assemble() = 0;
+  };
+
+  System(EquationSystems &es, const std::string &name, unsigned number);
+  virtual ~System() = default;
+
+  const std::string &name() const { return _name; }
+  EquationSystems &get_equation_systems() { return _es; }
+  DofMap &get_dof_map() { return _dof_map; }
+  const DofMap &get_dof_map() const { return _dof_map; }
+  unsigned add_variable(const std::string &var, Order order = FIRST, FEFamily family = LAGRANGE) {
+    return _dof_map.add_variable(var, FEType(order, family));
+  }
+
+  // System::add_matrix with a caller-built matrix (system.C:1041-1061)
+  SparseMatrix<Number> &add_matrix(const std::string &mat_name, std::unique_ptr<SparseMatrix<Number>> matrix);
+  SparseMatrix<Number> &get_matrix(const std::string &mat_name);
+  bool have_matrix(const std::string &mat_name) const { return _matrices.count(mat_name) != 0; }
+
+  void attach_assemble_function(void (*fptr)(EquationSystems &, const std::string &));
+  void attach_assemble_object(Assembly &assemble);
+
+  // System::init -> init_data (dof distribution, vectors) + init_matrices (system.C:196-371)
+  virtual void init();
+  // System::assemble -> user_assembly (system.C:554-561, 2109-2119)
+  virtual void assemble();
+  virtual void solve() {}
+
+  std::unique_ptr<NumericVector<Number>> solution;
+  NumericVector<Number> *rhs = nullptr;
+
+ protected:
+  virtual void init_matrices();
+  EquationSystems &_es;
+  std::string _name;
+  unsigned _number;
+  DofMap _dof_map;
+  std::map<std::string, std::unique_ptr<SparseMatrix<Number>>> _matrices;
+  std::unique_ptr<NumericVector<Number>> _rhs;
+  void (*_assemble_fptr)(EquationSystems &, const std::string &) = nullptr;
+  Assembly *_assemble_obj = nullptr;
+};
+
+class ImplicitSystem : public System {
+ public:
+  using System::System;
+  SparseMatrix<Number> &get_system_matrix() { return get_matrix("System Matrix"); }
+  // ImplicitSystem::assemble zeroes matrix and rhs first (implicit_system.C:94-108)
+  void assemble() override;
+  // ImplicitSystem::adjoint_solve with a caller-assembled QoI derivative:
+  // K^T z = dq (implicit_system.C:196-240)
+  std::pair<unsigned int, Real> adjoint_solve(NumericVector<Number> &z, NumericVector<Number> &dq);
+  // ImplicitSystem::linear_solver (public, include/systems/implicit_system.h:328)
+  mutable std::unique_ptr<LinearSolver<Number>> linear_solver;
+
+ protected:
+  void init_matrices() override;
+};
+
+class LinearImplicitSystem : public ImplicitSystem {
+ public:
+  using ImplicitSystem::ImplicitSystem;
+  // LinearImplicitSystem::solve (src/systems/linear_implicit_system.C:122-161)
+  void solve() override;
+  unsigned n_linear_iterations() const { return _n_linear_iterations; }
+  Real final_linear_residual() const { return _final_linear_residual; }
+  bool assemble_before_solve = true;
+
+ private:
+  unsigned _n_linear_iterations = 0;
+  Real _final_linear_residual = 0;
+};
+
+class EquationSystems {
+ public:
+  explicit EquationSystems(MeshBase &mesh) : _mesh(mesh) {}
+  template <typename T>
+  T &add_system(const std::string &name) {
+    auto s = std::make_unique<T>(*this, name, (unsigned)_systems.size());
+    T &ref = *s;
+    _systems[name] = std::move(s);
+    return ref;
+  }
+  template <typename T>
+  T &get_system(const std::string &name) {
+    auto it = _systems.find(name);
+    if (it == _systems.end()) libmesh_error_msg("no system named " << name);
+    return dynamic_cast<T &>(*it->second);
+  }
+  MeshBase &get_mesh() { return _mesh; }
+  void init() {
+    for (auto &kv : _systems) kv.second->init();
+  }
+  // EquationSystems parameters (equation_systems.C:59-60)
+  Real linear_solver_tolerance = 1e-12;
+  unsigned linear_solver_maximum_iterations = 5000;
+
+ private:
+  MeshBase &_mesh;
+  std::map<std::string, std::unique_ptr<System>> _systems;
+};
+
+}  // namespace libMesh

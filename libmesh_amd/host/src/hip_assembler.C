@@ -1,0 +1,74 @@
+// hip_assembler.C -- System::Assembly object that runs the element loop on the GPU.
+#include "libmesh/hip_assembler.h"
+
+namespace libMesh {
+
+HipAssembler::HipAssembler(ImplicitSystem &system, const MeshBase &mesh, const DofMap &dof_map,
+                           const FEType &fe_type, const QGauss &qrule, Forcing f)
+    : _system(system), _mesh(mesh), _dof_map(dof_map), _fe_type(fe_type), _qrule(qrule), _f(f) {}
+
+HipAssembler::~HipAssembler() {
+  if (_kappa_d) lmh_free(HipDevice::context(), _kappa_d);
+}
+
+void HipAssembler::set_coefficient(const std::vector<Real> &kappa) {
+  lmh_context *ctx = HipDevice::context();
+  if (_kappa_d) lmh_free(ctx, _kappa_d);
+  _kappa_d = nullptr;
+  _kappa_n = kappa.size();
+  if (kappa.empty()) return;
+  if (kappa.size() != _mesh.n_elem()) libmesh_error_msg("coefficient needs one value per element");
+  void *p = nullptr;
+  hip_check(lmh_malloc(ctx, sizeof(double) * kappa.size(), &p), "HipAssembler::set_coefficient");
+  hip_check(lmh_memcpy_h2d(ctx, p, kappa.data(), sizeof(double) * kappa.size()), "HipAssembler::set_coefficient");
+  _kappa_d = (double *)p;
+}
+
+void HipAssembler::assemble() {
+  auto *A = dynamic_cast<HipCsrMatrix<Number> *>(&_system.get_system_matrix());
+  auto *b = dynamic_cast<HipVector<Number> *>(_system.rhs);
+  if (!A || !b) libmesh_error_msg("HipAssembler needs a HipCsrMatrix system matrix and HipVector rhs");
+  if (!A->device_matrix() || !A->device_dofmap()) libmesh_error_msg("HipAssembler: matrix not initialised from the DofMap");
+  if (&_dof_map != A->dof_map()) libmesh_error_msg("HipAssembler: DofMap differs from the matrix's");
+  if (_fe_type.order != _dof_map.variable_type(0).order) libmesh_error_msg("HipAssembler: FEType mismatch");
+  const int arg = _qrule.kernel_arg(_mesh.elem_type());
+  hip_check(lmh_assemble_poisson(A->device_matrix(), A->device_mesh(), A->device_dofmap(), arg, _kappa_d, (int)_f,
+                                 (double *)b->device_data(), LMH_FP64, _scatter),
+            "HipAssembler::assemble");
+  if (_fp32) hip_check(lmh_matrix_make_fp32(A->device_matrix()), "HipAssembler::assemble (FP32 operator)");
+  A->mark_device_modified();
+  b->mark_device_modified();
+}
+
+void HipAssembler::qoi_box_derivative(Real lo, Real hi, NumericVector<Number> &dq_in) {
+  auto *A = dynamic_cast<HipCsrMatrix<Number> *>(&_system.get_system_matrix());
+  auto *dq = dynamic_cast<HipVector<Number> *>(&dq_in);
+  if (!A || !dq) libmesh_error_msg("qoi_box_derivative: HipCsrMatrix / HipVector required");
+  if (dq->size() != _dof_map.n_dofs()) dq->init(_dof_map.n_dofs());
+  const int arg = _qrule.kernel_arg(_mesh.elem_type());
+  hip_check(lmh_qoi_box_derivative(A->device_mesh(), A->device_dofmap(), arg, lo, hi, (double *)dq->device_data()),
+            "qoi_box_derivative");
+  dq->mark_device_modified();
+}
+
+std::vector<Real> HipAssembler::kappa_sensitivities(const NumericVector<Number> &u_in,
+                                                    const NumericVector<Number> &z_in) {
+  auto *A = dynamic_cast<HipCsrMatrix<Number> *>(&_system.get_system_matrix());
+  const auto *u = dynamic_cast<const HipVector<Number> *>(&u_in);
+  const auto *z = dynamic_cast<const HipVector<Number> *>(&z_in);
+  if (!A || !u || !z) libmesh_error_msg("kappa_sensitivities: HipCsrMatrix / HipVector required");
+  lmh_context *ctx = HipDevice::context();
+  void *g = nullptr;
+  hip_check(lmh_malloc(ctx, sizeof(double) * _mesh.n_elem(), &g), "kappa_sensitivities");
+  const int arg = _qrule.kernel_arg(_mesh.elem_type());
+  const int rc = lmh_adjoint_kappa_gradient(A->device_mesh(), A->device_dofmap(), arg, _kappa_d,
+                                            (const double *)u->device_data(), (const double *)z->device_data(),
+                                            (double *)g);
+  std::vector<Real> out(_mesh.n_elem());
+  if (rc == LMH_OK) lmh_memcpy_d2h(ctx, out.data(), g, sizeof(double) * out.size());
+  lmh_free(ctx, g);
+  hip_check(rc, "kappa_sensitivities");
+  return out;
+}
+
+}  // namespace libMesh

@@ -1,0 +1,69 @@
+"""The libMesh-API drop-in classes (HipCsrMatrix, HipLinearSolver, HipAssembler)
+driven from C++ exactly as a libMesh application would, checked against the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "libmesh_amd", "host", "bin")
+
+
+def run(args, tmp_path, expect=0):
+    r = subprocess.run([os.path.join(BIN, args[0])] + [str(a) for a in args[1:]], capture_output=True, text=True,
+                       timeout=300, cwd=tmp_path)
+    assert r.returncode == expect, r.stdout + r.stderr
+    return r.stdout
+
+
+def test_sparse_matrix_contract(tmp_path):
+    out = run(["sparse_matrix_contract"], tmp_path)
+    assert "FAIL" not in out and "0 failures" in out, out
+
+
+def test_poisson_c0_dropin(tmp_path):
+    out = run(["poisson_dropin", 16, 10, tmp_path / "c0", "sine"], tmp_path)
+    u = np.fromfile(tmp_path / "c0_u.bin")
+    m = O.Mesh(16, O.HEX8)
+    s = O.System(m)
+    v, b = s.assemble_poisson(2, None, 1)
+    rp, ci = s.pattern()
+    x, its, _ = O.cg_eigen(rp, ci, v, b, 1e-10, 5000)
+    assert np.abs(u - x).max() < 1e-12, out
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_poisson_c1_recipe_dropin_with_adjoint(tmp_path, mixed):
+    n = 5
+    flags = ["perturb", "kappa", "adjoint"] + (["mixed"] if mixed else [])
+    out = run(["poisson_dropin", n, 12, tmp_path / "c1"] + flags, tmp_path)
+    u = np.fromfile(tmp_path / "c1_u.bin")
+    z = np.fromfile(tmp_path / "c1_z.bin")
+    grad = np.fromfile(tmp_path / "c1_grad.bin")
+    m = O.Mesh(n, O.HEX27)
+    m.perturb(0x5EED0001, 0.2)
+    kappa = O.kappa_lognormal(m.n_elem, 0x5EED0002)
+    s = O.System(m)
+    v, b = s.assemble_poisson(3, kappa, 0)
+    rp, ci = s.pattern()
+    true_res = np.linalg.norm(b - O.spmv(rp, ci, v, u)) / np.linalg.norm(b)
+    assert true_res <= 1.05e-10, out
+    ut, _, _ = O.cg_eigen(rp, ci, v, b, 1e-14, 20000)
+    assert np.linalg.norm(u - ut) / np.linalg.norm(ut) < 1e-7
+    dq = s.qoi_box_derivative(3)
+    zt, _, _ = O.cg_eigen(rp, ci, v, dq, 1e-14, 20000)
+    exact = s.adjoint_exact(3, kappa, ut, zt)
+    assert np.abs(grad - exact).max() / np.abs(exact).max() < 1e-6
+    elems = np.arange(0, m.n_elem, 11)
+    fd = s.adjoint_sensitivity(3, kappa, ut, zt, elems)
+    assert np.abs(grad[elems] - fd).max() / np.abs(fd).max() < 1e-6
+
+
+def test_degenerate_map_raises(tmp_path):
+    """An inverted element raises DegenerateMap (fe_map.C:467-505) through the C ABI."""
+    out = run(["poisson_dropin", 2, 12, tmp_path / "bad", "distort"], tmp_path, expect=3)
