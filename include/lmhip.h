@@ -174,6 +174,13 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
                     int64_t maxits, int64_t *iters, double *rel_res, int *reason, double *stats_h);
 /* Build the FP32 operator by rounding the FP64 values. */
 int lmh_matrix_make_fp32(lmh_matrix *A);
+/* Column storage of the FP32 operator: on (default) stores 16-bit column
+ * offsets against two bases per SpMV row block (6 bytes per entry with the
+ * FP32 value instead of 8) whenever every block's columns fit; off keeps the
+ * int32 CSR columns.  lmh_matrix_fp32_compression reports 1 when the
+ * compressed columns are in use, 0 otherwise (builds them if needed). */
+int lmh_matrix_set_fp32_compression(lmh_matrix *A, int on);
+int lmh_matrix_fp32_compression(lmh_matrix *A);
 
 /* ---- adjoint sensitivity (ImplicitSystem::adjoint_qoi_parameter_sensitivity,
  * implicit_system.C:517-623, for per-element kappa parameters) ---- */

@@ -82,6 +82,8 @@ HIP_API = {
     "lmh_cg_solve": (_I, [_P, _P, _P, _D, _I64, C.POINTER(_I64), C.POINTER(_D), C.POINTER(_I)]),
     "lmh_mixed_solve": (_I, [_P, _P, _P, _D, _D, _I, _I64, C.POINTER(_I64), C.POINTER(_D), C.POINTER(_I), _P]),
     "lmh_matrix_make_fp32": (_I, [_P]),
+    "lmh_matrix_set_fp32_compression": (_I, [_P, _I]),
+    "lmh_matrix_fp32_compression": (_I, [_P]),
     "lmh_qoi_box_derivative": (_I, [_P, _P, _I, _D, _D, _P]),
     "lmh_adjoint_kappa_gradient": (_I, [_P, _P, _I, _P, _P, _P, _P]),
     "lmh_profile_enable": (_I, [_P, _I]),

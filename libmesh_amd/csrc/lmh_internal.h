@@ -109,6 +109,12 @@ struct lmh_matrix {
   double *vals = nullptr;
   float *vals32 = nullptr;
   bool have32 = false;
+  // FP32 operator column compression: per row block {b0, base1, s0, -} and a
+  // 16-bit offset per entry, col = off + (off <= s0 ? b0 : base1)
+  int fp32_index = 1;         // 1: use the compressed columns when they fit
+  int idx16_state = 0;        // 0 not built, 1 built, -1 does not fit
+  uint16_t *off16 = nullptr;  // nnz
+  int4 *rbmeta = nullptr;     // nrb
   uint16_t *off = nullptr;  // n_elem*dpe*dpe slot offsets within rows
   int ncolors = 0;
   std::vector<int64_t> color_ptr;  // host, ncolors+1

@@ -9,6 +9,7 @@
 // with a deterministic "last block reduces the partials" finish, so the host
 // only polls a converged flag every few iterations.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <functional>
 
@@ -16,11 +17,6 @@
 
 namespace lmh {
 
-// solver scalar slots in ctx->d_scal
-enum {
-  S_BB = 0, S_THR, S_ABSNEW, S_ABSOLD, S_PAP, S_ALPHA, S_BETA, S_RR, S_ITS, S_DONE, S_REASON,
-  S_MAXRR = 16, S_REFRESH, S_RRTRUE, S_NREF, S_DELTA2, S_MODE, S_RZ
-};
 
 
 
@@ -103,10 +99,32 @@ __device__ inline void reduce_partials(const double *__restrict__ part, int G, d
 // writes the products to LDS, then 8 lanes per row sum each row's products in a
 // fixed order.  A single row longer than RB_CAP is one block, reduced directly.
 // epi(row, sum) consumes each row result in the calling thread.
-template <typename VT, typename XT, typename AT, typename Epi>
-__device__ inline void rowblock_spmv(const int32_t *__restrict__ rb, int64_t nrb, const int32_t *__restrict__ rp,
-                                     const int32_t *__restrict__ ci, const VT *__restrict__ v,
-                                     const XT *__restrict__ x, Epi &&epi) {
+// Column decoders: plain int32 CSR columns, or the compressed FP32-operator
+// columns (16-bit offsets against two per-block bases).
+struct PlainCols {
+  const int32_t *ci;
+  __device__ inline void begin(int64_t) {}
+  __device__ inline int32_t load(int32_t k) const { return __builtin_nontemporal_load(&ci[k]); }
+};
+struct Off16Cols {
+  const uint16_t *off;
+  const int4 *meta;
+  int32_t b0 = 0, b1 = 0, s0 = 0;
+  __device__ inline void begin(int64_t bk) {
+    const int4 m = meta[bk];
+    b0 = m.x;
+    b1 = m.y;
+    s0 = m.z;
+  }
+  __device__ inline int32_t load(int32_t k) const {
+    const int32_t o = (int32_t)__builtin_nontemporal_load(&off[k]);
+    return o + (o <= s0 ? b0 : b1);
+  }
+};
+
+template <typename VT, typename XT, typename AT, typename Epi, typename Cols = PlainCols>
+__device__ inline void rowblock_spmv_c(const int32_t *__restrict__ rb, int64_t nrb, const int32_t *__restrict__ rp,
+                                       Cols cols, const VT *__restrict__ v, const XT *__restrict__ x, Epi &&epi) {
   __shared__ AT prod[RB_CAP];
   __shared__ double red[BLK / 64];
   constexpr int PER = RB_CAP / BLK;
@@ -116,10 +134,10 @@ __device__ inline void rowblock_spmv(const int32_t *__restrict__ rb, int64_t nrb
   for (int64_t bk = lo; bk < hi; ++bk) {
     const int32_t r0 = rb[bk], r1 = rb[bk + 1];
     const int32_t k0 = rp[r0], cnt = rp[r1] - k0;
+    cols.begin(bk);
     if (cnt > RB_CAP) {
       AT s = 0;
-      for (int32_t t = tid; t < cnt; t += BLK)
-        s += (AT)__builtin_nontemporal_load(&v[k0 + t]) * (AT)x[__builtin_nontemporal_load(&ci[k0 + t])];
+      for (int32_t t = tid; t < cnt; t += BLK) s += (AT)__builtin_nontemporal_load(&v[k0 + t]) * (AT)x[cols.load(k0 + t)];
       double w = wave_sum((double)s);
       if ((tid & 63) == 0) red[tid >> 6] = w;
       __syncthreads();
@@ -141,7 +159,7 @@ __device__ inline void rowblock_spmv(const int32_t *__restrict__ rb, int64_t nrb
       const int32_t t = tid + j * BLK;
       const int32_t k = k0 + (t < cnt ? t : 0);
       vv[j] = __builtin_nontemporal_load(&v[k]);
-      cc[j] = __builtin_nontemporal_load(&ci[k]);
+      cc[j] = cols.load(k);
     }
 #pragma unroll
     for (int j = 0; j < PER; ++j) xx[j] = (AT)x[cc[j]];
@@ -163,6 +181,13 @@ __device__ inline void rowblock_spmv(const int32_t *__restrict__ rb, int64_t nrb
     }
     __syncthreads();
   }
+}
+
+template <typename VT, typename XT, typename AT, typename Epi>
+__device__ inline void rowblock_spmv(const int32_t *__restrict__ rb, int64_t nrb, const int32_t *__restrict__ rp,
+                                     const int32_t *__restrict__ ci, const VT *__restrict__ v,
+                                     const XT *__restrict__ x, Epi &&epi) {
+  rowblock_spmv_c<VT, XT, AT>(rb, nrb, rp, PlainCols{ci}, v, x, epi);
 }
 
 // ---------------------------------------------------------------------------
@@ -193,12 +218,13 @@ __global__ __launch_bounds__(BLK) void k_residual(const int32_t *__restrict__ rb
   publish<2>(acc, part);
 }
 
-__global__ __launch_bounds__(BLK) void k_fin_residual(const double *__restrict__ part, int G, double *__restrict__ S) {
+// out[0] = |r|^2, out[1] = |b|^2
+__global__ __launch_bounds__(BLK) void k_fin_residual(const double *__restrict__ part, int G, double *__restrict__ out) {
   double t[2];
   reduce_partials<2>(part, G, t);
   if (threadIdx.x == 0) {
-    S[S_RR] = t[0];
-    S[S_BB] = t[1];
+    out[0] = t[0];
+    out[1] = t[1];
   }
 }
 
@@ -223,153 +249,295 @@ __global__ void k_invdiag(int64_t n, const int32_t *__restrict__ rp, const int32
 }
 
 // ---------------------------------------------------------------------------
-// FP64 CG
+// Solver state: two banks of scalars.  Iteration k reads bank k&1 and its last
+// kernel writes bank (k+1)&1, so no block ever reads a scalar that another
+// block of the same kernel writes.  Each kernel that needs a global sum
+// reduces the previous kernel's per-block partials itself (every block, same
+// order), so an iteration is three kernels: spmv | update | p-update.
 // ---------------------------------------------------------------------------
-// after k_residual + k_fin_residual: p = dinv r (x = p = 0 when b = 0); partial r.p
-__global__ __launch_bounds__(BLK) void k_cg_init(int64_t n, const double *__restrict__ dinv,
-                                                 const double *__restrict__ r, double *__restrict__ p,
-                                                 double *__restrict__ x, double *__restrict__ part,
-                                                 const double *__restrict__ S) {
-  int64_t lo, hi;
-  block_range(n, lo, hi);
-  const bool zero_rhs = S[S_BB] == 0.0;
-  double acc[1] = {0};
-  for (int64_t i = lo + threadIdx.x; i < hi; i += BLK) {
-    if (zero_rhs) {
-      x[i] = 0.0;
-      p[i] = 0.0;
-    } else {
-      const double z = dinv[i] * r[i];
-      p[i] = z;
-      acc[0] += r[i] * z;
-    }
-  }
-  publish<1>(acc, part);
-}
+enum { B_RZ = 0, B_RR, B_ITS, B_DONE, B_REASON, B_REFRESH, B_MAXRR, B_NREF, B_SIZE = 16 };
+enum { G_THR = 32, G_BB, G_DELTA2, G_MODE, G_RRTRUE };
 
-// the early exits of ConjugateGradient.h:44-60
-__global__ __launch_bounds__(BLK) void k_fin_cg_init(const double *__restrict__ part, int G, double tol,
-                                                     double *__restrict__ S) {
-  double t[1];
-  reduce_partials<1>(part, G, t);
+template <int NV>
+__device__ inline void reduce_all(const double *__restrict__ part, int G, double (&t)[NV]) {
+  __shared__ double sh[NV][BLK / 64];
+  __shared__ double res[NV];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0;
+    for (int b = threadIdx.x; b < G; b += BLK) s += part[k * G + b];
+    s = wave_sum(s);
+    if (lane == 0) sh[k][wv] = s;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    const double bb = S[S_BB];
-    const double thr = fmax(tol * tol * bb, 2.2250738585072014e-308);
-    S[S_THR] = thr;
-    S[S_ABSNEW] = t[0];
-    S[S_ITS] = 0;
-    S[S_DONE] = 0;
-    S[S_REASON] = 0;
-    if (bb == 0.0) {
-      S[S_RR] = 0;
-      S[S_DONE] = 1;
-      S[S_REASON] = LMH_CONVERGED_RTOL;
-    } else if (S[S_RR] < thr) {
-      S[S_DONE] = 1;
-      S[S_REASON] = LMH_CONVERGED_RTOL;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0;
+      for (int w = 0; w < BLK / 64; ++w) s += sh[k][w];
+      res[k] = s;
     }
   }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) t[k] = res[k];
 }
 
-// q = A p; partial p.q  (template VT/AT: FP64 for CG, FP32 for the mixed inner solver)
-template <typename VT, typename AT>
-__global__ __launch_bounds__(BLK) void k_spmv_dot(const int32_t *__restrict__ rb, int64_t nrb,
-                                                  const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                                                  const VT *__restrict__ v, const VT *__restrict__ p,
-                                                  VT *__restrict__ q, double *__restrict__ part,
-                                                  const double *__restrict__ S) {
-  if (S[S_DONE] != 0.0) return;
+// q = A p; partials (p, q)
+template <typename VT>
+__global__ __launch_bounds__(BLK) void k_it_spmv(const int32_t *__restrict__ rb, int64_t nrb,
+                                                 const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                 const VT *__restrict__ v, const VT *__restrict__ p,
+                                                 VT *__restrict__ q, double *__restrict__ part,
+                                                 const double *__restrict__ S, int bank) {
+  if (S[bank * B_SIZE + B_DONE] != 0.0) return;
   double acc[1] = {0};
-  rowblock_spmv<VT, VT, AT>(rb, nrb, rp, ci, v, p, [&](int32_t r, AT s) {
-    q[r] = (VT)s;
+  rowblock_spmv<VT, VT, VT>(rb, nrb, rp, ci, v, p, [&](int32_t r, VT s) {
+    q[r] = s;
     acc[0] += (double)p[r] * (double)s;
   });
   publish<1>(acc, part);
 }
 
-// alpha = (r,z) / (p,Ap); slot holds (r,z): S_ABSNEW for CG, S_RZ for the mixed solver
-__global__ __launch_bounds__(BLK) void k_fin_alpha(const double *__restrict__ part, int G, int rz_slot,
-                                                   double *__restrict__ S) {
-  if (S[S_DONE] != 0.0) return;
-  double t[1];
-  reduce_partials<1>(part, G, t);
-  if (threadIdx.x == 0) {
-    const double pap = t[0];
-    S[S_PAP] = pap;
-    const double alpha = S[rz_slot] / pap;
-    S[S_ALPHA] = alpha;
-    if (!(pap != 0.0) || !isfinite(alpha)) {
-      S[S_DONE] = 1;
-      S[S_REASON] = LMH_DIVERGED_BREAKDOWN;
+// q = A32 p on the compressed-column FP32 operator; partials (p, q)
+__global__ __launch_bounds__(BLK) void k_it_spmv16(const int32_t *__restrict__ rb, int64_t nrb,
+                                                   const int32_t *__restrict__ rp, const uint16_t *__restrict__ off,
+                                                   const int4 *__restrict__ meta, const float *__restrict__ v,
+                                                   const float *__restrict__ p, float *__restrict__ q,
+                                                   double *__restrict__ part, const double *__restrict__ S, int bank) {
+  if (S[bank * B_SIZE + B_DONE] != 0.0) return;
+  double acc[1] = {0};
+  Off16Cols cols;
+  cols.off = off;
+  cols.meta = meta;
+  rowblock_spmv_c<float, float, float>(rb, nrb, rp, cols, v, p, [&](int32_t r, float s) {
+    q[r] = s;
+    acc[0] += (double)p[r] * (double)s;
+  });
+  publish<1>(acc, part);
+}
+
+__global__ __launch_bounds__(BLK) void k_spmv16(const int32_t *__restrict__ rb, int64_t nrb,
+                                                const int32_t *__restrict__ rp, const uint16_t *__restrict__ off,
+                                                const int4 *__restrict__ meta, const float *__restrict__ v,
+                                                const float *__restrict__ x, float *__restrict__ y) {
+  Off16Cols cols;
+  cols.off = off;
+  cols.meta = meta;
+  rowblock_spmv_c<float, float, float>(rb, nrb, rp, cols, v, x, [&](int32_t r, float s) { y[r] = s; });
+}
+
+// Build the compressed columns of one row block per workgroup: sort the
+// block's columns in LDS, split at the largest gap into two clusters of spans
+// s0, s1; they fit when s0 + s1 < 65535 (or the whole block spans < 65536).
+__global__ __launch_bounds__(BLK) void k_build_off16(const int32_t *__restrict__ rb, int64_t nrb,
+                                                     const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                     uint16_t *__restrict__ off, int4 *__restrict__ meta,
+                                                     int *__restrict__ bad) {
+  __shared__ int32_t keys[RB_CAP];
+  __shared__ int32_t gv[BLK], gi[BLK];
+  const int tid = threadIdx.x;
+  for (int64_t bk = blockIdx.x; bk < nrb; bk += gridDim.x) {
+    const int32_t k0 = rp[rb[bk]], cnt = rp[rb[bk + 1]] - k0;
+    if (cnt > RB_CAP) {
+      if (tid == 0) atomicOr(bad, 1);
+      continue;
     }
+    for (int t = tid; t < RB_CAP; t += BLK) keys[t] = t < cnt ? ci[k0 + t] : INT_MAX;
+    __syncthreads();
+    for (int k = 2; k <= RB_CAP; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = tid; i < RB_CAP; i += BLK) {
+          const int pj = i ^ j;
+          if (pj > i) {
+            const int32_t a = keys[i], c = keys[pj];
+            if ((a > c) == ((i & k) == 0)) {
+              keys[i] = c;
+              keys[pj] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    // largest gap between consecutive sorted columns
+    int32_t bestg = -1, besti = 0;
+    for (int i = tid; i + 1 < cnt; i += BLK) {
+      const int32_t g = keys[i + 1] - keys[i];
+      if (g > bestg) {
+        bestg = g;
+        besti = i;
+      }
+    }
+    gv[tid] = bestg;
+    gi[tid] = besti;
+    __syncthreads();
+    for (int w = BLK / 2; w > 0; w >>= 1) {
+      if (tid < w && (gv[tid + w] > gv[tid] || (gv[tid + w] == gv[tid] && gi[tid + w] < gi[tid]))) {
+        gv[tid] = gv[tid + w];
+        gi[tid] = gi[tid + w];
+      }
+      __syncthreads();
+    }
+    const int32_t cmin = cnt > 0 ? keys[0] : 0, cmax = cnt > 0 ? keys[cnt - 1] : 0;
+    int32_t b0 = cmin, b1 = cmin, s0 = 65535, split = INT_MAX;
+    bool ok = true;
+    if (cmax - cmin > 65535) {
+      const int32_t glo = keys[gi[0]], ghi = keys[gi[0] + 1];
+      s0 = glo - cmin;
+      const int32_t s1 = cmax - ghi;
+      ok = s0 + s1 + 1 <= 65535;
+      b1 = ghi - (s0 + 1);
+      split = glo;
+    }
+    if (!ok) {
+      if (tid == 0) atomicOr(bad, 1);
+    } else {
+      for (int t = tid; t < cnt; t += BLK) {
+        const int32_t c = ci[k0 + t];
+        off[k0 + t] = (uint16_t)(c <= split ? c - b0 : c - b1);
+      }
+      if (tid == 0) meta[bk] = make_int4(b0, b1, s0, 0);
+    }
+    __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(BLK) void k_cg_update(int64_t n, const double *__restrict__ dinv,
-                                                   const double *__restrict__ p, const double *__restrict__ q,
-                                                   double *__restrict__ x, double *__restrict__ r,
-                                                   double *__restrict__ part, const double *__restrict__ S) {
-  if (S[S_DONE] != 0.0) return;
+// alpha = (r,z)/(p,Ap); x += alpha p (x in XT, FP64 for both solvers);
+// r -= alpha q; partials |r|^2 and (r, dinv r)
+template <typename VT, typename XT>
+__global__ __launch_bounds__(BLK) void k_it_update(int64_t n, const double *__restrict__ part_in, int G,
+                                                   const VT *__restrict__ dinv, const VT *__restrict__ p,
+                                                   const VT *__restrict__ q, XT *__restrict__ x,
+                                                   VT *__restrict__ r, double *__restrict__ part,
+                                                   double *__restrict__ S, int bank) {
+  double *Sb = S + bank * B_SIZE;
+  if (Sb[B_DONE] != 0.0) return;
+  double t[1];
+  reduce_all<1>(part_in, G, t);
+  const double alpha = Sb[B_RZ] / t[0];
+  if (!(t[0] != 0.0) || !isfinite(alpha)) {  // every block sees the same totals
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      Sb[B_DONE] = 1;
+      Sb[B_REASON] = LMH_DIVERGED_BREAKDOWN;
+    }
+    return;
+  }
+  const VT a = (VT)alpha;
   int64_t lo, hi;
   block_range(n, lo, hi);
-  const double alpha = S[S_ALPHA];
   double acc[2] = {0, 0};
   for (int64_t i = lo + threadIdx.x; i < hi; i += BLK) {
-    x[i] += alpha * p[i];
-    const double rv = r[i] - alpha * q[i];
+    x[i] += (XT)(alpha * (double)p[i]);
+    const VT rv = r[i] - a * q[i];
     r[i] = rv;
-    acc[0] += rv * rv;
-    acc[1] += rv * (dinv[i] * rv);
+    acc[0] += (double)rv * (double)rv;
+    acc[1] += (double)rv * (double)(dinv[i] * rv);
   }
   publish<2>(acc, part);
 }
 
-// residualNorm2 test, then absOld/absNew/beta and i++ (ConjugateGradient.h:76-87)
-__global__ __launch_bounds__(BLK) void k_fin_cg_beta(const double *__restrict__ part, int G, double *__restrict__ S) {
-  if (S[S_DONE] != 0.0) return;
-  double t[2];
-  reduce_partials<2>(part, G, t);
-  if (threadIdx.x == 0) {
-    S[S_RR] = t[0];
-    if (!isfinite(t[0])) {
-      S[S_DONE] = 1;
-      S[S_REASON] = LMH_DIVERGED_NANORINF;
-    } else if (t[0] < S[S_THR]) {
-      S[S_DONE] = 1;
-      S[S_REASON] = LMH_CONVERGED_RTOL;
-    } else {
-      const double absOld = S[S_ABSNEW];
-      S[S_ABSOLD] = absOld;
-      S[S_ABSNEW] = t[1];
-      S[S_BETA] = t[1] / absOld;
-      S[S_ITS] += 1;
-    }
+// Convergence test / refresh request, then beta and p = dinv r + beta p
+// (ConjugateGradient.h:76-87).  mixed: convergence is decided only on the FP64
+// true residual; an FP32 residual below delta * (last true residual) or below
+// the threshold raises the refresh request that the host serves.
+template <typename VT>
+__global__ __launch_bounds__(BLK) void k_it_pupdate(int64_t n, const double *__restrict__ part_in, int G,
+                                                    const VT *__restrict__ dinv, const VT *__restrict__ r,
+                                                    VT *__restrict__ p, double *__restrict__ S, int bank,
+                                                    int mixed) {
+  const double *Sb = S + bank * B_SIZE;
+  double *Sn = S + (bank ^ 1) * B_SIZE;
+  if (Sb[B_DONE] != 0.0) {
+    if (blockIdx.x == 0 && threadIdx.x < B_SIZE) Sn[threadIdx.x] = Sb[threadIdx.x];
+    return;
   }
-}
-
-__global__ __launch_bounds__(BLK) void k_cg_pupdate(int64_t n, const double *__restrict__ dinv,
-                                                    const double *__restrict__ r, double *__restrict__ p,
-                                                    const double *__restrict__ S) {
-  if (S[S_DONE] != 0.0) return;
+  double t[2];
+  reduce_all<2>(part_in, G, t);
+  const double rr = t[0], rz = t[1];
+  bool done = false;
+  int reason = 0;
+  double refresh = Sb[B_REFRESH];
+  if (!isfinite(rr)) {
+    done = true;
+    reason = LMH_DIVERGED_NANORINF;
+  } else if (!mixed && rr < S[G_THR]) {
+    done = true;
+    reason = LMH_CONVERGED_RTOL;
+  } else if (mixed && (rr < S[G_DELTA2] * Sb[B_MAXRR] || rr < S[G_THR])) {
+    refresh = 1;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    Sn[B_RR] = rr;
+    Sn[B_DONE] = done ? 1.0 : 0.0;
+    Sn[B_REASON] = reason;
+    Sn[B_ITS] = Sb[B_ITS] + (done ? 0.0 : 1.0);
+    Sn[B_RZ] = done ? Sb[B_RZ] : rz;
+    Sn[B_REFRESH] = refresh;
+    Sn[B_MAXRR] = Sb[B_MAXRR];
+    Sn[B_NREF] = Sb[B_NREF];
+  }
+  if (done) return;
+  const VT beta = (VT)(rz / Sb[B_RZ]);
   int64_t lo, hi;
   block_range(n, lo, hi);
-  const double beta = S[S_BETA];
   for (int64_t i = lo + threadIdx.x; i < hi; i += BLK) p[i] = dinv[i] * r[i] + beta * p[i];
 }
 
+// FP64 CG start: after k_residual (r = b - A x), p = dinv r; partials (r, p).
+__global__ __launch_bounds__(BLK) void k_cg_init(int64_t n, const double *__restrict__ dinv,
+                                                 const double *__restrict__ r, double *__restrict__ p,
+                                                 double *__restrict__ part) {
+  int64_t lo, hi;
+  block_range(n, lo, hi);
+  double acc[1] = {0};
+  for (int64_t i = lo + threadIdx.x; i < hi; i += BLK) {
+    const double z = dinv[i] * r[i];
+    p[i] = z;
+    acc[0] += r[i] * z;
+  }
+  publish<1>(acc, part);
+}
+
+// the early exits of ConjugateGradient.h:44-60; rr, bb from k_residual's partials
+__global__ __launch_bounds__(BLK) void k_cg_init_fin(const double *__restrict__ part_res,
+                                                     const double *__restrict__ part_rz, int G, double tol,
+                                                     double *__restrict__ S, double *__restrict__ x, int64_t n) {
+  double t[2], u[1];
+  reduce_all<2>(part_res, G, t);
+  reduce_all<1>(part_rz, G, u);
+  const double rr = t[0], bb = t[1];
+  if (bb == 0.0)  // b = 0: x = 0 (this kernel runs with a full grid for that case)
+    for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) x[i] = 0.0;
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const double thr = fmax(tol * tol * bb, 2.2250738585072014e-308);
+  S[G_THR] = thr;
+  S[G_BB] = bb;
+  double *B0 = S;
+  for (int k = 0; k < B_SIZE; ++k) B0[k] = 0.0;
+  B0[B_RZ] = u[0];
+  B0[B_RR] = rr;
+  if (bb == 0.0) {
+    B0[B_RR] = 0;
+    B0[B_DONE] = 1;
+    B0[B_REASON] = LMH_CONVERGED_RTOL;
+  } else if (rr < thr) {
+    B0[B_DONE] = 1;
+    B0[B_REASON] = LMH_CONVERGED_RTOL;
+  }
+}
+
 // ---------------------------------------------------------------------------
-// Mixed precision.  FP32 recurrence (x_s, r_s, p_s, q_s) on the FP32 operator;
-// FP64 solution x and true residual r = b - A x on the FP64 operator.
+// Mixed precision refresh: r64 = b - A64 x (FP64 operator); r32 = (float) r64;
+// z = dinv32 r32 (p32 = z on the initial call); partials |r|^2, |b|^2, (r32, z).
 // ---------------------------------------------------------------------------
-// r64 = b - A64 x; r32 = (float) r64; z = dinv32 r32 (p32 = z on the initial
-// call); partials |r|^2, |b|^2, (r32, z).  Used initially and on every refresh.
-__global__ __launch_bounds__(BLK) void k_mx_residual(int init, const int32_t *__restrict__ rb, int64_t nrb,
-                                                     const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                                                     const double *__restrict__ v, const double *__restrict__ b,
-                                                     const double *__restrict__ x, const float *__restrict__ dinv32,
-                                                     float *__restrict__ r32, float *__restrict__ p32,
-                                                     double *__restrict__ part, const double *__restrict__ S) {
-  if (!init && (S[S_DONE] != 0.0 || S[S_REFRESH] == 0.0)) return;
+__global__ __launch_bounds__(BLK) void k_refresh(int init, const int32_t *__restrict__ rb, int64_t nrb,
+                                                 const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                 const double *__restrict__ v, const double *__restrict__ b,
+                                                 const double *__restrict__ x, const float *__restrict__ dinv32,
+                                                 float *__restrict__ r32, float *__restrict__ p32,
+                                                 double *__restrict__ part, const double *__restrict__ S, int bank) {
+  if (!init && S[bank * B_SIZE + B_DONE] != 0.0) return;
   double acc[3] = {0, 0, 0};
   rowblock_spmv<double, double, double>(rb, nrb, rp, ci, v, x, [&](int32_t row, double s) {
     const double bv = b[row], rv = bv - s;
@@ -384,108 +552,54 @@ __global__ __launch_bounds__(BLK) void k_mx_residual(int init, const int32_t *__
   publish<3>(acc, part);
 }
 
-__global__ __launch_bounds__(BLK) void k_fin_mx_residual(const double *__restrict__ part, int G, int init,
-                                                         double *__restrict__ S) {
-  if (!init && (S[S_DONE] != 0.0 || S[S_REFRESH] == 0.0)) return;
+// One block.  init: |b|^2, threshold and the starting bank; otherwise the
+// FP64 convergence test on the true residual and the restart of the FP32
+// recurrence's (r, z) for the next alpha.
+__global__ __launch_bounds__(BLK) void k_refresh_fin(const double *__restrict__ part, int G, int init,
+                                                     double *__restrict__ S, int bank) {
   double t[3];
   reduce_partials<3>(part, G, t);
   if (threadIdx.x != 0) return;
+  double *Sb = S + bank * B_SIZE;
+  if (!init && Sb[B_DONE] != 0.0) return;
   const double rr = t[0];
-  S[S_RRTRUE] = rr;
+  S[G_RRTRUE] = rr;
   if (init) {
-    const double tol2 = S[S_THR];  // rtol^2 on entry
-    S[S_BB] = t[1];
-    S[S_THR] = tol2 * t[1];
-    S[S_ITS] = 0;
-    S[S_NREF] = 0;
-    S[S_DONE] = 0;
-    S[S_REASON] = 0;
-    S[S_REFRESH] = 0;
-    S[S_RZ] = t[2];
-    S[S_MAXRR] = rr;
-    if (t[1] == 0.0 || rr <= S[S_THR]) {
-      S[S_DONE] = 1;
-      S[S_REASON] = LMH_CONVERGED_RTOL;
+    const double tol2 = S[G_THR];  // rtol^2 on entry
+    S[G_BB] = t[1];
+    S[G_THR] = tol2 * t[1];
+    for (int k = 0; k < B_SIZE; ++k) Sb[k] = 0.0;
+    Sb[B_RZ] = t[2];
+    Sb[B_RR] = rr;
+    Sb[B_MAXRR] = rr;
+    if (t[1] == 0.0 || rr <= S[G_THR]) {
+      Sb[B_DONE] = 1;
+      Sb[B_REASON] = LMH_CONVERGED_RTOL;
     }
     return;
   }
-  S[S_NREF] += 1;
-  if (rr <= S[S_THR]) {
-    S[S_DONE] = 1;
-    S[S_REASON] = LMH_CONVERGED_RTOL;
+  Sb[B_NREF] += 1;
+  Sb[B_REFRESH] = 0;
+  if (rr <= S[G_THR]) {
+    Sb[B_DONE] = 1;
+    Sb[B_REASON] = LMH_CONVERGED_RTOL;
   } else if (!isfinite(rr)) {
-    S[S_DONE] = 1;
-    S[S_REASON] = LMH_DIVERGED_NANORINF;
+    Sb[B_DONE] = 1;
+    Sb[B_REASON] = LMH_DIVERGED_NANORINF;
   } else {
-    // keep the search direction (mode 0, reliable update) or restart (mode 1)
-    S[S_MAXRR] = rr;
-    S[S_BETA] = (S[S_MODE] == 0.0) ? t[2] / S[S_RZ] : 0.0;
-    S[S_RZ] = t[2];
+    Sb[B_MAXRR] = rr;
+    Sb[B_RZ] = t[2];
   }
 }
 
-__global__ __launch_bounds__(BLK) void k_mx_update(int64_t n, const float *__restrict__ dinv32,
-                                                   const float *__restrict__ p, const float *__restrict__ q,
-                                                   float *__restrict__ xs, float *__restrict__ r,
-                                                   double *__restrict__ part, const double *__restrict__ S) {
-  if (S[S_DONE] != 0.0) return;
+// restarted refinement (mode 1): the correction solve starts over, p = dinv r
+__global__ __launch_bounds__(BLK) void k_restart_p(int64_t n, const float *__restrict__ dinv32,
+                                                   const float *__restrict__ r32, float *__restrict__ p32,
+                                                   const double *__restrict__ S, int bank) {
+  if (S[bank * B_SIZE + B_DONE] != 0.0) return;
   int64_t lo, hi;
   block_range(n, lo, hi);
-  const float alpha = (float)S[S_ALPHA];
-  double acc[2] = {0, 0};
-  for (int64_t i = lo + threadIdx.x; i < hi; i += BLK) {
-    xs[i] += alpha * p[i];
-    const float rv = r[i] - alpha * q[i];
-    r[i] = rv;
-    const float z = dinv32[i] * rv;
-    acc[0] += (double)rv * (double)rv;
-    acc[1] += (double)rv * (double)z;
-  }
-  publish<2>(acc, part);
-}
-
-// refresh when the FP32 residual fell below delta times the last true residual
-// (or below the final threshold); otherwise the ordinary CG beta
-__global__ __launch_bounds__(BLK) void k_fin_mx_update(const double *__restrict__ part, int G,
-                                                       double *__restrict__ S) {
-  if (S[S_DONE] != 0.0) return;
-  double t[2];
-  reduce_partials<2>(part, G, t);
-  if (threadIdx.x != 0) return;
-  S[S_ITS] += 1;
-  const double rr = t[0];
-  if (!isfinite(rr)) {
-    S[S_DONE] = 1;
-    S[S_REASON] = LMH_DIVERGED_NANORINF;
-  } else if (rr < S[S_DELTA2] * S[S_MAXRR] || rr < S[S_THR]) {
-    S[S_REFRESH] = 1;
-  } else {
-    S[S_REFRESH] = 0;
-    S[S_BETA] = t[1] / S[S_RZ];
-    S[S_RZ] = t[1];
-  }
-}
-
-// x += xs; xs = 0 (on a refresh, or when forced)
-__global__ __launch_bounds__(BLK) void k_mx_fold(int64_t n, int force, float *__restrict__ xs, double *__restrict__ x,
-                                                 const double *__restrict__ S) {
-  if (!force && (S[S_DONE] != 0.0 || S[S_REFRESH] == 0.0)) return;
-  int64_t lo, hi;
-  block_range(n, lo, hi);
-  for (int64_t i = lo + threadIdx.x; i < hi; i += BLK) {
-    x[i] += (double)xs[i];
-    xs[i] = 0.0f;
-  }
-}
-
-__global__ __launch_bounds__(BLK) void k_mx_pupdate(int64_t n, const float *__restrict__ dinv32,
-                                                    const float *__restrict__ r, float *__restrict__ p,
-                                                    const double *__restrict__ S) {
-  if (S[S_DONE] != 0.0) return;
-  int64_t lo, hi;
-  block_range(n, lo, hi);
-  const float beta = (float)S[S_BETA];
-  for (int64_t i = lo + threadIdx.x; i < hi; i += BLK) p[i] = dinv32[i] * r[i] + beta * p[i];
+  for (int64_t i = lo + threadIdx.x; i < hi; i += BLK) p32[i] = dinv32[i] * r32[i];
 }
 
 __global__ void k_round32(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
@@ -513,9 +627,34 @@ static int vec_grid(lmh_context *ctx) {
   return g < ctx->n_part ? g : ctx->n_part;
 }
 
+// compressed FP32-operator columns, built once per pattern
+static int ensure_off16(lmh_matrix *A) {
+  if (A->idx16_state != 0 || !A->fp32_index) return LMH_OK;
+  lmh_context *ctx = A->ctx;
+  LMH_HIP(hipMalloc((void **)&A->off16, sizeof(uint16_t) * (A->nnz ? A->nnz : 1)));
+  LMH_HIP(hipMalloc((void **)&A->rbmeta, sizeof(int4) * (A->nrb ? A->nrb : 1)));
+  LMH_HIP(hipMemsetAsync(ctx->d_flags, 0, sizeof(int), ctx->stream));
+  LMH_LAUNCH(ctx, "build_off16", k_build_off16, dim3(grid_for(A->nrb, 1, 16384)), dim3(BLK), 0, A->rb, A->nrb, A->rp,
+             A->ci, A->off16, A->rbmeta, ctx->d_flags);
+  LMH_TRY(check_launch("k_build_off16"));
+  int bad = 0;
+  LMH_HIP(hipMemcpyAsync(&bad, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  LMH_HIP(hipStreamSynchronize(ctx->stream));
+  A->idx16_state = bad ? -1 : 1;
+  if (bad) {
+    LMH_HIP(hipFree(A->off16));
+    LMH_HIP(hipFree(A->rbmeta));
+    A->off16 = nullptr;
+    A->rbmeta = nullptr;
+  }
+  return LMH_OK;
+}
+
+static bool use_off16(const lmh_matrix *A) { return A->fp32_index && A->idx16_state == 1; }
+
 static int ensure_work(lmh_matrix *A, bool mixed) {
   if (!A->w64) LMH_HIP(hipMalloc((void **)&A->w64, sizeof(double) * 4 * A->rows));
-  if (mixed && !A->w32) LMH_HIP(hipMalloc((void **)&A->w32, sizeof(float) * 5 * A->rows));
+  if (mixed && !A->w32) LMH_HIP(hipMalloc((void **)&A->w32, sizeof(float) * 4 * A->rows));
   return LMH_OK;
 }
 
@@ -539,6 +678,12 @@ int lmh_spmv_fp32(lmh_matrix *A, const float *x_d, float *y_d) {
   LMH_CHECK(A->have32 && A->vals32, LMH_ERR_INVALID, "matrix has no FP32 operator");
   lmh_context *ctx = A->ctx;
   const int g = vec_grid(ctx);
+  LMH_TRY(ensure_off16(A));
+  if (use_off16(A)) {
+    LMH_LAUNCH(ctx, "spmv_fp32", k_spmv16, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->off16, A->rbmeta,
+               A->vals32, x_d, y_d);
+    return check_launch("k_spmv16");
+  }
   LMH_LAUNCH(ctx, "spmv_fp32", (k_spmv<float, float, float, float>), dim3(g),
                                           dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals32, x_d, y_d);
   return check_launch("k_spmv");
@@ -550,7 +695,7 @@ int lmh_residual(lmh_matrix *A, const double *b_d, const double *x_d, double *r_
   const int g = vec_grid(ctx);
   LMH_LAUNCH(ctx, "residual_fp64", k_residual, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp,
                                           A->ci, A->vals, b_d, x_d, r_d, ctx->d_part);
-  LMH_LAUNCH(ctx, "fin", k_fin_residual, dim3(1), dim3(BLK), 0, ctx->d_part, g, ctx->d_scal + 32);
+  LMH_LAUNCH(ctx, "fin", k_fin_residual, dim3(1), dim3(BLK), 0, ctx->d_part, g, ctx->d_scal + 48);
   return check_launch("k_residual");
 }
 
@@ -558,9 +703,9 @@ double lmh_norm2(lmh_context *ctx, const double *x_d, int64_t n) {
   if (!ctx || !x_d) return NAN;
   const int g = vec_grid(ctx);
   hipLaunchKernelGGL(k_norm2, dim3(g), dim3(BLK), 0, ctx->stream, n, x_d, ctx->d_part);
-  hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(BLK), 0, ctx->stream, ctx->d_part, g, ctx->d_scal + 48);
+  hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(BLK), 0, ctx->stream, ctx->d_part, g, ctx->d_scal + 56);
   double v = NAN;
-  if (hipMemcpyAsync(&v, ctx->d_scal + 48, sizeof(double), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+  if (hipMemcpyAsync(&v, ctx->d_scal + 56, sizeof(double), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
     return NAN;
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) return NAN;
   return std::sqrt(v);
@@ -576,32 +721,11 @@ int lmh_matrix_make_fp32(lmh_matrix *A) {
   return check_launch("k_round32");
 }
 
-// Host driver: launches iterations in batches and polls the device "done" flag
-// of the previous batch (one batch of lag) through pinned memory.
-static int run_batches(lmh_context *ctx, int64_t maxits, int batch, const std::function<void()> &one_iter) {
-  double *flag = ctx->h_pinned;
-  hipEvent_t ev[2] = {get_event(ctx), get_event(ctx)};
-  int64_t launched = 0;
-  int k = 0;
-  bool pending = false;
-  while (launched < maxits) {
-    const int64_t nb = std::min<int64_t>(batch, maxits - launched);
-    for (int64_t i = 0; i < nb; ++i) one_iter();
-    launched += nb;
-    LMH_HIP(hipMemcpyAsync(&flag[k], ctx->d_scal + S_DONE, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    LMH_HIP(hipEventRecord(ev[k], ctx->stream));
-    if (pending) {
-      LMH_HIP(hipEventSynchronize(ev[k ^ 1]));
-      if (flag[k ^ 1] != 0.0) break;
-    }
-    pending = true;
-    k ^= 1;
-  }
-  LMH_HIP(hipStreamSynchronize(ctx->stream));
-  ctx->event_pool.push_back(ev[0]);
-  ctx->event_pool.push_back(ev[1]);
-  return check_launch("solver iteration");
-}
+// Poll slots in pinned memory: the two banks are copied after every batch.
+struct Poll {
+  hipEvent_t ev = nullptr;
+  int64_t end = 0;  // iterations launched when the copy was enqueued
+};
 
 int lmh_cg_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int64_t maxits, int64_t *iters,
                  double *rel_res, int *reason) {
@@ -611,31 +735,57 @@ int lmh_cg_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int
   LMH_TRY(ensure_work(A, false));
   const int64_t n = A->rows;
   double *r = A->w64, *p = r + n, *q = p + n, *dinv = q + n;
-  double *S = ctx->d_scal, *part = ctx->d_part;
+  double *S = ctx->d_scal;
+  double *part1 = ctx->d_part, *part2 = ctx->d_part + 2 * ctx->n_part;  // 1 and 2 values per block
   const int g = vec_grid(ctx);
   LMH_LAUNCH(ctx, "invdiag", (k_invdiag<double, double>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, n, A->rp, A->ci,
              A->vals, dinv);
-  LMH_LAUNCH(ctx, "residual_fp64", k_residual, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals,
-                                b_d, x_d, r, part);
-  LMH_LAUNCH(ctx, "fin", k_fin_residual, dim3(1), dim3(BLK), 0, part, g, S);
-  LMH_LAUNCH(ctx, "cg_init", k_cg_init, dim3(g), dim3(BLK), 0, n, dinv, r, p, x_d, part, S);
-  LMH_LAUNCH(ctx, "fin", k_fin_cg_init, dim3(1), dim3(BLK), 0, part, g, rtol, S);
+  LMH_LAUNCH(ctx, "residual_fp64", k_residual, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals, b_d, x_d,
+             r, part2);
+  LMH_LAUNCH(ctx, "cg_init", k_cg_init, dim3(g), dim3(BLK), 0, n, dinv, r, p, part1);
+  LMH_LAUNCH(ctx, "cg_init", k_cg_init_fin, dim3(g), dim3(BLK), 0, part2, part1, g, rtol, S, x_d, n);
   LMH_TRY(check_launch("cg setup"));
-  auto iter = [&]() {
-    LMH_LAUNCH(ctx, "cg_spmv_fp64", (k_spmv_dot<double, double>), dim3(g), dim3(BLK), 0, A->rb,
-                                  A->nrb, A->rp, A->ci, A->vals, p, q, part, S);
-    LMH_LAUNCH(ctx, "fin", k_fin_alpha, dim3(1), dim3(BLK), 0, part, g, (int)S_ABSNEW, S);
-    LMH_LAUNCH(ctx, "cg_update_fp64", k_cg_update, dim3(g), dim3(BLK), 0, n, dinv, p, q, x_d, r, part, S);
-    LMH_LAUNCH(ctx, "fin", k_fin_cg_beta, dim3(1), dim3(BLK), 0, part, g, S);
-    LMH_LAUNCH(ctx, "cg_pupdate_fp64", k_cg_pupdate, dim3(g), dim3(BLK), 0, n, dinv, r, p, S);
-  };
-  LMH_TRY(run_batches(ctx, maxits, 4, iter));
-  double h[16];
+  const int batch = 4;
+  Poll poll[2];
+  double *snap = ctx->h_pinned;  // 2 x (2 banks)
+  int64_t K = 0;
+  int slot = 0;
+  bool pending = false, done = false;
+  while (K < maxits && !done) {
+    const int64_t nb = std::min<int64_t>(batch, maxits - K);
+    for (int64_t i = 0; i < nb; ++i, ++K) {
+      const int bank = (int)(K & 1);
+      LMH_LAUNCH(ctx, "cg_spmv_fp64", k_it_spmv<double>, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals,
+                 p, q, part1, S, bank);
+      LMH_LAUNCH(ctx, "cg_update_fp64", (k_it_update<double, double>), dim3(g), dim3(BLK), 0, n, part1, g, dinv, p, q,
+                 x_d, r, part2, S, bank);
+      LMH_LAUNCH(ctx, "cg_pupdate_fp64", k_it_pupdate<double>, dim3(g), dim3(BLK), 0, n, part2, g, dinv, r, p, S, bank,
+                 0);
+    }
+    LMH_HIP(hipMemcpyAsync(snap + 32 * slot, S, sizeof(double) * 2 * B_SIZE, hipMemcpyDeviceToHost, ctx->stream));
+    if (!poll[slot].ev) poll[slot].ev = get_event(ctx);
+    LMH_HIP(hipEventRecord(poll[slot].ev, ctx->stream));
+    poll[slot].end = K;
+    if (pending) {
+      const int o = slot ^ 1;
+      LMH_HIP(hipEventSynchronize(poll[o].ev));
+      done = snap[32 * o + (poll[o].end & 1) * B_SIZE + B_DONE] != 0.0;
+    }
+    pending = true;
+    slot ^= 1;
+  }
+  LMH_HIP(hipStreamSynchronize(ctx->stream));
+  LMH_TRY(check_launch("cg iterations"));
+  for (auto &pl : poll)
+    if (pl.ev) ctx->event_pool.push_back(pl.ev);
+  double h[2 * B_SIZE], gbb = 0;
   LMH_HIP(hipMemcpy(h, S, sizeof(h), hipMemcpyDeviceToHost));
-  const bool done = h[S_DONE] != 0.0;
-  if (iters) *iters = done ? (int64_t)h[S_ITS] : maxits;
-  if (rel_res) *rel_res = h[S_BB] > 0 ? std::sqrt(h[S_RR] / h[S_BB]) : 0.0;
-  if (reason) *reason = done ? (int)h[S_REASON] : LMH_DIVERGED_ITS;
+  LMH_HIP(hipMemcpy(&gbb, S + G_BB, sizeof(double), hipMemcpyDeviceToHost));
+  const double *Bk = h + (K & 1) * B_SIZE;
+  const bool conv = Bk[B_DONE] != 0.0;
+  if (iters) *iters = conv ? (int64_t)Bk[B_ITS] : maxits;
+  if (rel_res) *rel_res = gbb > 0 ? std::sqrt(Bk[B_RR] / gbb) : 0.0;
+  if (reason) *reason = conv ? (int)Bk[B_REASON] : LMH_DIVERGED_ITS;
   return LMH_OK;
 }
 
@@ -647,56 +797,108 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
   LMH_CHECK(rtol > 0 && inner_rtol > 0 && inner_rtol < 1, LMH_ERR_INVALID, "bad tolerances");
   lmh_context *ctx = A->ctx;
   LMH_TRY(ensure_work(A, true));
+  LMH_TRY(ensure_off16(A));
+  const bool c16 = use_off16(A);
   const int64_t n = A->rows;
-  float *xs = A->w32, *r32 = xs + n, *p32 = r32 + n, *q32 = p32 + n, *d32 = q32 + n;
-  double *S = ctx->d_scal, *part = ctx->d_part;
+  float *r32 = A->w32, *p32 = r32 + n, *q32 = p32 + n, *d32 = q32 + n;
+  double *S = ctx->d_scal;
+  double *part1 = ctx->d_part, *part2 = ctx->d_part + 2 * ctx->n_part;
   const int g = vec_grid(ctx);
-  double init[32] = {0};
-  init[S_THR] = rtol * rtol;
-  init[S_DELTA2] = inner_rtol * inner_rtol;
-  init[S_MODE] = mode;
+  double init[40] = {0};
+  init[G_THR] = rtol * rtol;
+  init[G_DELTA2] = inner_rtol * inner_rtol;
+  init[G_MODE] = mode;
   LMH_HIP(hipMemcpyAsync(S, init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
-  LMH_HIP(hipMemsetAsync(xs, 0, sizeof(float) * n, ctx->stream));
   LMH_LAUNCH(ctx, "invdiag", (k_invdiag<float, float>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, n, A->rp, A->ci,
              A->vals32, d32);
-  LMH_LAUNCH(ctx, "mx_residual_fp64", k_mx_residual, dim3(g), dim3(BLK), 0, 1, A->rb, A->nrb, A->rp, A->ci,
-                                A->vals, b_d, x_d, d32, r32, p32, part, S);
-  LMH_LAUNCH(ctx, "fin", k_fin_mx_residual, dim3(1), dim3(BLK), 0, part, g, 1, S);
+  LMH_LAUNCH(ctx, "mx_refresh_fp64", k_refresh, dim3(g), dim3(BLK), 0, 1, A->rb, A->nrb, A->rp, A->ci, A->vals, b_d,
+             x_d, d32, r32, p32, part2, S, 0);
+  LMH_LAUNCH(ctx, "fin", k_refresh_fin, dim3(1), dim3(BLK), 0, part2, g, 1, S, 0);
   LMH_TRY(check_launch("mixed setup"));
-  auto iter = [&]() {
-    LMH_LAUNCH(ctx, "mx_spmv_fp32", (k_spmv_dot<float, float>), dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp,
-                                  A->ci, A->vals32, p32, q32, part, S);
-    LMH_LAUNCH(ctx, "fin", k_fin_alpha, dim3(1), dim3(BLK), 0, part, g, (int)S_RZ, S);
-    LMH_LAUNCH(ctx, "mx_update_fp32", k_mx_update, dim3(g), dim3(BLK), 0, n, d32, p32, q32, xs, r32, part, S);
-    LMH_LAUNCH(ctx, "fin", k_fin_mx_update, dim3(1), dim3(BLK), 0, part, g, S);
-    LMH_LAUNCH(ctx, "mx_fold", k_mx_fold, dim3(g), dim3(BLK), 0, n, 0, xs, x_d, S);
-    LMH_LAUNCH(ctx, "mx_residual_fp64", k_mx_residual, dim3(g), dim3(BLK), 0, 0, A->rb, A->nrb, A->rp, A->ci,
-                                  A->vals, b_d, x_d, d32, r32, p32, part, S);
-    LMH_LAUNCH(ctx, "fin", k_fin_mx_residual, dim3(1), dim3(BLK), 0, part, g, 0, S);
-    LMH_LAUNCH(ctx, "mx_pupdate_fp32", k_mx_pupdate, dim3(g), dim3(BLK), 0, n, d32, r32, p32, S);
+  auto refresh = [&](int bank) {
+    LMH_LAUNCH(ctx, "mx_refresh_fp64", k_refresh, dim3(g), dim3(BLK), 0, 0, A->rb, A->nrb, A->rp, A->ci, A->vals, b_d,
+               x_d, d32, r32, p32, part2, S, bank);
+    LMH_LAUNCH(ctx, "fin", k_refresh_fin, dim3(1), dim3(BLK), 0, part2, g, 0, S, bank);
+    if (mode == 1) LMH_LAUNCH(ctx, "mx_restart", k_restart_p, dim3(g), dim3(BLK), 0, n, d32, r32, p32, S, bank);
   };
-  LMH_TRY(run_batches(ctx, maxits, 4, iter));
-  double h[32];
-  LMH_HIP(hipMemcpy(h, S, sizeof(h), hipMemcpyDeviceToHost));
-  const bool done = h[S_DONE] != 0.0;
-  if (!done) {
-    // iteration limit: fold the pending correction and report the true residual
-    LMH_LAUNCH(ctx, "mx_fold", k_mx_fold, dim3(g), dim3(BLK), 0, n, 1, xs, x_d, S);
-    LMH_LAUNCH(ctx, "residual_fp64", k_residual, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals,
-                                  b_d, x_d, A->w64, part);
-    LMH_LAUNCH(ctx, "fin", k_fin_residual, dim3(1), dim3(BLK), 0, part, g, ctx->d_scal + 32);
-    LMH_TRY(check_launch("mixed finish"));
-    LMH_HIP(hipMemcpy(&h[S_RRTRUE], ctx->d_scal + 32 + S_RR, sizeof(double), hipMemcpyDeviceToHost));
+  const int batch = 4;
+  Poll poll[2];
+  double *snap = ctx->h_pinned;
+  int64_t K = 0, last_refresh = 0;
+  int slot = 0;
+  bool pending = false, done = false;
+  while (K < maxits && !done) {
+    const int64_t nb = std::min<int64_t>(batch, maxits - K);
+    for (int64_t i = 0; i < nb; ++i, ++K) {
+      const int bank = (int)(K & 1);
+      if (c16)
+        LMH_LAUNCH(ctx, "mx_spmv_fp32", k_it_spmv16, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->off16, A->rbmeta,
+                   A->vals32, p32, q32, part1, S, bank);
+      else
+        LMH_LAUNCH(ctx, "mx_spmv_fp32", k_it_spmv<float>, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci,
+                   A->vals32, p32, q32, part1, S, bank);
+      LMH_LAUNCH(ctx, "mx_update_fp32", (k_it_update<float, double>), dim3(g), dim3(BLK), 0, n, part1, g, d32, p32, q32,
+                 x_d, r32, part2, S, bank);
+      LMH_LAUNCH(ctx, "mx_pupdate_fp32", k_it_pupdate<float>, dim3(g), dim3(BLK), 0, n, part2, g, d32, r32, p32, S,
+                 bank, 1);
+    }
+    LMH_HIP(hipMemcpyAsync(snap + 32 * slot, S, sizeof(double) * 2 * B_SIZE, hipMemcpyDeviceToHost, ctx->stream));
+    if (!poll[slot].ev) poll[slot].ev = get_event(ctx);
+    LMH_HIP(hipEventRecord(poll[slot].ev, ctx->stream));
+    poll[slot].end = K;
+    if (pending) {
+      const int o = slot ^ 1;
+      LMH_HIP(hipEventSynchronize(poll[o].ev));
+      const double *Bo = snap + 32 * o + (poll[o].end & 1) * B_SIZE;
+      done = Bo[B_DONE] != 0.0;
+      // serve a refresh request raised after the last refresh was enqueued
+      if (!done && Bo[B_REFRESH] != 0.0 && poll[o].end > last_refresh) {
+        refresh((int)(K & 1));
+        last_refresh = K;
+      }
+    }
+    pending = true;
+    slot ^= 1;
   }
-  if (iters) *iters = (int64_t)h[S_ITS];
-  if (rel_res) *rel_res = h[S_BB] > 0 ? std::sqrt(h[S_RRTRUE] / h[S_BB]) : 0.0;
-  if (reason) *reason = done ? (int)h[S_REASON] : LMH_DIVERGED_ITS;
+  LMH_HIP(hipStreamSynchronize(ctx->stream));
+  LMH_TRY(check_launch("mixed iterations"));
+  for (auto &pl : poll)
+    if (pl.ev) ctx->event_pool.push_back(pl.ev);
+  double h[2 * B_SIZE], gl[8];
+  LMH_HIP(hipMemcpy(h, S, sizeof(h), hipMemcpyDeviceToHost));
+  const double *Bk = h + (K & 1) * B_SIZE;
+  bool conv = Bk[B_DONE] != 0.0;
+  if (!conv) {
+    // iteration limit: report the FP64 true residual of the current x
+    LMH_LAUNCH(ctx, "residual_fp64", k_residual, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals, b_d, x_d,
+               A->w64, part2);
+    LMH_LAUNCH(ctx, "fin", k_fin_residual, dim3(1), dim3(BLK), 0, part2, g, S + G_RRTRUE);
+    LMH_TRY(check_launch("mixed finish"));
+  }
+  LMH_HIP(hipMemcpy(gl, S + G_THR, sizeof(gl), hipMemcpyDeviceToHost));
+  const double bb = gl[G_BB - G_THR], rrt = gl[G_RRTRUE - G_THR];
+  const double res = bb > 0 ? std::sqrt(rrt / bb) : 0.0;
+  if (iters) *iters = (int64_t)Bk[B_ITS];
+  if (rel_res) *rel_res = res;
+  if (reason) *reason = conv ? (int)Bk[B_REASON] : LMH_DIVERGED_ITS;
   if (stats_h) {
-    stats_h[0] = h[S_ITS];
-    stats_h[1] = h[S_NREF];
-    stats_h[2] = h[S_BB] > 0 ? std::sqrt(h[S_RRTRUE] / h[S_BB]) : 0.0;
+    stats_h[0] = Bk[B_ITS];
+    stats_h[1] = Bk[B_NREF];
+    stats_h[2] = res;
   }
   return LMH_OK;
+}
+
+int lmh_matrix_set_fp32_compression(lmh_matrix *A, int on) {
+  LMH_CHECK(A, LMH_ERR_INVALID, "null matrix");
+  A->fp32_index = on ? 1 : 0;
+  return LMH_OK;
+}
+
+int lmh_matrix_fp32_compression(lmh_matrix *A) {
+  if (!A) return -1;
+  if (ensure_off16(A) != LMH_OK) return -1;
+  return use_off16(A) ? 1 : 0;
 }
 
 }  // extern "C"

@@ -269,6 +269,8 @@ int lmh_matrix_destroy(lmh_matrix *A) {
   (void)hipFree(A->rp);
   (void)hipFree(A->ci);
   (void)hipFree(A->rb);
+  (void)hipFree(A->off16);
+  (void)hipFree(A->rbmeta);
   (void)hipFree(A->vals);
   (void)hipFree(A->vals32);
   (void)hipFree(A->off);
