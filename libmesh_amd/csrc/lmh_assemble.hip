@@ -725,7 +725,7 @@ int lmh_assemble_poisson(lmh_matrix *A, const lmh_mesh *m, const lmh_dofmap *dm,
   } else {
     LMH_CHECK(precision == LMH_FP32, LMH_ERR_INVALID, "unknown precision %d", precision);
     if (!A->Ke32) LMH_HIP(hipMalloc((void **)&A->Ke32, sizeof(float) * ne * dd));
-    if (!A->vals32) LMH_HIP(hipMalloc((void **)&A->vals32, sizeof(float) * A->nnz));
+    if (!A->vals32) LMH_HIP(hipMalloc((void **)&A->vals32, sizeof(float) * (A->nnz + LMH_PAD)));
     LMH_TRY(element_poisson_impl<float>(ctx, m, dm, npts, kappa_d, f_kind, 1, A->Ke32, A->Fe));
     LMH_TRY(scatter_impl<float>(A, A->Ke32, A->Fe, A->vals32, rhs_d, scatter));
     A->have32 = true;

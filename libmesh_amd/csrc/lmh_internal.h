@@ -14,6 +14,19 @@
 
 namespace lmh {
 
+// Tail padding (entries) of CSR value/column arrays: the SpMV reads aligned
+// 16-byte chunks and may touch up to 3 entries past the last stored one.
+constexpr int64_t LMH_PAD = 4;
+
+// SpMV row blocks hold at most this many stored entries
+#ifndef LMH_RB_CAP
+#define LMH_RB_CAP 2048
+#endif
+// workgroups per CU of the vector/SpMV kernels of the solvers
+#ifndef LMH_GRID_PER_CU
+#define LMH_GRID_PER_CU 8
+#endif
+
 // ---- errors ---------------------------------------------------------------
 int set_error(int code, const char *fmt, ...);
 

@@ -22,7 +22,7 @@ namespace lmh {
 
 constexpr int BLK = 256;
 constexpr int NXCD = 8;
-constexpr int RB_CAP = 2048;  // stored entries per SpMV row block
+constexpr int RB_CAP = LMH_RB_CAP;  // stored entries per SpMV row block
 
 // Blocks are dispatched round-robin over the XCDs; remap so each XCD owns one
 // contiguous range of rows (its L2 then holds the x entries its rows gather).
@@ -116,10 +116,8 @@ struct Off16Cols {
     b1 = m.y;
     s0 = m.z;
   }
-  __device__ inline int32_t load(int32_t k) const {
-    const int32_t o = (int32_t)__builtin_nontemporal_load(&off[k]);
-    return o + (o <= s0 ? b0 : b1);
-  }
+  __device__ inline int32_t dec(int32_t o) const { return o + (o <= s0 ? b0 : b1); }
+  __device__ inline int32_t load(int32_t k) const { return dec((int32_t)__builtin_nontemporal_load(&off[k])); }
 };
 
 template <typename VT, typename XT, typename AT, typename Epi, typename Cols = PlainCols>
@@ -623,7 +621,7 @@ __global__ __launch_bounds__(BLK) void k_fin_sum(const double *__restrict__ part
 
 static int vec_grid(lmh_context *ctx) {
   // the partial buffer holds 4 * n_part doubles; publish<3> uses 3 * grid
-  const int g = ctx->cu_count * 8;
+  const int g = ctx->cu_count * LMH_GRID_PER_CU;
   return g < ctx->n_part ? g : ctx->n_part;
 }
 
@@ -631,7 +629,7 @@ static int vec_grid(lmh_context *ctx) {
 static int ensure_off16(lmh_matrix *A) {
   if (A->idx16_state != 0 || !A->fp32_index) return LMH_OK;
   lmh_context *ctx = A->ctx;
-  LMH_HIP(hipMalloc((void **)&A->off16, sizeof(uint16_t) * (A->nnz ? A->nnz : 1)));
+  LMH_HIP(hipMalloc((void **)&A->off16, sizeof(uint16_t) * (A->nnz + LMH_PAD)));
   LMH_HIP(hipMalloc((void **)&A->rbmeta, sizeof(int4) * (A->nrb ? A->nrb : 1)));
   LMH_HIP(hipMemsetAsync(ctx->d_flags, 0, sizeof(int), ctx->stream));
   LMH_LAUNCH(ctx, "build_off16", k_build_off16, dim3(grid_for(A->nrb, 1, 16384)), dim3(BLK), 0, A->rb, A->nrb, A->rp,
@@ -714,7 +712,7 @@ double lmh_norm2(lmh_context *ctx, const double *x_d, int64_t n) {
 int lmh_matrix_make_fp32(lmh_matrix *A) {
   LMH_CHECK(A, LMH_ERR_INVALID, "null matrix");
   lmh_context *ctx = A->ctx;
-  if (!A->vals32) LMH_HIP(hipMalloc((void **)&A->vals32, sizeof(float) * A->nnz));
+  if (!A->vals32) LMH_HIP(hipMalloc((void **)&A->vals32, sizeof(float) * (A->nnz + LMH_PAD)));
   LMH_LAUNCH(ctx, "round_fp32", k_round32, dim3(grid_for(A->nnz, 256, 16384)), dim3(256), 0, A->nnz, A->vals,
              A->vals32);
   A->have32 = true;

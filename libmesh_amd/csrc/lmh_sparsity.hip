@@ -141,7 +141,7 @@ static int color_elements(const std::vector<int32_t> &edofs, int64_t n_elem, int
 static int build_row_blocks(lmh_matrix *A, const std::vector<int32_t> &rp) {
   // SpMV row blocks: consecutive rows holding at most kRowBlockCap stored
   // entries (a longer row forms a block of its own)
-  const int kRowBlockCap = 2048;
+  const int kRowBlockCap = LMH_RB_CAP;
   std::vector<int32_t> rb{0};
   const int64_t n = A->rows;
   for (int64_t r = 0; r < n; ++r)
@@ -190,7 +190,8 @@ int lmh_matrix_create(lmh_context *ctx, const lmh_dofmap *dm, lmh_matrix **out) 
   int64_t nnz = 0;
   LMH_TRY(exclusive_scan_i32(ctx, rowlen, A->rp, n, &nnz));
   A->nnz = nnz;
-  LMH_HIP(hipMalloc((void **)&A->ci, sizeof(int32_t) * nnz));
+  LMH_HIP(hipMalloc((void **)&A->ci, sizeof(int32_t) * (nnz + LMH_PAD)));
+  LMH_HIP(hipMemsetAsync(A->ci + nnz, 0, sizeof(int32_t) * LMH_PAD, ctx->stream));
   LMH_LAUNCH(ctx, "sparsity_fill", k_row_union<true>, dim3(g), dim3(64), shm, dm->inc_ptr, dm->inc, dm->edofs, dpe,
              n, cap, (int32_t *)nullptr, (const int32_t *)A->rp, A->ci);
   LMH_TRY(check_launch("k_row_union<fill>"));
@@ -214,8 +215,8 @@ int lmh_matrix_create(lmh_context *ctx, const lmh_dofmap *dm, lmh_matrix **out) 
   LMH_HIP(hipStreamSynchronize(ctx->stream));
   LMH_CHECK(!bad, LMH_ERR_INVALID, "internal: element DoF pair missing from the sparsity pattern");
 
-  LMH_HIP(hipMalloc((void **)&A->vals, sizeof(double) * nnz));
-  LMH_HIP(hipMemsetAsync(A->vals, 0, sizeof(double) * nnz, ctx->stream));
+  LMH_HIP(hipMalloc((void **)&A->vals, sizeof(double) * (nnz + LMH_PAD)));
+  LMH_HIP(hipMemsetAsync(A->vals, 0, sizeof(double) * (nnz + LMH_PAD), ctx->stream));
 
   // element colouring (host greedy)
   std::vector<int32_t> ed((size_t)ne * dpe);
@@ -253,11 +254,12 @@ int lmh_matrix_create_csr(lmh_context *ctx, int64_t rows, const int32_t *row_ptr
   A->nnz = nnz;
   A->max_row = max_row;
   LMH_HIP(hipMalloc((void **)&A->rp, sizeof(int32_t) * (rows + 1)));
-  LMH_HIP(hipMalloc((void **)&A->ci, sizeof(int32_t) * (nnz ? nnz : 1)));
-  LMH_HIP(hipMalloc((void **)&A->vals, sizeof(double) * (nnz ? nnz : 1)));
+  LMH_HIP(hipMalloc((void **)&A->ci, sizeof(int32_t) * (nnz + LMH_PAD)));
+  LMH_HIP(hipMalloc((void **)&A->vals, sizeof(double) * (nnz + LMH_PAD)));
+  LMH_HIP(hipMemset(A->ci, 0, sizeof(int32_t) * (nnz + LMH_PAD)));
   LMH_HIP(hipMemcpy(A->rp, rp.data(), sizeof(int32_t) * (rows + 1), hipMemcpyHostToDevice));
   if (nnz) LMH_HIP(hipMemcpy(A->ci, col_idx_h, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
-  LMH_HIP(hipMemset(A->vals, 0, sizeof(double) * (nnz ? nnz : 1)));
+  LMH_HIP(hipMemset(A->vals, 0, sizeof(double) * (nnz + LMH_PAD)));
   LMH_TRY(build_row_blocks(A.get(), rp));
   *out = A.release();
   return LMH_OK;
