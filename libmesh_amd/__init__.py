@@ -260,6 +260,10 @@ class DeviceArray:
     def zero(self):
         check(hip().lmh_memset(self.ctx.h, self.ptr, 0, self.nbytes))
 
+    def copy_from(self, other):
+        assert other.nbytes == self.nbytes
+        check(hip().lmh_memcpy_d2d(self.ctx.h, self.ptr, other.ptr, self.nbytes))
+
     def free(self):
         if self.ptr:
             hip().lmh_free(self.ctx.h, self.ptr)

@@ -310,6 +310,13 @@ int build_incidence(lmh_context *ctx, const int32_t *edofs, int64_t n_elem, int 
   return LMH_OK;
 }
 
+// Plain device copy, 16 bytes per thread per step.  Also the reference kernel
+// of known traffic (reads and writes `n16*16` bytes) for hardware counters.
+__global__ void k_copy16(const int4 *__restrict__ src, int4 *__restrict__ dst, int64_t n16) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n16; t += (int64_t)gridDim.x * blockDim.x)
+    dst[t] = src[t];
+}
+
 }  // namespace lmh
 
 using namespace lmh;
@@ -404,6 +411,13 @@ int lmh_memcpy_d2h(lmh_context *ctx, void *h_dst, const void *d_src, size_t byte
 }
 int lmh_memcpy_d2d(lmh_context *ctx, void *d_dst, const void *d_src, size_t bytes) {
   LMH_CHECK(ctx, LMH_ERR_INVALID, "null context");
+  if (bytes == 0) return LMH_OK;
+  if (((uintptr_t)d_dst | (uintptr_t)d_src | bytes) % 16 == 0) {
+    const int64_t n16 = (int64_t)(bytes / 16);
+    LMH_LAUNCH(ctx, "copy", k_copy16, dim3(grid_for(n16, 256, ctx->cu_count * 16)), dim3(256), 0,
+               (const int4 *)d_src, (int4 *)d_dst, n16);
+    return LMH_OK;
+  }
   LMH_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
   return LMH_OK;
 }
