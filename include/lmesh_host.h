@@ -30,6 +30,10 @@ const char *lmesh_last_error(void);
 void lmesh_perturb(lmesh_problem *p, uint64_t seed, double frac);
 /* kappa_e = exp(N(0,1)) per element from mt19937_64(seed) + Box-Muller. */
 void lmesh_lognormal(int64_t n, uint64_t seed, double *out);
+/* Batch of cube problems: per problem, from mt19937_64(seed), edge N =
+ * round(nmin (nmax/nmin)^u) and element type uniform over HEX8, HEX27, TET4,
+ * TET10 (libMesh ElemType values). */
+void lmesh_batch_recipe(uint64_t seed, int count, int nmin, int nmax, int *edge, int *elem_type);
 
 int64_t lmesh_n_nodes(const lmesh_problem *p);
 int64_t lmesh_n_elem(const lmesh_problem *p);

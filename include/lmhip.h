@@ -52,6 +52,7 @@ typedef struct lmh_context lmh_context;
 typedef struct lmh_mesh lmh_mesh;
 typedef struct lmh_dofmap lmh_dofmap;
 typedef struct lmh_matrix lmh_matrix;
+typedef struct lmh_batch lmh_batch;
 
 /* Last error message of the calling thread ("" if none). */
 const char *lmh_last_error(void);
@@ -190,6 +191,37 @@ int lmh_qoi_box_derivative(const lmh_mesh *m, const lmh_dofmap *dm, int npts, do
 /* dJ/dkappa_e = -(1/kappa_e) lambda_e^T Kc_e u_e for every element, FP64. */
 int lmh_adjoint_kappa_gradient(const lmh_mesh *m, const lmh_dofmap *dm, int npts, const double *kappa_d,
                                const double *u_d, const double *lambda_d, double *grad_d);
+
+/* ---- batched FP32 linear-elasticity assembly of independent problems ----
+ * Each problem is one LinearElasticity system of systems_of_equations_ex6.C
+ * (assemble_elasticity, :127-236: C_ijkl = l1 d_ij d_kl + l2 (d_ik d_jl +
+ * d_il d_jk), body force (0,0,fz), QGauss at the FE type's default order,
+ * element constraints, add_matrix / add_vector).  libMesh assembles one system
+ * per System::assemble call; here a whole batch of problems of uneven size and
+ * mixed element type (HEX8, HEX27, TET4, TET10; 3 variables per node) is
+ * assembled by one hipGraph launch of a persistent element kernel followed by a
+ * persistent row-gather scatter.  Values are FP32 (lmh_matrix_get_values_fp32,
+ * lmh_spmv_fp32), right-hand sides FP64. */
+int lmh_batch_create(lmh_context *ctx, int n_prob, const lmh_dofmap *const *dms, lmh_batch **out);
+int lmh_batch_destroy(lmh_batch *b);
+int lmh_batch_size(const lmh_batch *b);
+/* The batch owns each problem's matrix and right-hand side. */
+int lmh_batch_matrix(const lmh_batch *b, int p, lmh_matrix **A);
+int lmh_batch_rhs(const lmh_batch *b, int p, double **rhs_d);
+/* Tile shape per element type, indexed HEX8, HEX27, TET4, TET10: lanes per
+ * element (16..256) and passes (groups of concurrently assembled elements) per
+ * tile.  The defaults are the smallest lane count the LDS budget allows. */
+int lmh_batch_set_tiles(lmh_batch *b, const int *lanes_per_elem, const int *passes_per_tile);
+int lmh_batch_get_tiles(const lmh_batch *b, int *lanes_per_elem, int *passes_per_tile);
+/* Time every admissible tile shape of each element type present on the batch
+ * itself (best of `reps` runs after one warm-up), keep the fastest, and
+ * capture the assembly graph for (E, nu, fz).  best_ms[4] (may be NULL)
+ * receives the element-kernel time of the chosen shape per type (-1: absent). */
+int lmh_batch_autotune(lmh_batch *b, double E, double nu, double fz, int reps, double *best_ms);
+/* Assemble every problem: one hipGraphLaunch (the graph is captured on first
+ * use and again when E, nu or fz change).  LMH_ERR_DEGENERATE if any element
+ * map has det J <= 0. */
+int lmh_batch_assemble_elasticity(lmh_batch *b, double E, double nu, double fz);
 
 /* ---- per-kernel timing with HIP events on the context stream ---- */
 int lmh_profile_enable(lmh_context *ctx, int on);

@@ -86,6 +86,15 @@ HIP_API = {
     "lmh_matrix_fp32_compression": (_I, [_P]),
     "lmh_qoi_box_derivative": (_I, [_P, _P, _I, _D, _D, _P]),
     "lmh_adjoint_kappa_gradient": (_I, [_P, _P, _I, _P, _P, _P, _P]),
+    "lmh_batch_create": (_I, [_P, _I, _P, C.POINTER(_P)]),
+    "lmh_batch_destroy": (_I, [_P]),
+    "lmh_batch_size": (_I, [_P]),
+    "lmh_batch_matrix": (_I, [_P, _I, C.POINTER(_P)]),
+    "lmh_batch_rhs": (_I, [_P, _I, C.POINTER(_P)]),
+    "lmh_batch_set_tiles": (_I, [_P, _P, _P]),
+    "lmh_batch_get_tiles": (_I, [_P, _P, _P]),
+    "lmh_batch_autotune": (_I, [_P, _D, _D, _D, _I, _P]),
+    "lmh_batch_assemble_elasticity": (_I, [_P, _D, _D, _D]),
     "lmh_profile_enable": (_I, [_P, _I]),
     "lmh_profile_query": (_I, [_P, C.c_char_p, C.POINTER(_D), C.POINTER(_I64)]),
     "lmh_profile_reset": (_I, [_P]),
@@ -97,6 +106,7 @@ HOST_API = {
     "lmesh_last_error": (C.c_char_p, []),
     "lmesh_perturb": (None, [_P, C.c_uint64, _D]),
     "lmesh_lognormal": (None, [_I64, C.c_uint64, _P]),
+    "lmesh_batch_recipe": (None, [C.c_uint64, _I, _I, _I, _P, _P]),
     "lmesh_n_nodes": (_I64, [_P]),
     "lmesh_n_elem": (_I64, [_P]),
     "lmesh_n_dofs": (_I64, [_P]),
@@ -192,6 +202,14 @@ def lognormal(n, seed):
     out = np.empty(n)
     host().lmesh_lognormal(n, seed, _ptr(out))
     return out
+
+
+def batch_recipe(seed, count, nmin=8, nmax=48):
+    """(edge, element type) per problem of a seeded batch of cube problems."""
+    edge = np.empty(count, dtype=np.int32)
+    kind = np.empty(count, dtype=np.int32)
+    host().lmesh_batch_recipe(seed, count, nmin, nmax, _ptr(edge), _ptr(kind))
+    return edge, kind
 
 
 # ---------------------------------------------------------------------------
@@ -385,5 +403,79 @@ class GpuProblem:
         try:
             if self.ctx.h:
                 self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Batched FP32 elasticity assembly of many GpuProblems (lmh_batch); the
+    problems keep their meshes and DoF maps, the batch owns matrices and rhs."""
+
+    TYPES = (HEX8, HEX27, TET4, TET10)
+
+    def __init__(self, ctx, problems):
+        self.ctx = ctx
+        self.problems = list(problems)
+        arr = (_P * len(self.problems))(*[gp.dofmap for gp in self.problems])
+        self.h = _P()
+        check(hip().lmh_batch_create(ctx.h, len(self.problems), arr, C.byref(self.h)))
+        self.matrices = []
+        self.rhs_ptrs = []
+        for p in range(len(self.problems)):
+            A = _P()
+            check(hip().lmh_batch_matrix(self.h, p, C.byref(A)))
+            r = _P()
+            check(hip().lmh_batch_rhs(self.h, p, C.byref(r)))
+            self.matrices.append(A)
+            self.rhs_ptrs.append(r)
+
+    def tiles(self):
+        lpe = np.zeros(4, dtype=np.int32)
+        pp = np.zeros(4, dtype=np.int32)
+        check(hip().lmh_batch_get_tiles(self.h, _ptr(lpe), _ptr(pp)))
+        return lpe, pp
+
+    def set_tiles(self, lpe, passes):
+        lpe = np.ascontiguousarray(lpe, dtype=np.int32)
+        passes = np.ascontiguousarray(passes, dtype=np.int32)
+        check(hip().lmh_batch_set_tiles(self.h, _ptr(lpe), _ptr(passes)))
+
+    def autotune(self, E=1.0, nu=0.3, fz=-1.0, reps=2):
+        best = np.zeros(4)
+        check(hip().lmh_batch_autotune(self.h, E, nu, fz, reps, _ptr(best)))
+        return best
+
+    def assemble(self, E=1.0, nu=0.3, fz=-1.0):
+        check(hip().lmh_batch_assemble_elasticity(self.h, E, nu, fz))
+
+    def nnz(self, p):
+        return hip().lmh_matrix_nnz(self.matrices[p])
+
+    def pattern(self, p):
+        n = self.problems[p].n_dofs
+        rp = np.empty(n + 1, dtype=np.int32)
+        ci = np.empty(self.nnz(p), dtype=np.int32)
+        check(hip().lmh_matrix_get_pattern(self.matrices[p], _ptr(rp), _ptr(ci)))
+        return rp, ci
+
+    def values(self, p):
+        v = np.empty(self.nnz(p), dtype=np.float32)
+        check(hip().lmh_matrix_get_values_fp32(self.matrices[p], _ptr(v)))
+        return v
+
+    def rhs(self, p):
+        n = self.problems[p].n_dofs
+        out = np.empty(n)
+        check(hip().lmh_memcpy_d2h(self.ctx.h, _ptr(out), self.rhs_ptrs[p], 8 * n))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            hip().lmh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass

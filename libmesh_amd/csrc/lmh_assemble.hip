@@ -32,37 +32,10 @@ __device__ inline double forcing(int f_kind, double x, double y, double z) {
 // rule), and returns det J (<= 0 signals a degenerate map, fe_map.C:467-505).
 __device__ inline double qp_geometry(const double *X, const double *dref, int n, int nq, int q, double w,
                                      double kap, double *M, double &jxw_out) {
-  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // J[r][c] = d x_c / d xi_r
-  for (int a = 0; a < n; ++a) {
-    const double *d = &dref[(a * nq + q) * 3];
-    const double x = X[a * 3], y = X[a * 3 + 1], z = X[a * 3 + 2];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      J[r][0] += x * d[r];
-      J[r][1] += y * d[r];
-      J[r][2] += z * d[r];
-    }
-  }
-  // cofactor inverse as FEMap::compute_single_point_map (fe_map.C:1046-1068)
-  const double dx_dxi = J[0][0], dy_dxi = J[0][1], dz_dxi = J[0][2];
-  const double dx_deta = J[1][0], dy_deta = J[1][1], dz_deta = J[1][2];
-  const double dx_dzeta = J[2][0], dy_dzeta = J[2][1], dz_dzeta = J[2][2];
-  const double det = (dx_dxi * (dy_deta * dz_dzeta - dz_deta * dy_dzeta) +
-                      dy_dxi * (dz_deta * dx_dzeta - dx_deta * dz_dzeta) +
-                      dz_dxi * (dx_deta * dy_dzeta - dy_deta * dx_dzeta));
+  double G[3][3];
+  const double det = qp_map(X, dref, n, nq, q, G);
   jxw_out = 0.0;
   if (!(det > 0.0)) return -1.0;
-  const double inv = 1.0 / det;
-  double G[3][3];  // G[r][c] = d xi_r / d x_c
-  G[0][0] = (dy_deta * dz_dzeta - dz_deta * dy_dzeta) * inv;
-  G[0][1] = (dz_deta * dx_dzeta - dx_deta * dz_dzeta) * inv;
-  G[0][2] = (dx_deta * dy_dzeta - dy_deta * dx_dzeta) * inv;
-  G[1][0] = (dz_dxi * dy_dzeta - dy_dxi * dz_dzeta) * inv;
-  G[1][1] = (dx_dxi * dz_dzeta - dz_dxi * dx_dzeta) * inv;
-  G[1][2] = (dy_dxi * dx_dzeta - dx_dxi * dy_dzeta) * inv;
-  G[2][0] = (dy_dxi * dz_deta - dz_dxi * dy_deta) * inv;
-  G[2][1] = (dz_dxi * dx_deta - dx_dxi * dz_deta) * inv;
-  G[2][2] = (dx_dxi * dy_deta - dy_dxi * dx_deta) * inv;
   const double jxw = det * w;
   const double s = jxw * kap;
 #pragma unroll
@@ -683,6 +656,7 @@ static int scatter_impl(lmh_matrix *A, const T *Ke, const double *Fe, T *vals, d
     return check_launch("k_scatter_gather");
   }
   LMH_CHECK(scatter == LMH_SCATTER_COLORED, LMH_ERR_INVALID, "unknown scatter mode %d", scatter);
+  LMH_CHECK(A->ncolors > 0 || dm->mesh->n_elem == 0, LMH_ERR_UNSUPPORTED, "matrix was created without a colouring");
   LMH_LAUNCH(ctx, "fill", k_fill<T>, dim3(grid_for(A->nnz, 256, 8192)), dim3(256), 0, vals, A->nnz, (T)0);
   if (rhs) LMH_LAUNCH(ctx, "fill", k_fill<double>, dim3(grid_for(A->rows, 256, 8192)), dim3(256), 0, rhs, A->rows, 0.0);
   for (int c = 0; c < A->ncolors; ++c) {

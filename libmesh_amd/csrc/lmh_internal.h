@@ -166,6 +166,45 @@ int exclusive_scan_i32(lmh_context *ctx, const int32_t *in, int32_t *out, int64_
 int build_incidence(lmh_context *ctx, const int32_t *edofs, int64_t n_elem, int dpe, int64_t n_dofs,
                     int32_t **inc_ptr, int32_t **inc);
 int fe_table(lmh_context *ctx, int type, int arg, const FETable **out);
+// CSR pattern, scatter map and (if colour) element colouring for a DoF map
+int matrix_create_impl(lmh_context *ctx, const lmh_dofmap *dm, bool colour, lmh_matrix **out);
+
+// ---- element map at one quadrature point ----------------------------------
+// From LDS coordinates X[a*3+c] and the reference derivative table
+// dref[(a*nq+q)*3+r]: J[r][c] = sum_a X_a[c] dphi_a/dxi_r(q), G = J^{-1} by
+// cofactors as FEMap::compute_single_point_map (fe_map.C:1046-1068),
+// G[r][c] = d xi_r / d x_c.  Returns det J; G is only meaningful when det > 0.
+__device__ inline double qp_map(const double *X, const double *dref, int n, int nq, int q, double G[3][3]) {
+  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // J[r][c] = d x_c / d xi_r
+  for (int a = 0; a < n; ++a) {
+    const double *d = &dref[(a * nq + q) * 3];
+    const double x = X[a * 3], y = X[a * 3 + 1], z = X[a * 3 + 2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      J[r][0] += x * d[r];
+      J[r][1] += y * d[r];
+      J[r][2] += z * d[r];
+    }
+  }
+  const double dx_dxi = J[0][0], dy_dxi = J[0][1], dz_dxi = J[0][2];
+  const double dx_deta = J[1][0], dy_deta = J[1][1], dz_deta = J[1][2];
+  const double dx_dzeta = J[2][0], dy_dzeta = J[2][1], dz_dzeta = J[2][2];
+  const double det = (dx_dxi * (dy_deta * dz_dzeta - dz_deta * dy_dzeta) +
+                      dy_dxi * (dz_deta * dx_dzeta - dx_deta * dz_dzeta) +
+                      dz_dxi * (dx_deta * dy_dzeta - dy_deta * dx_dzeta));
+  if (!(det > 0.0)) return det;
+  const double inv = 1.0 / det;
+  G[0][0] = (dy_deta * dz_dzeta - dz_deta * dy_dzeta) * inv;
+  G[0][1] = (dz_deta * dx_dzeta - dx_deta * dz_dzeta) * inv;
+  G[0][2] = (dx_deta * dy_dzeta - dy_deta * dx_dzeta) * inv;
+  G[1][0] = (dz_dxi * dy_dzeta - dy_dxi * dz_dzeta) * inv;
+  G[1][1] = (dx_dxi * dz_dzeta - dz_dxi * dx_dzeta) * inv;
+  G[1][2] = (dy_dxi * dx_dzeta - dx_dxi * dy_dzeta) * inv;
+  G[2][0] = (dy_dxi * dz_deta - dz_dxi * dy_deta) * inv;
+  G[2][1] = (dz_dxi * dx_deta - dx_dxi * dz_deta) * inv;
+  G[2][2] = (dx_dxi * dy_deta - dy_dxi * dx_deta) * inv;
+  return det;
+}
 
 // grid for an XCD-aware 1-D launch
 inline int grid_for(int64_t work, int per_block, int max_blocks) {

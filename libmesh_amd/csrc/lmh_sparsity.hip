@@ -157,9 +157,9 @@ static int build_row_blocks(lmh_matrix *A, const std::vector<int32_t> &rp) {
 
 using namespace lmh;
 
-extern "C" {
+namespace lmh {
 
-int lmh_matrix_create(lmh_context *ctx, const lmh_dofmap *dm, lmh_matrix **out) {
+int matrix_create_impl(lmh_context *ctx, const lmh_dofmap *dm, bool colour, lmh_matrix **out) {
   LMH_CHECK(ctx && dm && out, LMH_ERR_INVALID, "null argument");
   LMH_HIP(hipSetDevice(ctx->device));
   const int64_t n = dm->n_dofs, ne = dm->mesh->n_elem;
@@ -218,16 +218,27 @@ int lmh_matrix_create(lmh_context *ctx, const lmh_dofmap *dm, lmh_matrix **out) 
   LMH_HIP(hipMalloc((void **)&A->vals, sizeof(double) * (nnz + LMH_PAD)));
   LMH_HIP(hipMemsetAsync(A->vals, 0, sizeof(double) * (nnz + LMH_PAD), ctx->stream));
 
-  // element colouring (host greedy)
-  std::vector<int32_t> ed((size_t)ne * dpe);
-  LMH_HIP(hipMemcpy(ed.data(), dm->edofs, sizeof(int32_t) * ed.size(), hipMemcpyDeviceToHost));
-  std::vector<int32_t> celems;
-  LMH_TRY(color_elements(ed, ne, dpe, n, A->color_ptr, celems, A->ncolors));
-  LMH_HIP(hipMalloc((void **)&A->color_elems, sizeof(int32_t) * ne));
-  LMH_HIP(hipMemcpy(A->color_elems, celems.data(), sizeof(int32_t) * ne, hipMemcpyHostToDevice));
+  // element colouring (host greedy); matrices assembled only by the gather
+  // scatter skip it
+  if (colour) {
+    std::vector<int32_t> ed((size_t)ne * dpe);
+    LMH_HIP(hipMemcpy(ed.data(), dm->edofs, sizeof(int32_t) * ed.size(), hipMemcpyDeviceToHost));
+    std::vector<int32_t> celems;
+    LMH_TRY(color_elements(ed, ne, dpe, n, A->color_ptr, celems, A->ncolors));
+    LMH_HIP(hipMalloc((void **)&A->color_elems, sizeof(int32_t) * ne));
+    LMH_HIP(hipMemcpy(A->color_elems, celems.data(), sizeof(int32_t) * ne, hipMemcpyHostToDevice));
+  }
   LMH_HIP(hipStreamSynchronize(ctx->stream));
   *out = A.release();
   return LMH_OK;
+}
+
+}  // namespace lmh
+
+extern "C" {
+
+int lmh_matrix_create(lmh_context *ctx, const lmh_dofmap *dm, lmh_matrix **out) {
+  return matrix_create_impl(ctx, dm, true, out);
 }
 
 int lmh_matrix_create_csr(lmh_context *ctx, int64_t rows, const int32_t *row_ptr_h, const int32_t *col_idx_h,

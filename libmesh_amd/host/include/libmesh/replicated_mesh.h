@@ -76,6 +76,10 @@ void perturb_interior(ReplicatedMesh &mesh, uint64_t seed, Real frac);
 }  // namespace Modification
 // kappa_e = exp(z_e), z_e ~ N(0,1) from Box-Muller over mt19937_64(seed), in element order.
 std::vector<Real> lognormal_coefficients(dof_id_type n_elem, uint64_t seed);
+// Batch of cube problems: for each problem, from mt19937_64(seed), edge
+// N = round(nmin (nmax/nmin)^u) (log-uniform in [nmin, nmax]) and element type
+// uniform over HEX8, HEX27, TET4, TET10 (two draws per problem, in order).
+void batch_recipe(uint64_t seed, int count, int nmin, int nmax, std::vector<int> &edge, std::vector<ElemType> &type);
 }  // namespace MeshTools
 
 }  // namespace libMesh

@@ -53,6 +53,16 @@ void lmesh_lognormal(int64_t n, uint64_t seed, double *out) {
   std::memcpy(out, k.data(), sizeof(double) * k.size());
 }
 
+void lmesh_batch_recipe(uint64_t seed, int count, int nmin, int nmax, int *edge, int *elem_type) {
+  std::vector<int> e;
+  std::vector<ElemType> t;
+  MeshTools::batch_recipe(seed, count, nmin, nmax, e, t);
+  for (int p = 0; p < count; ++p) {
+    edge[p] = e[p];
+    elem_type[p] = (int)t[p];
+  }
+}
+
 int64_t lmesh_n_nodes(const lmesh_problem *p) { return p->mesh.n_nodes(); }
 int64_t lmesh_n_elem(const lmesh_problem *p) { return p->mesh.n_elem(); }
 int64_t lmesh_n_dofs(const lmesh_problem *p) { return p->dof_map.n_dofs(); }

@@ -203,5 +203,18 @@ std::vector<Real> lognormal_coefficients(dof_id_type n_elem, uint64_t seed) {
   return k;
 }
 
+void batch_recipe(uint64_t seed, int count, int nmin, int nmax, std::vector<int> &edge, std::vector<ElemType> &type) {
+  static const ElemType kinds[4] = {HEX8, HEX27, TET4, TET10};
+  std::mt19937_64 g(seed);
+  edge.resize(count);
+  type.resize(count);
+  for (int p = 0; p < count; ++p) {
+    const double u = unit_uniform(g);
+    int n = (int)std::floor(nmin * std::pow((double)nmax / nmin, u) + 0.5);
+    edge[p] = std::min(nmax, std::max(nmin, n));
+    type[p] = kinds[std::min(3, (int)(4.0 * unit_uniform(g)))];
+  }
+}
+
 }  // namespace MeshTools
 }  // namespace libMesh

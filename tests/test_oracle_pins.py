@@ -335,3 +335,67 @@ def test_adjoint_homogeneity_and_fd():
             uu, _, _ = O.cg_eigen(rp, ci, vv, bb, 1e-15, 20000)
             Js.append(s.qoi_box(3, uu))
         assert abs((Js[0] - Js[1]) / (2 * d) - g[e]) < 1e-6 * abs(g).max()
+
+
+@pytest.mark.parametrize("t,npts", [(O.HEX8, 2), (O.HEX27, 3), (O.TET4, 3), (O.TET10, 5)])
+def test_elasticity_element_rigid_modes_and_trace(t, npts):
+    """Linear-elasticity element matrices (systems_of_equations_ex6.C:127-150):
+    the six rigid-body modes (3 translations, 3 infinitesimal rotations) are in
+    the kernel of the unconstrained Ke on perturbed elements (isoparametric
+    Lagrange spaces reproduce linear fields), the sum of the three diagonal
+    component blocks is (lambda + 4 mu) times the Poisson element matrix, and
+    the body-force vector integrates fz over the unit cube."""
+    m = O.Mesh(2, t)
+    m.perturb(5, 0.2 if t in (O.HEX8, O.HEX27) else 0.1)  # 0.2 h inverts some TET10 sub-tets
+    E, nu, fz = 1.0, 0.3, -1.0
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    s3 = O.System(m, n_vars=3, dirichlet_mask=0)
+    s1 = O.System(m, n_vars=1, dirichlet_mask=0)
+    K, F = s3.element_elasticity(npts, E, nu, fz, constrain=False)
+    Kp, _ = s1.element_poisson(npts, None, 0, constrain=False)
+    xyz, conn = m.coords(), m.conn()
+    n = conn.shape[1]
+    for e in range(m.n_elem):
+        X = xyz[conn[e]]
+        modes = []
+        for c in range(3):
+            v = np.zeros(3 * n)
+            v[c * n:(c + 1) * n] = 1.0
+            modes.append(v)
+        for c in range(3):
+            w = np.zeros(3)
+            w[c] = 1.0
+            u = np.cross(w, X)
+            modes.append(np.concatenate([u[:, 0], u[:, 1], u[:, 2]]))
+        scale = np.abs(K[e]).max()
+        for v in modes:
+            assert np.abs(K[e] @ v).max() < 1e-12 * scale * np.abs(v).max()
+        assert np.abs(K[e] - K[e].T).max() < 1e-14 * scale
+        diag = sum(K[e][i * n:(i + 1) * n, i * n:(i + 1) * n] for i in range(3))
+        assert np.abs(diag - (lam + 4 * mu) * Kp[e]).max() < 1e-13 * scale
+    assert np.abs(F[:, :2 * n]).max() == 0.0
+    assert abs(F[:, 2 * n:].sum() - fz) < 1e-12
+
+
+def test_elasticity_constrained_rows():
+    """Clamped x = 0 face (boundary id 4): every DoF of every node on that face is
+    constrained; assembled rows of constrained DoFs hold only the diagonal, which
+    counts the incident elements, and their rhs entries are zero."""
+    m = O.Mesh(3, O.HEX8)
+    s = O.System(m, n_vars=3, dirichlet_mask=1 << 4)
+    vals, rhs = s.assemble_elasticity(2)
+    rp, ci = s.pattern()
+    cons = s.constrained().astype(bool)
+    assert cons.sum() == 3 * 16
+    xyz = m.coords()
+    face = np.abs(xyz[:, 0]) < 1e-14
+    node_of = np.arange(s.n_dofs) // 3
+    assert np.array_equal(cons, face[node_of])
+    ed = s.elem_dofs()
+    for r in np.nonzero(cons)[0]:
+        row = slice(rp[r], rp[r + 1])
+        d = vals[row][ci[row] == r][0]
+        assert d == float((ed == r).any(axis=1).sum())
+        assert np.count_nonzero(vals[row]) == 1
+        assert rhs[r] == 0.0
