@@ -687,6 +687,7 @@ int lmh_assemble_poisson(lmh_matrix *A, const lmh_mesh *m, const lmh_dofmap *dm,
                          int f_kind, double *rhs_d, int precision, int scatter) {
   LMH_CHECK(A && m && dm && rhs_d, LMH_ERR_INVALID, "null argument");
   LMH_CHECK(A->dm == dm && dm->mesh == m, LMH_ERR_INVALID, "matrix, dofmap and mesh do not belong together");
+  LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
   lmh_context *ctx = A->ctx;
   const int64_t ne = m->n_elem, dd = (int64_t)dm->dpe * dm->dpe;
   if (!A->Fe) LMH_HIP(hipMalloc((void **)&A->Fe, sizeof(double) * ne * dm->dpe));
@@ -717,6 +718,7 @@ int lmh_matrix_get_values_fp32(const lmh_matrix *A, float *vals_h) {
 
 int lmh_matrix_add_element_matrices(lmh_matrix *A, int64_t n_elem, int n, const double *Ke_d, const int32_t *dofs_d) {
   LMH_CHECK(A && Ke_d && dofs_d && n > 0, LMH_ERR_INVALID, "bad argument");
+  LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
   if (n_elem == 0) return LMH_OK;
   lmh_context *ctx = A->ctx;
   LMH_HIP(hipMemsetAsync(ctx->d_flags, 0, sizeof(int), ctx->stream));

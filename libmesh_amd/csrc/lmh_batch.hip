@@ -344,7 +344,7 @@ static int batch_setup(lmh_batch *b, lmh_context *ctx, int n_prob, const lmh_dof
     const FETable *t = nullptr;
     LMH_TRY(fe_table(ctx, dm->mesh->type, arg, &t));
     lmh_matrix *A = nullptr;
-    LMH_TRY(matrix_create_impl(ctx, dm, false, &A));
+    LMH_TRY(matrix_create_impl(ctx, dm, false, false, &A));
     b->mats.push_back(A);
     b->dms.push_back(dm);
     b->slot.push_back(s);

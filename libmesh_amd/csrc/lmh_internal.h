@@ -166,8 +166,9 @@ int exclusive_scan_i32(lmh_context *ctx, const int32_t *in, int32_t *out, int64_
 int build_incidence(lmh_context *ctx, const int32_t *edofs, int64_t n_elem, int dpe, int64_t n_dofs,
                     int32_t **inc_ptr, int32_t **inc);
 int fe_table(lmh_context *ctx, int type, int arg, const FETable **out);
-// CSR pattern, scatter map and (if colour) element colouring for a DoF map
-int matrix_create_impl(lmh_context *ctx, const lmh_dofmap *dm, bool colour, lmh_matrix **out);
+// CSR pattern, scatter map, (if colour) element colouring and (if fp64) zeroed
+// FP64 values for a DoF map
+int matrix_create_impl(lmh_context *ctx, const lmh_dofmap *dm, bool colour, bool fp64, lmh_matrix **out);
 
 // ---- element map at one quadrature point ----------------------------------
 // From LDS coordinates X[a*3+c] and the reference derivative table

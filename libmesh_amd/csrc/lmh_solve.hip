@@ -664,6 +664,7 @@ extern "C" {
 
 int lmh_spmv(lmh_matrix *A, const double *x_d, double *y_d) {
   LMH_CHECK(A && x_d && y_d, LMH_ERR_INVALID, "null argument");
+  LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
   lmh_context *ctx = A->ctx;
   const int g = vec_grid(ctx);
   LMH_LAUNCH(ctx, "spmv_fp64", (k_spmv<double, double, double, double>), dim3(g),
@@ -689,6 +690,7 @@ int lmh_spmv_fp32(lmh_matrix *A, const float *x_d, float *y_d) {
 
 int lmh_residual(lmh_matrix *A, const double *b_d, const double *x_d, double *r_d) {
   LMH_CHECK(A && b_d && x_d && r_d, LMH_ERR_INVALID, "null argument");
+  LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
   lmh_context *ctx = A->ctx;
   const int g = vec_grid(ctx);
   LMH_LAUNCH(ctx, "residual_fp64", k_residual, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp,
@@ -711,6 +713,7 @@ double lmh_norm2(lmh_context *ctx, const double *x_d, int64_t n) {
 
 int lmh_matrix_make_fp32(lmh_matrix *A) {
   LMH_CHECK(A, LMH_ERR_INVALID, "null matrix");
+  LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
   lmh_context *ctx = A->ctx;
   if (!A->vals32) LMH_HIP(hipMalloc((void **)&A->vals32, sizeof(float) * (A->nnz + LMH_PAD)));
   LMH_LAUNCH(ctx, "round_fp32", k_round32, dim3(grid_for(A->nnz, 256, 16384)), dim3(256), 0, A->nnz, A->vals,
@@ -729,6 +732,7 @@ int lmh_cg_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int
                  double *rel_res, int *reason) {
   LMH_CHECK(A && b_d && x_d, LMH_ERR_INVALID, "null argument");
   LMH_CHECK(rtol >= 0 && maxits >= 0, LMH_ERR_INVALID, "bad tolerance or iteration limit");
+  LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
   lmh_context *ctx = A->ctx;
   LMH_TRY(ensure_work(A, false));
   const int64_t n = A->rows;
@@ -790,6 +794,7 @@ int lmh_cg_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int
 int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, double inner_rtol, int mode,
                     int64_t maxits, int64_t *iters, double *rel_res, int *reason, double *stats_h) {
   LMH_CHECK(A && b_d && x_d, LMH_ERR_INVALID, "null argument");
+  LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
   LMH_CHECK(A->have32 && A->vals32, LMH_ERR_INVALID, "mixed solve needs the FP32 operator (assemble FP32 or make_fp32)");
   LMH_CHECK(mode == 0 || mode == 1, LMH_ERR_INVALID, "mode must be 0 (reliable update) or 1 (restarted refinement)");
   LMH_CHECK(rtol > 0 && inner_rtol > 0 && inner_rtol < 1, LMH_ERR_INVALID, "bad tolerances");

@@ -159,7 +159,7 @@ using namespace lmh;
 
 namespace lmh {
 
-int matrix_create_impl(lmh_context *ctx, const lmh_dofmap *dm, bool colour, lmh_matrix **out) {
+int matrix_create_impl(lmh_context *ctx, const lmh_dofmap *dm, bool colour, bool fp64, lmh_matrix **out) {
   LMH_CHECK(ctx && dm && out, LMH_ERR_INVALID, "null argument");
   LMH_HIP(hipSetDevice(ctx->device));
   const int64_t n = dm->n_dofs, ne = dm->mesh->n_elem;
@@ -215,8 +215,10 @@ int matrix_create_impl(lmh_context *ctx, const lmh_dofmap *dm, bool colour, lmh_
   LMH_HIP(hipStreamSynchronize(ctx->stream));
   LMH_CHECK(!bad, LMH_ERR_INVALID, "internal: element DoF pair missing from the sparsity pattern");
 
-  LMH_HIP(hipMalloc((void **)&A->vals, sizeof(double) * (nnz + LMH_PAD)));
-  LMH_HIP(hipMemsetAsync(A->vals, 0, sizeof(double) * (nnz + LMH_PAD), ctx->stream));
+  if (fp64) {
+    LMH_HIP(hipMalloc((void **)&A->vals, sizeof(double) * (nnz + LMH_PAD)));
+    LMH_HIP(hipMemsetAsync(A->vals, 0, sizeof(double) * (nnz + LMH_PAD), ctx->stream));
+  }
 
   // element colouring (host greedy); matrices assembled only by the gather
   // scatter skip it
@@ -238,7 +240,7 @@ int matrix_create_impl(lmh_context *ctx, const lmh_dofmap *dm, bool colour, lmh_
 extern "C" {
 
 int lmh_matrix_create(lmh_context *ctx, const lmh_dofmap *dm, lmh_matrix **out) {
-  return matrix_create_impl(ctx, dm, true, out);
+  return matrix_create_impl(ctx, dm, true, true, out);
 }
 
 int lmh_matrix_create_csr(lmh_context *ctx, int64_t rows, const int32_t *row_ptr_h, const int32_t *col_idx_h,
@@ -319,6 +321,10 @@ int lmh_matrix_device_arrays(lmh_matrix *A, const int32_t **row_ptr_d, const int
 
 int lmh_matrix_zero(lmh_matrix *A) {
   LMH_CHECK(A, LMH_ERR_INVALID, "null matrix");
+  if (!A->vals) {  // FP32-only matrix
+    LMH_HIP(hipMemsetAsync(A->vals32, 0, sizeof(float) * A->nnz, A->ctx->stream));
+    return LMH_OK;
+  }
   LMH_HIP(hipMemsetAsync(A->vals, 0, sizeof(double) * A->nnz, A->ctx->stream));
   A->have32 = false;
   return LMH_OK;
@@ -326,6 +332,7 @@ int lmh_matrix_zero(lmh_matrix *A) {
 
 int lmh_matrix_get_values(const lmh_matrix *A, double *vals_h) {
   LMH_CHECK(A && vals_h, LMH_ERR_INVALID, "null argument");
+  LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
   LMH_HIP(hipMemcpyAsync(vals_h, A->vals, sizeof(double) * A->nnz, hipMemcpyDeviceToHost, A->ctx->stream));
   LMH_HIP(hipStreamSynchronize(A->ctx->stream));
   return LMH_OK;
@@ -333,6 +340,7 @@ int lmh_matrix_get_values(const lmh_matrix *A, double *vals_h) {
 
 int lmh_matrix_set_values(lmh_matrix *A, const double *vals_h) {
   LMH_CHECK(A && vals_h, LMH_ERR_INVALID, "null argument");
+  LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
   LMH_HIP(hipMemcpyAsync(A->vals, vals_h, sizeof(double) * A->nnz, hipMemcpyHostToDevice, A->ctx->stream));
   LMH_HIP(hipStreamSynchronize(A->ctx->stream));
   A->have32 = false;
