@@ -14,9 +14,9 @@
 
 namespace lmh {
 
-// Tail padding (entries) of CSR value/column arrays: the SpMV reads aligned
-// 16-byte chunks and may touch up to 3 entries past the last stored one.
-constexpr int64_t LMH_PAD = 4;
+// Tail padding (entries) of CSR value/column arrays (kept for aligned vector
+// reads past the last stored entry).
+constexpr int64_t LMH_PAD = 8;
 
 // SpMV row blocks hold at most this many stored entries
 #ifndef LMH_RB_CAP

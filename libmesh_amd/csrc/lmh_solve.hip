@@ -6,8 +6,8 @@
 // libMesh runs for CG + JACOBI_PRECOND (eigen_sparse_linear_solver.C:106-160).
 // All scalars live on the device; each iteration is three kernels
 //   spmv+dot(p,Ap) -> alpha | x,r update + |r|^2 + (r,z) -> beta | p = z + beta p
-// with a deterministic "last block reduces the partials" finish, so the host
-// only polls a converged flag every few iterations.
+// where each kernel reduces the previous kernel's per-block partials itself
+// (deterministic), so the host only polls a converged flag every few iterations.
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -103,23 +103,23 @@ __device__ inline void reduce_partials(const double *__restrict__ part, int G, d
 // columns (16-bit offsets against two per-block bases).
 struct PlainCols {
   const int32_t *ci;
-  __device__ inline void begin(int64_t) {}
-  __device__ inline int32_t load(int32_t k) const { return __builtin_nontemporal_load(&ci[k]); }
+  struct Meta {};
+  __device__ inline Meta meta_of(int64_t) const { return Meta{}; }
+  __device__ inline int32_t raw(int32_t k) const { return __builtin_nontemporal_load(&ci[k]); }
+  __device__ inline int32_t dec(int32_t c, const Meta &) const { return c; }
 };
 struct Off16Cols {
   const uint16_t *off;
   const int4 *meta;
-  int32_t b0 = 0, b1 = 0, s0 = 0;
-  __device__ inline void begin(int64_t bk) {
-    const int4 m = meta[bk];
-    b0 = m.x;
-    b1 = m.y;
-    s0 = m.z;
-  }
-  __device__ inline int32_t dec(int32_t o) const { return o + (o <= s0 ? b0 : b1); }
-  __device__ inline int32_t load(int32_t k) const { return dec((int32_t)__builtin_nontemporal_load(&off[k])); }
+  using Meta = int4;  // {b0, b1, s0, -}
+  __device__ inline Meta meta_of(int64_t bk) const { return meta[bk]; }
+  __device__ inline int32_t raw(int32_t k) const { return (int32_t)__builtin_nontemporal_load(&off[k]); }
+  __device__ inline int32_t dec(int32_t o, const Meta &m) const { return o + (o <= m.z ? m.x : m.y); }
 };
 
+// Software-pipelined over the workgroup's row blocks: the value/column loads of
+// block bk+1 are issued right after the x gathers of block bk, so they are in
+// flight while block bk's products are formed and its rows reduced.
 template <typename VT, typename XT, typename AT, typename Epi, typename Cols = PlainCols>
 __device__ inline void rowblock_spmv_c(const int32_t *__restrict__ rb, int64_t nrb, const int32_t *__restrict__ rp,
                                        Cols cols, const VT *__restrict__ v, const XT *__restrict__ x, Epi &&epi) {
@@ -129,47 +129,65 @@ __device__ inline void rowblock_spmv_c(const int32_t *__restrict__ rb, int64_t n
   int64_t lo, hi;
   block_range(nrb, lo, hi);
   const int tid = threadIdx.x;
+  // state of the block whose loads are in flight
+  int32_t r0 = 0, r1 = 0, k0 = 0, cnt = 0;
+  typename Cols::Meta meta{};
+  VT vv[PER];
+  int32_t cr[PER];
+  // branch-free loads (out-of-range slots re-read entry k0 and are dropped)
+  auto fetch = [&](int64_t bk) {
+    r0 = rb[bk];
+    r1 = rb[bk + 1];
+    k0 = rp[r0];
+    cnt = rp[r1] - k0;
+    meta = cols.meta_of(bk);
+    if (cnt <= RB_CAP) {
+#pragma unroll
+      for (int j = 0; j < PER; ++j) {
+        const int32_t t = tid + j * BLK;
+        const int32_t k = k0 + (t < cnt ? t : 0);
+        vv[j] = __builtin_nontemporal_load(&v[k]);
+        cr[j] = cols.raw(k);
+      }
+    }
+  };
+  if (lo < hi) fetch(lo);
   for (int64_t bk = lo; bk < hi; ++bk) {
-    const int32_t r0 = rb[bk], r1 = rb[bk + 1];
-    const int32_t k0 = rp[r0], cnt = rp[r1] - k0;
-    cols.begin(bk);
-    if (cnt > RB_CAP) {
+    const int32_t R0 = r0, R1 = r1, K0 = k0, CNT = cnt;
+    const typename Cols::Meta M = meta;
+    if (CNT > RB_CAP) {  // one long row: reduce it directly
       AT s = 0;
-      for (int32_t t = tid; t < cnt; t += BLK) s += (AT)__builtin_nontemporal_load(&v[k0 + t]) * (AT)x[cols.load(k0 + t)];
+      for (int32_t t = tid; t < CNT; t += BLK)
+        s += (AT)__builtin_nontemporal_load(&v[K0 + t]) * (AT)x[cols.dec(cols.raw(K0 + t), M)];
       double w = wave_sum((double)s);
       if ((tid & 63) == 0) red[tid >> 6] = w;
       __syncthreads();
       if (tid == 0) {
         double t2 = 0;
         for (int k = 0; k < BLK / 64; ++k) t2 += red[k];
-        epi(r0, (AT)t2);
+        epi(R0, (AT)t2);
       }
       __syncthreads();
+      if (bk + 1 < hi) fetch(bk + 1);
       continue;
     }
-    // branch-free loads (out-of-range slots re-read entry k0 and are dropped)
-    // so all PER value/index loads, then all PER gathers, are in flight at once
-    VT vv[PER];
-    int32_t cc[PER];
     AT xx[PER];
+    VT cv[PER];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-      const int32_t t = tid + j * BLK;
-      const int32_t k = k0 + (t < cnt ? t : 0);
-      vv[j] = __builtin_nontemporal_load(&v[k]);
-      cc[j] = cols.load(k);
+      xx[j] = (AT)x[cols.dec(cr[j], M)];
+      cv[j] = vv[j];
     }
-#pragma unroll
-    for (int j = 0; j < PER; ++j) xx[j] = (AT)x[cc[j]];
+    if (bk + 1 < hi) fetch(bk + 1);
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
       const int32_t t = tid + j * BLK;
-      if (t < cnt) prod[t] = (AT)vv[j] * xx[j];
+      if (t < CNT) prod[t] = (AT)cv[j] * xx[j];
     }
     __syncthreads();
     const int g = tid >> 3, sub = tid & 7;
-    for (int32_t r = r0 + g; r < r1; r += BLK / 8) {
-      const int32_t a = rp[r] - k0, e = rp[r + 1] - k0;
+    for (int32_t r = R0 + g; r < R1; r += BLK / 8) {
+      const int32_t a = rp[r] - K0, e = rp[r + 1] - K0;
       AT s = 0;
       for (int32_t t = a + sub; t < e; t += 8) s += prod[t];
       s += __shfl_xor(s, 4, 8);
@@ -298,17 +316,16 @@ __global__ __launch_bounds__(BLK) void k_it_spmv(const int32_t *__restrict__ rb,
   publish<1>(acc, part);
 }
 
-// q = A32 p on the compressed-column FP32 operator; partials (p, q)
-__global__ __launch_bounds__(BLK) void k_it_spmv16(const int32_t *__restrict__ rb, int64_t nrb,
+// q = A32 p on the compressed-column FP32 operator; partials (p, q).  Held to
+// 64 VGPRs (8 waves per SIMD): the pipelined loop needs every resident wave.
+__global__ __launch_bounds__(BLK, 8) void k_it_spmv16(const int32_t *__restrict__ rb, int64_t nrb,
                                                    const int32_t *__restrict__ rp, const uint16_t *__restrict__ off,
                                                    const int4 *__restrict__ meta, const float *__restrict__ v,
                                                    const float *__restrict__ p, float *__restrict__ q,
                                                    double *__restrict__ part, const double *__restrict__ S, int bank) {
   if (S[bank * B_SIZE + B_DONE] != 0.0) return;
   double acc[1] = {0};
-  Off16Cols cols;
-  cols.off = off;
-  cols.meta = meta;
+  const Off16Cols cols{off, meta};
   rowblock_spmv_c<float, float, float>(rb, nrb, rp, cols, v, p, [&](int32_t r, float s) {
     q[r] = s;
     acc[0] += (double)p[r] * (double)s;
@@ -316,13 +333,11 @@ __global__ __launch_bounds__(BLK) void k_it_spmv16(const int32_t *__restrict__ r
   publish<1>(acc, part);
 }
 
-__global__ __launch_bounds__(BLK) void k_spmv16(const int32_t *__restrict__ rb, int64_t nrb,
+__global__ __launch_bounds__(BLK, 8) void k_spmv16(const int32_t *__restrict__ rb, int64_t nrb,
                                                 const int32_t *__restrict__ rp, const uint16_t *__restrict__ off,
                                                 const int4 *__restrict__ meta, const float *__restrict__ v,
                                                 const float *__restrict__ x, float *__restrict__ y) {
-  Off16Cols cols;
-  cols.off = off;
-  cols.meta = meta;
+  const Off16Cols cols{off, meta};
   rowblock_spmv_c<float, float, float>(rb, nrb, rp, cols, v, x, [&](int32_t r, float s) { y[r] = s; });
 }
 
