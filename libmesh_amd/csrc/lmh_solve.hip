@@ -117,20 +117,24 @@ struct Off16Cols {
   __device__ inline int32_t dec(int32_t o, const Meta &m) const { return o + (o <= m.z ? m.x : m.y); }
 };
 
-// Software-pipelined over the workgroup's row blocks: the value/column loads of
-// block bk+1 are issued right after the x gathers of block bk, so they are in
-// flight while block bk's products are formed and its rows reduced.
+// Software-pipelined over the workgroup's row blocks: the value/column loads
+// and the row pointers of block bk+1 are issued right after the x gathers of
+// block bk, so they are in flight while block bk's products are formed and its
+// rows reduced.  Row pointers of a block with fewer than BLK rows are staged in
+// LDS; FP32 products are double-buffered so one barrier per block suffices.
 template <typename VT, typename XT, typename AT, typename Epi, typename Cols = PlainCols>
 __device__ inline void rowblock_spmv_c(const int32_t *__restrict__ rb, int64_t nrb, const int32_t *__restrict__ rp,
                                        Cols cols, const VT *__restrict__ v, const XT *__restrict__ x, Epi &&epi) {
-  __shared__ AT prod[RB_CAP];
+  constexpr int NBUF = sizeof(AT) == 4 ? 2 : 1;
+  __shared__ AT prodbuf[NBUF][RB_CAP];
+  __shared__ int32_t rowbuf[NBUF][BLK];
   __shared__ double red[BLK / 64];
   constexpr int PER = RB_CAP / BLK;
   int64_t lo, hi;
   block_range(nrb, lo, hi);
   const int tid = threadIdx.x;
   // state of the block whose loads are in flight
-  int32_t r0 = 0, r1 = 0, k0 = 0, cnt = 0;
+  int32_t r0 = 0, r1 = 0, k0 = 0, cnt = 0, rpn = 0;
   typename Cols::Meta meta{};
   VT vv[PER];
   int32_t cr[PER];
@@ -141,6 +145,7 @@ __device__ inline void rowblock_spmv_c(const int32_t *__restrict__ rb, int64_t n
     k0 = rp[r0];
     cnt = rp[r1] - k0;
     meta = cols.meta_of(bk);
+    rpn = rp[r0 + (tid <= r1 - r0 ? tid : 0)];
     if (cnt <= RB_CAP) {
 #pragma unroll
       for (int j = 0; j < PER; ++j) {
@@ -152,14 +157,16 @@ __device__ inline void rowblock_spmv_c(const int32_t *__restrict__ rb, int64_t n
     }
   };
   if (lo < hi) fetch(lo);
+  int buf = 0;
   for (int64_t bk = lo; bk < hi; ++bk) {
-    const int32_t R0 = r0, R1 = r1, K0 = k0, CNT = cnt;
+    const int32_t R0 = r0, R1 = r1, K0 = k0, CNT = cnt, RPN = rpn;
     const typename Cols::Meta M = meta;
     if (CNT > RB_CAP) {  // one long row: reduce it directly
       AT s = 0;
       for (int32_t t = tid; t < CNT; t += BLK)
         s += (AT)__builtin_nontemporal_load(&v[K0 + t]) * (AT)x[cols.dec(cols.raw(K0 + t), M)];
       double w = wave_sum((double)s);
+      __syncthreads();
       if ((tid & 63) == 0) red[tid >> 6] = w;
       __syncthreads();
       if (tid == 0) {
@@ -167,10 +174,12 @@ __device__ inline void rowblock_spmv_c(const int32_t *__restrict__ rb, int64_t n
         for (int k = 0; k < BLK / 64; ++k) t2 += red[k];
         epi(R0, (AT)t2);
       }
-      __syncthreads();
       if (bk + 1 < hi) fetch(bk + 1);
       continue;
     }
+    AT *prod = prodbuf[buf];
+    int32_t *rows = rowbuf[buf];
+    const bool staged = R1 - R0 < BLK;
     AT xx[PER];
     VT cv[PER];
 #pragma unroll
@@ -179,6 +188,8 @@ __device__ inline void rowblock_spmv_c(const int32_t *__restrict__ rb, int64_t n
       cv[j] = vv[j];
     }
     if (bk + 1 < hi) fetch(bk + 1);
+    if (NBUF == 1) __syncthreads();  // the previous block's rows are reduced
+    if (staged && tid <= R1 - R0) rows[tid] = RPN - K0;
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
       const int32_t t = tid + j * BLK;
@@ -187,7 +198,8 @@ __device__ inline void rowblock_spmv_c(const int32_t *__restrict__ rb, int64_t n
     __syncthreads();
     const int g = tid >> 3, sub = tid & 7;
     for (int32_t r = R0 + g; r < R1; r += BLK / 8) {
-      const int32_t a = rp[r] - K0, e = rp[r + 1] - K0;
+      const int32_t a = staged ? rows[r - R0] : rp[r] - K0;
+      const int32_t e = staged ? rows[r - R0 + 1] : rp[r + 1] - K0;
       AT s = 0;
       for (int32_t t = a + sub; t < e; t += 8) s += prod[t];
       s += __shfl_xor(s, 4, 8);
@@ -195,7 +207,7 @@ __device__ inline void rowblock_spmv_c(const int32_t *__restrict__ rb, int64_t n
       s += __shfl_xor(s, 1, 8);
       if (sub == 0) epi(r, s);
     }
-    __syncthreads();
+    buf ^= NBUF - 1;
   }
 }
 
