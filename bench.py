@@ -323,6 +323,39 @@ def main():
         log(f"{mode}: asm {ta*1e3:.2f} ms, solve {ts*1e3:.1f} ms, its {rs[-1][2]}, refreshes {rs[-1][4]}, "
             f"res {rs[-1][3]:.2e}, {N/(ta+ts)/1e6:.2f} Mdof/s, {kname} avg {kavg*1e3:.1f} us x{kcnt}")
 
+    # adjoint sensitivities (all per-element kappa) of J = integral of u over
+    # [0.25,0.75]^3 at the mixed forward solution: QoI derivative, adjoint solve
+    # (mixed, same tolerance), backward kernel
+    dq = L.DeviceArray(ctx, N)
+    lam = L.DeviceArray(ctx, N)
+    grad = L.DeviceArray(ctx, p.n_elem)
+    lam.zero()
+    ctx.sync()
+    ctx.profile_reset()
+    ctx.profile(True)
+    ta0 = time.perf_counter()
+    g.qoi_box_derivative(3, 0.25, 0.75, dq)
+    a_its, a_res, a_reason, a_st = g.mixed(dq, lam, args.rtol, args.inner, 0, 20000)
+    g.adjoint_kappa_gradient(3, kd, x, lam, grad)
+    ctx.sync()
+    ta1 = time.perf_counter()
+    ctx.profile(False)
+    gk_ms, _ = ctx.profile_query("adjoint_kappa")
+    qd_ms, _ = ctx.profile_query("qoi_box")
+    gh = grad.download()
+    J = float(dq.download() @ results["mixed"]["x"])
+    adjoint = {
+        "gradients": int(p.n_elem),
+        "total_ms": round((ta1 - ta0) * 1e3, 3),
+        "qoi_derivative_ms": round(qd_ms, 4),
+        "adjoint_solve_iterations": int(a_its),
+        "backward_kernel_ms": round(gk_ms, 4),
+        "backward_kernel_tflops_fp64": round(9.7e9 / (gk_ms * 1e-3) / 1e12, 3),
+        "J": J,
+        "homogeneity_rel_err": abs(float(kappa @ gh) + J) / abs(J),
+    }
+    log(f"adjoint: {adjoint}")
+
     # assembly kernel timings (FP64 path, one profiled assembly)
     ctx.profile_reset()
     ctx.profile(True)
@@ -407,6 +440,7 @@ def main():
                                "algorithmic_bytes": alg64,
                                "avg_working_launch_ms": round(working_ms(results["fp64"]), 5)},
         "assembly_kernels_ms": {k: round(v, 4) for k, v in asm_k.items()},
+        "adjoint": adjoint,
         "contraction_gflop": flops_contract / 1e9,
         "setup_s": round(t_setup, 3),
     }

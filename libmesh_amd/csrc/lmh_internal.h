@@ -129,6 +129,7 @@ struct lmh_matrix {
   uint16_t *off16 = nullptr;  // nnz
   int4 *rbmeta = nullptr;     // nrb
   uint16_t *off = nullptr;  // n_elem*dpe*dpe slot offsets within rows
+  int32_t *diag = nullptr;  // rows: position of the diagonal entry, -1 if absent
   int ncolors = 0;
   std::vector<int64_t> color_ptr;  // host, ncolors+1
   int32_t *color_elems = nullptr;  // device, n_elem (grouped by colour, ascending id)

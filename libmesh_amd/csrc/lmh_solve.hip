@@ -256,10 +256,10 @@ __global__ __launch_bounds__(BLK) void k_fin_residual(const double *__restrict__
   }
 }
 
-// Jacobi inverse diagonal: 1/a_rr, or 1 when the diagonal is absent or zero
-template <typename VT, typename OT>
-__global__ void k_invdiag(int64_t n, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                          const VT *__restrict__ v, OT *__restrict__ dinv) {
+// Position of each row's diagonal entry (binary search of the sorted columns),
+// -1 when the row stores none.  Depends on the pattern only.
+__global__ void k_diag_index(int64_t n, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                             int32_t *__restrict__ diag) {
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     int32_t lo = rp[r], hi = rp[r + 1];
     const int32_t end = hi;
@@ -270,8 +270,18 @@ __global__ void k_invdiag(int64_t n, const int32_t *__restrict__ rp, const int32
       else
         hi = mid;
     }
+    diag[r] = (lo < end && ci[lo] == r) ? lo : -1;
+  }
+}
+
+// Jacobi inverse diagonal: 1/a_rr, or 1 when the diagonal is absent or zero
+template <typename VT, typename OT>
+__global__ void k_invdiag(int64_t n, const int32_t *__restrict__ diag, const VT *__restrict__ v,
+                          OT *__restrict__ dinv) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t k = diag[r];
     OT d = (OT)1;
-    if (lo < end && ci[lo] == r && v[lo] != (VT)0) d = (OT)1 / (OT)v[lo];
+    if (k >= 0 && v[k] != (VT)0) d = (OT)1 / (OT)v[k];
     dinv[r] = d;
   }
 }
@@ -675,6 +685,14 @@ static int ensure_off16(lmh_matrix *A) {
   return LMH_OK;
 }
 
+static int ensure_diag(lmh_matrix *A) {
+  if (A->diag) return LMH_OK;
+  LMH_HIP(hipMalloc((void **)&A->diag, sizeof(int32_t) * std::max<int64_t>(1, A->rows)));
+  LMH_LAUNCH(A->ctx, "diag_index", k_diag_index, dim3(grid_for(A->rows, 256, 8192)), dim3(256), 0, A->rows, A->rp,
+             A->ci, A->diag);
+  return check_launch("k_diag_index");
+}
+
 static bool use_off16(const lmh_matrix *A) { return A->fp32_index && A->idx16_state == 1; }
 
 static int ensure_work(lmh_matrix *A, bool mixed) {
@@ -767,7 +785,8 @@ int lmh_cg_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int
   double *S = ctx->d_scal;
   double *part1 = ctx->d_part, *part2 = ctx->d_part + 2 * ctx->n_part;  // 1 and 2 values per block
   const int g = vec_grid(ctx);
-  LMH_LAUNCH(ctx, "invdiag", (k_invdiag<double, double>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, n, A->rp, A->ci,
+  LMH_TRY(ensure_diag(A));
+  LMH_LAUNCH(ctx, "invdiag", (k_invdiag<double, double>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, n, A->diag,
              A->vals, dinv);
   LMH_LAUNCH(ctx, "residual_fp64", k_residual, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals, b_d, x_d,
              r, part2);
@@ -839,7 +858,8 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
   init[G_DELTA2] = inner_rtol * inner_rtol;
   init[G_MODE] = mode;
   LMH_HIP(hipMemcpyAsync(S, init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
-  LMH_LAUNCH(ctx, "invdiag", (k_invdiag<float, float>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, n, A->rp, A->ci,
+  LMH_TRY(ensure_diag(A));
+  LMH_LAUNCH(ctx, "invdiag", (k_invdiag<float, float>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, n, A->diag,
              A->vals32, d32);
   LMH_LAUNCH(ctx, "mx_refresh_fp64", k_refresh, dim3(g), dim3(BLK), 0, 1, A->rb, A->nrb, A->rp, A->ci, A->vals, b_d,
              x_d, d32, r32, p32, part2, S, 0);

@@ -289,6 +289,7 @@ int lmh_matrix_destroy(lmh_matrix *A) {
   (void)hipFree(A->vals);
   (void)hipFree(A->vals32);
   (void)hipFree(A->off);
+  (void)hipFree(A->diag);
   (void)hipFree(A->color_elems);
   (void)hipFree(A->Ke);
   (void)hipFree(A->Fe);
