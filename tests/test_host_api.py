@@ -37,3 +37,21 @@ def test_bad_arguments():
         L.Problem(0, L.HEX27)
     with pytest.raises(L.LmhError):
         L.Problem(2, 99)
+
+
+def test_batch_recipe():
+    """Seeded batch recipe: deterministic, edges in [nmin, nmax] spread
+    log-uniformly, all four element types drawn."""
+    e1, t1 = L.batch_recipe(0x5EED0003, 64, 8, 48)
+    e2, t2 = L.batch_recipe(0x5EED0003, 64, 8, 48)
+    assert np.array_equal(e1, e2) and np.array_equal(t1, t2)
+    assert e1.min() >= 8 and e1.max() <= 48
+    assert set(t1.tolist()) == {L.HEX8, L.HEX27, L.TET4, L.TET10}
+    e, t = L.batch_recipe(7, 4000, 8, 48)
+    # log-uniform: the median edge is near the geometric mean sqrt(8*48) ~ 19.6
+    assert 17 <= np.median(e) <= 22
+    assert abs((e <= 12).mean() - np.log(12.5 / 8) / np.log(48 / 8)) < 0.05
+    for k in (L.HEX8, L.HEX27, L.TET4, L.TET10):
+        assert abs((t == k).mean() - 0.25) < 0.04
+    e3, _ = L.batch_recipe(0x5EED0004, 64, 8, 48)
+    assert not np.array_equal(e1, e3)
