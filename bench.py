@@ -269,16 +269,22 @@ def main():
     if copy:
         log(f"copy kernel: {copy['gbs']} GB/s ({args.copy_mib} MiB, avg {copy['avg_ms']} ms)")
 
-    def step(mixed):
+    def step(mode):
+        """mode fp64: FP64 assembly + FP64 CG.  mixed: FP64 assembly, FP32 inner
+        operator rounded from it, mixed solve.  mixed_mfma32: FP64 assembly for
+        the outer residual operator plus an FP32-MFMA assembly of the inner
+        operator, mixed solve."""
         x.zero()
         ctx.sync()
         ta = time.perf_counter()
         g.assemble_poisson(3, kd, 0, b, L.FP64, scatter)
-        if mixed:
+        if mode == "mixed":
             g.make_fp32()
+        elif mode == "mixed_mfma32":
+            g.assemble_poisson(3, kd, 0, b, L.FP32, scatter)
         ctx.sync()
         tb = time.perf_counter()
-        if mixed:
+        if mode != "fp64":
             its, res, reason, stats = g.mixed(b, x, args.rtol, args.inner, 0, 20000)
             nref = int(stats[1])
         else:
@@ -291,13 +297,13 @@ def main():
         return tb - ta, tc - tb, its, res, nref
 
     results = {}
-    for mode in ("fp64", "mixed"):
-        mixed = mode == "mixed"
+    for mode in ("fp64", "mixed_mfma32", "mixed"):
+        mixed = mode != "fp64"
         for _ in range(args.warmup):
-            step(mixed)
+            step(mode)
         ctx.sync()
         T0 = time.perf_counter()
-        rs = [step(mixed) for _ in range(args.steps)]
+        rs = [step(mode) for _ in range(args.steps)]
         ctx.sync()
         T1 = time.perf_counter()
         # kernel timing: the same steps again with HIP events around every
@@ -307,7 +313,7 @@ def main():
         ctx.profile(True)
         Tp0 = time.perf_counter()
         for _ in range(args.steps):
-            step(mixed)
+            step(mode)
         ctx.sync()
         Tp1 = time.perf_counter()
         ctx.profile(False)
@@ -389,6 +395,8 @@ def main():
     x64 = results["fp64"].pop("x")
     xmx = results["mixed"].pop("x")
     err_mixed_vs_fp64 = float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))
+    x32m = results["mixed_mfma32"].pop("x")
+    err_mfma32_vs_fp64 = float(np.linalg.norm(x32m - x64) / np.linalg.norm(x64))
 
     mx = results["mixed"]
 
@@ -442,6 +450,9 @@ def main():
         },
         "fp64": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in results["fp64"].items()},
         "mixed": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in results["mixed"].items()},
+        "mixed_mfma32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in results["mixed_mfma32"].items()},
+        "speedup_mixed_mfma32_vs_fp64": round(results["mixed_mfma32"]["mdofs"] / results["fp64"]["mdofs"], 4),
+        "mixed_mfma32_rel_diff_vs_fp64_solution": err_mfma32_vs_fp64,
         "speedup_mixed_vs_fp64": round(mx["mdofs"] / results["fp64"]["mdofs"], 4),
         "mixed_rel_diff_vs_fp64_solution": err_mixed_vs_fp64,
         "fp64_spmv_roofline": {"kernel_symbol": "lmh::k_it_spmv<double>", "achieved": round(ach64, 1),

@@ -401,7 +401,9 @@ __global__ __launch_bounds__(64) void k_scatter_gather(const int32_t *__restrict
   extern __shared__ unsigned char smraw[];
   T *acc = (T *)smraw;
   const int lane = threadIdx.x;
-  for (int64_t r = blockIdx.x; r < n_rows; r += gridDim.x) {
+  int64_t rlo, rhi;
+  block_range(n_rows, rlo, rhi);  // contiguous rows per workgroup, per XCD
+  for (int64_t r = rlo; r < rhi; ++r) {
     const int32_t b = rp[r], len = rp[r + 1] - b;
     for (int t = lane; t < len; t += 64) acc[t] = (T)0;
     __syncthreads();

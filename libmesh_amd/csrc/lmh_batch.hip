@@ -46,7 +46,7 @@ struct BScat {
   float *vals;
   double *rhs;
   int64_t rows;
-  int dpe, pad;
+  int dpe, w;  // element DoFs, lanes per row (16, 32 or 64)
 };
 
 struct BRTile {
@@ -70,11 +70,11 @@ inline int type_slot(int type) {
 
 // LDS bytes of one element slot: float4 gradients [nq][n] (coordinates overlay
 // them before the gradients are formed), J^{-1} [nq][9] and JxW [nq] in FP64,
-// body-force vector [n] in FP64.
+// body-force vector [n] in FP64, constraint flags of the 3n element DoFs.
 __host__ __device__ inline int slot_bytes(int n, int nq) {
   int gp = nq * n * 16;
   if (gp < n * 24) gp = n * 24;
-  const int s = gp + nq * 80 + n * 8;
+  const int s = gp + nq * 80 + n * 8 + ((3 * n + 15) & ~15);
   return (s + 15) & ~15;
 }
 
@@ -101,6 +101,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_elasticity(const BProb *_
     double *Gq = (double *)(S + gpb);
     double *Jw = Gq + nq * 9;
     double *Fz = Jw + nq;
+    uint8_t *cf = (uint8_t *)(Fz + n);  // constraint flag per element DoF
     const double *w = P.fe, *phi = P.fe + nq, *dref = P.fe + nq + (size_t)n * nq;
     for (int64_t eb = tl.e0; eb < tl.e1; eb += epp) {
       const int64_t e = eb + sub;
@@ -109,6 +110,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_elasticity(const BProb *_
         for (int k = l; k < n * 3; k += L) {
           const int a = k / 3, c = k - a * 3;
           X[k] = P.xyz[(int64_t)P.conn[e * n + a] * 3 + c];
+          cf[k] = P.cons[P.edofs[e * dpe + k]];
         }
       __syncthreads();
       if (act)
@@ -147,7 +149,6 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_elasticity(const BProb *_
       __syncthreads();
       if (act) {
         float *K = P.Ke + e * (int64_t)dpe * dpe;
-        const int32_t *ed = P.edofs + e * dpe;
         const int npair = n * (n + 1) / 2;
         for (int k = l; k < npair; k += L) {
           int a = 0, rem = k;
@@ -183,7 +184,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_elasticity(const BProb *_
               const int r = i * n + a, s = c * n + b;
               // element constraint (dof_map_constraints.C:2518-2616): zero row
               // and column of a constrained DoF, unit diagonal
-              if (P.cons[ed[r]] | P.cons[ed[s]]) v = (r == s) ? 1.f : 0.f;
+              if (cf[r] | cf[s]) v = (r == s) ? 1.f : 0.f;
               K[(int64_t)r * dpe + s] = v;
               if (a != b) K[(int64_t)s * dpe + r] = v;
             }
@@ -191,7 +192,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_elasticity(const BProb *_
         for (int k = l; k < dpe; k += L) {
           const int i = k / n, a = k - i * n;
           double f = (i == 2) ? Fz[a] : 0.0;
-          if (P.cons[ed[k]]) f = 0.0;
+          if (cf[k]) f = 0.0;
           P.Fe[e * dpe + k] = f;
         }
       }
@@ -200,33 +201,91 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_elasticity(const BProb *_
   }
 }
 
-// Row-gather scatter over all problems: one wave per row sums the row's
-// element contributions in element order in LDS and writes the row once
-// (SparseMatrix::add_matrix / NumericVector::add_vector for every element).
+// Row-gather scatter over all problems (SparseMatrix::add_matrix /
+// NumericVector::add_vector for every element): each CSR row is summed by a
+// sub-group of W lanes (W >= the element's DoF count up to 64; 64/W rows per
+// wave) in an LDS accumulator, contributions added in element order, the row
+// written once.  A row's incidences are taken in chunks: the chunk's element
+// ids are loaded at once, then all of its (offset, value) pairs (kScR per lane),
+// then they are added one incidence after the other from registers.
+constexpr int kScR = 8;      // contributions per lane per chunk
+constexpr int kScChunk = 64; // incidences per chunk at most
+
 __global__ __launch_bounds__(64) void k_batch_scatter(const BScat *__restrict__ probs,
-                                                      const BRTile *__restrict__ tiles, int ntiles) {
-  extern __shared__ float acc[];
+                                                      const BRTile *__restrict__ tiles, int ntiles, int stride) {
+  extern __shared__ __align__(16) unsigned char scsm[];
   const int lane = threadIdx.x;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const BRTile tl = tiles[t];
     const BScat P = probs[tl.p];
-    const int dpe = P.dpe;
-    for (int32_t r = tl.r0; r < tl.r1; ++r) {
-      const int32_t b = P.rp[r], len = P.rp[r + 1] - b;
-      for (int k = lane; k < len; k += 64) acc[k] = 0.f;
-      __syncthreads();
-      const int32_t k0 = P.inc_ptr[r], k1 = P.inc_ptr[r + 1];
-      double f = 0.0;
-      for (int32_t k = k0; k < k1; ++k) {
-        const int64_t ei = P.inc[k];
-        const int64_t base = ei * dpe;
-        for (int j = lane; j < dpe; j += 64) acc[P.off[base + j]] += P.Ke[base + j];
-        f += P.Fe[ei];
-        __syncthreads();
+    const int dpe = P.dpe, W = P.w, G = 64 / W;
+    const int sg = lane / W, l = lane - sg * W;
+    // per sub-group: acc[stride] floats | element ids [kScChunk] | Fe [kScChunk]
+    unsigned char *base = scsm + (size_t)sg * (stride * 4 + kScChunk * 12);
+    float *acc = (float *)base;
+    double *febuf = (double *)(base + stride * 4);
+    int32_t *incbuf = (int32_t *)(base + stride * 4 + kScChunk * 8);
+    const int ch = min(min((kScR * W) / dpe, W), kScChunk);  // incidences per chunk
+    for (int32_t rb = tl.r0; rb < tl.r1; rb += G) {
+      const int32_t r = rb + sg;
+      const bool act = r < tl.r1;
+      int32_t b = 0, len = 0, k0 = 0, cnt = 0;
+      if (act) {
+        b = P.rp[r];
+        len = P.rp[r + 1] - b;
+        k0 = P.inc_ptr[r];
+        cnt = P.inc_ptr[r + 1] - k0;
       }
-      for (int k = lane; k < len; k += 64) P.vals[b + k] = acc[k];
-      if (lane == 0) P.rhs[r] = f;
-      __syncthreads();
+      for (int k = l; k < len; k += W) acc[k] = 0.f;
+      int mx = cnt;
+      for (int o = W; o < 64; o <<= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+      double f = 0.0;
+      for (int c0 = 0; c0 < mx; c0 += ch) {
+        const int nch = min(ch, cnt - c0);  // incidences of this chunk (<= 0: none)
+        __builtin_amdgcn_wave_barrier();
+        if (l < nch) {
+          const int32_t ei = P.inc[k0 + c0 + l];
+          incbuf[l] = ei;
+          febuf[l] = P.Fe[ei];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        uint16_t oq[kScR];
+        float vq[kScR];
+#pragma unroll
+        for (int q = 0; q < kScR; ++q) {
+          const int tq = q * W + l;
+          const int kk = tq / dpe;
+          oq[q] = 0;
+          vq[q] = 0.f;
+          if (kk < nch) {
+            const int64_t at = (int64_t)incbuf[kk] * dpe + (tq - kk * dpe);
+            oq[q] = P.off[at];
+            vq[q] = P.Ke[at];
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < kScR; ++q) {
+          const int tq = q * W + l;
+          const int kk = tq / dpe;
+          const int lo = (q * W) / dpe, hi = (q * W + W - 1) / dpe;
+          for (int k = lo; k <= hi; ++k) {
+            if (k == kk && kk < nch) acc[oq[q]] += vq[q];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          }
+        }
+        if (l == 0)
+          for (int k = 0; k < nch; ++k) f += febuf[k];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (int k = l; k < len; k += W) P.vals[b + k] = acc[k];
+      if (act && l == 0) P.rhs[r] = f;
+      __builtin_amdgcn_wave_barrier();
     }
   }
 }
@@ -259,6 +318,11 @@ struct lmh_batch {
 };
 
 namespace lmh {
+
+// LDS of the scatter: 4 sub-groups at most, each an accumulator row (stride
+// floats, a multiple of 4) plus a chunk's element ids and Fe values
+static int scatter_stride(int max_row) { return (max_row + 3) & ~3; }
+static size_t scatter_lds(int max_row) { return 4 * ((size_t)scatter_stride(max_row) * 4 + kScChunk * 12); }
 
 static bool config_fits(int n, int nq, int L) {
   return L >= 16 && L <= kBatchBlock && (kBatchBlock / L) * slot_bytes(n, nq) <= kBatchLds;
@@ -313,8 +377,8 @@ static int build_graph(lmh_batch *b, double E, double nu, double fz) {
     hipLaunchKernelGGL(k_batch_elasticity, dim3(b->grid_elem), dim3(kBatchBlock), kBatchLds, ctx->stream,
                        (const BProb *)b->dprob, (const BTile *)b->dtile, b->ntile, l1, l2, fz, ctx->d_flags);
   if (b->nrt > 0)
-    hipLaunchKernelGGL(k_batch_scatter, dim3(b->grid_scat), dim3(64), sizeof(float) * b->max_row, ctx->stream,
-                       (const BScat *)b->dscat, (const BRTile *)b->drt, b->nrt);
+    hipLaunchKernelGGL(k_batch_scatter, dim3(b->grid_scat), dim3(64), scatter_lds(b->max_row), ctx->stream,
+                       (const BScat *)b->dscat, (const BRTile *)b->drt, b->nrt, scatter_stride(b->max_row));
   const hipError_t ce = hipStreamEndCapture(ctx->stream, &b->graph);
   LMH_HIP(ce);
   LMH_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
@@ -387,8 +451,9 @@ static int batch_setup(lmh_batch *b, lmh_context *ctx, int n_prob, const lmh_dof
   std::vector<BRTile> rt;
   for (int p = 0; p < n_prob; ++p) {
     lmh_matrix *A = b->mats[p];
+    const int dpe = b->dms[p]->dpe;
     sc[p] = BScat{b->dms[p]->inc_ptr, b->dms[p]->inc, A->rp, A->off, A->Ke32, A->Fe, A->vals32, b->rhs[p],
-                  A->rows, b->dms[p]->dpe, 0};
+                  A->rows, dpe, dpe <= 16 ? 16 : dpe <= 32 ? 32 : 64};
     for (int64_t r = 0; r < A->rows; r += kRowsPerTile)
       rt.push_back(BRTile{p, (int32_t)r, (int32_t)std::min<int64_t>(A->rows, r + kRowsPerTile), 0});
   }

@@ -208,6 +208,23 @@ __device__ inline double qp_map(const double *X, const double *dref, int n, int 
   return det;
 }
 
+// MI355X dispatches workgroups round-robin over its 8 XCDs; remap so each XCD
+// owns one contiguous range of logical blocks (its L2 then holds the data its
+// neighbouring rows share).
+constexpr int NXCD = 8;
+__device__ inline int logical_block() {
+  const int G = gridDim.x, b = blockIdx.x;
+  if (G % NXCD) return b;
+  return (b % NXCD) * (G / NXCD) + b / NXCD;
+}
+// this workgroup's contiguous share [lo, hi) of n items
+__device__ inline void block_range(int64_t n, int64_t &lo, int64_t &hi) {
+  const int lb = logical_block();
+  const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+  lo = (int64_t)lb * per;
+  hi = lo + per < n ? lo + per : n;
+}
+
 // grid for an XCD-aware 1-D launch
 inline int grid_for(int64_t work, int per_block, int max_blocks) {
   int64_t g = (work + per_block - 1) / per_block;

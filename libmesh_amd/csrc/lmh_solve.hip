@@ -17,27 +17,8 @@
 
 namespace lmh {
 
-
-
-
 constexpr int BLK = 256;
-constexpr int NXCD = 8;
 constexpr int RB_CAP = LMH_RB_CAP;  // stored entries per SpMV row block
-
-// Blocks are dispatched round-robin over the XCDs; remap so each XCD owns one
-// contiguous range of rows (its L2 then holds the x entries its rows gather).
-__device__ inline int logical_block() {
-  const int G = gridDim.x, b = blockIdx.x;
-  if (G % NXCD) return b;
-  return (b % NXCD) * (G / NXCD) + b / NXCD;
-}
-
-__device__ inline void block_range(int64_t n, int64_t &lo, int64_t &hi) {
-  const int lb = logical_block();
-  const int64_t per = (n + gridDim.x - 1) / gridDim.x;
-  lo = (int64_t)lb * per;
-  hi = lo + per < n ? lo + per : n;
-}
 
 template <typename T>
 __device__ inline T wave_sum(T v) {
@@ -211,6 +192,85 @@ __device__ inline void rowblock_spmv_c(const int32_t *__restrict__ rb, int64_t n
   }
 }
 
+// FP32 values with compressed columns: the 16-bit column offsets of a block
+// are read as aligned 8-entry chunks (one 16-byte load per thread instead of
+// eight 2-byte loads) and passed through LDS so that gathers and products keep
+// the consecutive-entries-per-lane layout of rowblock_spmv_c (same sums, same
+// order).  Row blocks hold at most RB_CAP - 8 entries, so a block's chunks
+// (its start rounded down to 8 entries) fit one per thread.
+typedef uint32_t vu4 __attribute__((ext_vector_type(4)));
+template <typename Epi>
+__device__ inline void rowblock_spmv16(const int32_t *__restrict__ rb, int64_t nrb, const int32_t *__restrict__ rp,
+                                       const uint16_t *__restrict__ off, const int4 *__restrict__ meta,
+                                       const float *__restrict__ v, const float *__restrict__ x, Epi &&epi) {
+  __shared__ float prod[RB_CAP];
+  __shared__ int32_t rowbuf[2][BLK];  // double-buffered: written before the first barrier
+  __shared__ vu4 colbuf[BLK];  // 8 offsets per thread
+  const uint16_t *cb = (const uint16_t *)colbuf;
+  constexpr int PER = RB_CAP / BLK;
+  static_assert(PER == 8, "one 8-entry offset chunk per thread");
+  int64_t lo, hi;
+  block_range(nrb, lo, hi);
+  const int tid = threadIdx.x;
+  int32_t r0 = 0, r1 = 0, k0 = 0, cnt = 0, rpn = 0;
+  int4 m{};
+  float vv[PER];
+  vu4 ov;
+  auto fetch = [&](int64_t bk) {
+    r0 = rb[bk];
+    r1 = rb[bk + 1];
+    k0 = rp[r0];
+    cnt = rp[r1] - k0;
+    m = meta[bk];
+    rpn = rp[r0 + (tid <= r1 - r0 ? tid : 0)];
+    const int32_t a0 = k0 & ~7, span = (k0 - a0) + cnt;
+    ov = __builtin_nontemporal_load((const vu4 *)(off + a0 + 8 * (8 * tid < span ? tid : 0)));
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int32_t t = tid + j * BLK;
+      vv[j] = __builtin_nontemporal_load(&v[k0 + (t < cnt ? t : 0)]);
+    }
+  };
+  if (lo < hi) fetch(lo);
+  int buf = 0;
+  for (int64_t bk = lo; bk < hi; ++bk) {
+    int32_t *rows = rowbuf[buf];
+    buf ^= 1;
+    const int32_t R0 = r0, R1 = r1, K0 = k0, CNT = cnt, RPN = rpn, lead = K0 & 7;
+    const int4 M = m;
+    const bool staged = R1 - R0 < BLK;
+    colbuf[tid] = ov;
+    if (staged && tid <= R1 - R0) rows[tid] = RPN - K0;
+    __syncthreads();  // offsets and row pointers visible; the previous block's rows are reduced
+    float xx[PER], cv[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int32_t t = tid + j * BLK;
+      const int32_t o = t < CNT ? cb[t + lead] : 0;
+      xx[j] = x[t < CNT ? o + (o <= M.z ? M.x : M.y) : 0];
+      cv[j] = vv[j];
+    }
+    if (bk + 1 < hi) fetch(bk + 1);
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int32_t t = tid + j * BLK;
+      if (t < CNT) prod[t] = cv[j] * xx[j];
+    }
+    __syncthreads();
+    const int g = tid >> 3, sub = tid & 7;
+    for (int32_t r = R0 + g; r < R1; r += BLK / 8) {
+      const int32_t a = staged ? rows[r - R0] : rp[r] - K0;
+      const int32_t e = staged ? rows[r - R0 + 1] : rp[r + 1] - K0;
+      float s = 0;
+      for (int32_t t = a + sub; t < e; t += 8) s += prod[t];
+      s += __shfl_xor(s, 4, 8);
+      s += __shfl_xor(s, 2, 8);
+      s += __shfl_xor(s, 1, 8);
+      if (sub == 0) epi(r, s);
+    }
+  }
+}
+
 template <typename VT, typename XT, typename AT, typename Epi>
 __device__ inline void rowblock_spmv(const int32_t *__restrict__ rb, int64_t nrb, const int32_t *__restrict__ rp,
                                      const int32_t *__restrict__ ci, const VT *__restrict__ v,
@@ -347,8 +407,7 @@ __global__ __launch_bounds__(BLK, 8) void k_it_spmv16(const int32_t *__restrict_
                                                    double *__restrict__ part, const double *__restrict__ S, int bank) {
   if (S[bank * B_SIZE + B_DONE] != 0.0) return;
   double acc[1] = {0};
-  const Off16Cols cols{off, meta};
-  rowblock_spmv_c<float, float, float>(rb, nrb, rp, cols, v, p, [&](int32_t r, float s) {
+  rowblock_spmv16(rb, nrb, rp, off, meta, v, p, [&](int32_t r, float s) {
     q[r] = s;
     acc[0] += (double)p[r] * (double)s;
   });
@@ -359,8 +418,7 @@ __global__ __launch_bounds__(BLK, 8) void k_spmv16(const int32_t *__restrict__ r
                                                 const int32_t *__restrict__ rp, const uint16_t *__restrict__ off,
                                                 const int4 *__restrict__ meta, const float *__restrict__ v,
                                                 const float *__restrict__ x, float *__restrict__ y) {
-  const Off16Cols cols{off, meta};
-  rowblock_spmv_c<float, float, float>(rb, nrb, rp, cols, v, x, [&](int32_t r, float s) { y[r] = s; });
+  rowblock_spmv16(rb, nrb, rp, off, meta, v, x, [&](int32_t r, float s) { y[r] = s; });
 }
 
 // Build the compressed columns of one row block per workgroup: sort the
@@ -375,7 +433,7 @@ __global__ __launch_bounds__(BLK) void k_build_off16(const int32_t *__restrict__
   const int tid = threadIdx.x;
   for (int64_t bk = blockIdx.x; bk < nrb; bk += gridDim.x) {
     const int32_t k0 = rp[rb[bk]], cnt = rp[rb[bk + 1]] - k0;
-    if (cnt > RB_CAP) {
+    if ((k0 & 7) + cnt > RB_CAP) {  // does not fit one 8-entry offset chunk per thread
       if (tid == 0) atomicOr(bad, 1);
       continue;
     }

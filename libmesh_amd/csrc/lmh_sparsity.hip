@@ -140,8 +140,9 @@ static int color_elements(const std::vector<int32_t> &edofs, int64_t n_elem, int
 
 static int build_row_blocks(lmh_matrix *A, const std::vector<int32_t> &rp) {
   // SpMV row blocks: consecutive rows holding at most kRowBlockCap stored
-  // entries (a longer row forms a block of its own)
-  const int kRowBlockCap = LMH_RB_CAP;
+  // entries (a longer row forms a block of its own).  The cap leaves room for
+  // the FP32 SpMV's alignment of a block's start down to 8 entries.
+  const int kRowBlockCap = LMH_RB_CAP - 8;
   std::vector<int32_t> rb{0};
   const int64_t n = A->rows;
   for (int64_t r = 0; r < n; ++r)
