@@ -87,6 +87,15 @@ def cpu_baseline(n_sample):
     import oracle as O
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    cpu_model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     t0 = time.perf_counter()
     m = O.Mesh(n_sample, O.HEX27)
     m.perturb(SEED_PERTURB, 0.2)
@@ -105,7 +114,7 @@ def cpu_baseline(n_sample):
         "kind": "port",
         "sample": f"{n_sample}^3 HEX27 ({s.n_dofs} dofs, same recipe), oracle assembly (serial element loop) "
                   f"{t2 - t1:.2f}s + Eigen-restated Jacobi-CG {its} its {t3 - t2:.2f}s (OpenMP SpMV), setup "
-                  f"{t1 - t0:.2f}s excluded",
+                  f"{t1 - t0:.2f}s excluded; host {cpu_model}, {os.cpu_count()} logical CPUs visible",
     }
 
 
