@@ -35,6 +35,18 @@ void lmesh_lognormal(int64_t n, uint64_t seed, double *out);
  * TET10 (libMesh ElemType values). */
 void lmesh_batch_recipe(uint64_t seed, int count, int nmin, int nmax, int *edge, int *elem_type);
 
+/* Sparse matrix files as SparseMatrix::read takes them (sparse_matrix.C:686):
+ * Matlab text (.m, .matlab, optionally .gz) or PETSc binary (.petsc32,
+ * .petsc64).  Call with row/col/val NULL to get m, n and the entry count in
+ * *nnz, then with arrays of *nnz entries (0-based indices, file order).
+ * Returns 0, or -1 with the message in lmesh_last_error. */
+int lmesh_matrix_read(const char *filename, int64_t *m, int64_t *n, int64_t *nnz, int64_t *row, int64_t *col,
+                      double *val);
+/* Matlab text of a CSR matrix in SparseMatrix::print_matlab's layout
+ * (sparse_matrix.C:398-491), 17 significant digits, exact zeros skipped. */
+int lmesh_matrix_write_matlab(const char *filename, int64_t m, int64_t n, const int32_t *row_ptr,
+                              const int32_t *col_idx, const double *vals);
+
 int64_t lmesh_n_nodes(const lmesh_problem *p);
 int64_t lmesh_n_elem(const lmesh_problem *p);
 int64_t lmesh_n_dofs(const lmesh_problem *p);

@@ -103,8 +103,14 @@ int lmh_matrix_create(lmh_context *ctx, const lmh_dofmap *dm, lmh_matrix **out);
  * values zeroed).  It has no DoF map: element assembly calls reject it. */
 int lmh_matrix_create_csr(lmh_context *ctx, int64_t rows, const int32_t *row_ptr_h, const int32_t *col_idx_h,
                           lmh_matrix **out);
+/* Rectangular variant (rows x cols), for matrices read from files
+ * (SparseMatrix::init(m, n, ...) with m != n, sparse_matrix.h); SpMV takes x of
+ * length cols.  The solvers require rows == cols. */
+int lmh_matrix_create_csr2(lmh_context *ctx, int64_t rows, int64_t cols, const int32_t *row_ptr_h,
+                           const int32_t *col_idx_h, lmh_matrix **out);
 int lmh_matrix_destroy(lmh_matrix *A);
 int64_t lmh_matrix_rows(const lmh_matrix *A);
+int64_t lmh_matrix_cols(const lmh_matrix *A);
 int64_t lmh_matrix_nnz(const lmh_matrix *A);
 int lmh_matrix_n_colors(const lmh_matrix *A);
 /* Download the pattern (row_ptr_h: rows+1, col_idx_h: nnz). */

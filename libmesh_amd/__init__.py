@@ -62,6 +62,8 @@ HIP_API = {
     "lmh_matrix_create": (_I, [_P, _P, C.POINTER(_P)]),
     "lmh_matrix_create_csr": (_I, [_P, _I64, _P, _P, C.POINTER(_P)]),
     "lmh_matrix_destroy": (_I, [_P]),
+    "lmh_matrix_cols": (_I64, [_P]),
+    "lmh_matrix_create_csr2": (_I, [_P, _I64, _I64, _P, _P, C.POINTER(_P)]),
     "lmh_matrix_rows": (_I64, [_P]),
     "lmh_matrix_nnz": (_I64, [_P]),
     "lmh_matrix_n_colors": (_I, [_P]),
@@ -107,6 +109,8 @@ HOST_API = {
     "lmesh_perturb": (None, [_P, C.c_uint64, _D]),
     "lmesh_lognormal": (None, [_I64, C.c_uint64, _P]),
     "lmesh_batch_recipe": (None, [C.c_uint64, _I, _I, _I, _P, _P]),
+    "lmesh_matrix_read": (_I, [C.c_char_p, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), _P, _P, _P]),
+    "lmesh_matrix_write_matlab": (_I, [C.c_char_p, _I64, _I64, _P, _P, _P]),
     "lmesh_n_nodes": (_I64, [_P]),
     "lmesh_n_elem": (_I64, [_P]),
     "lmesh_n_dofs": (_I64, [_P]),
@@ -202,6 +206,31 @@ def lognormal(n, seed):
     out = np.empty(n)
     host().lmesh_lognormal(n, seed, _ptr(out))
     return out
+
+
+def read_matrix(filename):
+    """(m, n, rows, cols, values) of a sparse matrix file (Matlab text, gzipped
+    or not, or PETSc binary), entries in file order, 0-based."""
+    L = host()
+    m, n, nnz = _I64(), _I64(), _I64()
+    fn = str(filename).encode()
+    if L.lmesh_matrix_read(fn, C.byref(m), C.byref(n), C.byref(nnz), None, None, None) != 0:
+        raise LmhError(L.lmesh_last_error().decode())
+    i = np.empty(nnz.value, dtype=np.int64)
+    j = np.empty(nnz.value, dtype=np.int64)
+    v = np.empty(nnz.value)
+    if L.lmesh_matrix_read(fn, C.byref(m), C.byref(n), C.byref(nnz), _ptr(i), _ptr(j), _ptr(v)) != 0:
+        raise LmhError(L.lmesh_last_error().decode())
+    return m.value, n.value, i, j, v
+
+
+def write_matrix_matlab(filename, m, n, row_ptr, col_idx, vals):
+    L = host()
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    ci = np.ascontiguousarray(col_idx, dtype=np.int32)
+    v = np.ascontiguousarray(vals, dtype=np.float64)
+    if L.lmesh_matrix_write_matlab(str(filename).encode(), m, n, _ptr(rp), _ptr(ci), _ptr(v)) != 0:
+        raise LmhError(L.lmesh_last_error().decode())
 
 
 def batch_recipe(seed, count, nmin=8, nmax=48):

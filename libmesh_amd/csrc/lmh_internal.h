@@ -114,7 +114,7 @@ struct lmh_dofmap {
 struct lmh_matrix {
   lmh_context *ctx = nullptr;
   const lmh_dofmap *dm = nullptr;
-  int64_t rows = 0, nnz = 0;
+  int64_t rows = 0, cols = 0, nnz = 0;  // cols == rows except for rectangular lmh_matrix_create_csr2
   int max_row = 0;
   int32_t *rp = nullptr, *ci = nullptr;
   int32_t *rb = nullptr;  // SpMV row-block boundaries (nrb+1 rows)

@@ -835,6 +835,7 @@ int lmh_cg_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int
                  double *rel_res, int *reason) {
   LMH_CHECK(A && b_d && x_d, LMH_ERR_INVALID, "null argument");
   LMH_CHECK(rtol >= 0 && maxits >= 0, LMH_ERR_INVALID, "bad tolerance or iteration limit");
+  LMH_CHECK(A->rows == A->cols, LMH_ERR_INVALID, "CG needs a square matrix");
   LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
   lmh_context *ctx = A->ctx;
   LMH_TRY(ensure_work(A, false));
@@ -901,6 +902,7 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
   LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
   LMH_CHECK(A->have32 && A->vals32, LMH_ERR_INVALID, "mixed solve needs the FP32 operator (assemble FP32 or make_fp32)");
   LMH_CHECK(mode == 0 || mode == 1, LMH_ERR_INVALID, "mode must be 0 (reliable update) or 1 (restarted refinement)");
+  LMH_CHECK(A->rows == A->cols, LMH_ERR_INVALID, "the solver needs a square matrix");
   LMH_CHECK(rtol > 0 && inner_rtol > 0 && inner_rtol < 1, LMH_ERR_INVALID, "bad tolerances");
   lmh_context *ctx = A->ctx;
   LMH_TRY(ensure_work(A, true));

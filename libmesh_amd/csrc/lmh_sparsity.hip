@@ -169,6 +169,7 @@ int matrix_create_impl(lmh_context *ctx, const lmh_dofmap *dm, bool colour, bool
   A->ctx = ctx;
   A->dm = dm;
   A->rows = n;
+  A->cols = n;
 
   // max candidates per row
   std::vector<int32_t> ip(n + 1);
@@ -244,9 +245,9 @@ int lmh_matrix_create(lmh_context *ctx, const lmh_dofmap *dm, lmh_matrix **out) 
   return matrix_create_impl(ctx, dm, true, true, out);
 }
 
-int lmh_matrix_create_csr(lmh_context *ctx, int64_t rows, const int32_t *row_ptr_h, const int32_t *col_idx_h,
-                          lmh_matrix **out) {
-  LMH_CHECK(ctx && row_ptr_h && out && rows > 0, LMH_ERR_INVALID, "bad argument");
+int lmh_matrix_create_csr2(lmh_context *ctx, int64_t rows, int64_t cols, const int32_t *row_ptr_h,
+                           const int32_t *col_idx_h, lmh_matrix **out) {
+  LMH_CHECK(ctx && row_ptr_h && out && rows > 0 && cols > 0 && cols <= INT32_MAX, LMH_ERR_INVALID, "bad argument");
   LMH_CHECK(row_ptr_h[0] == 0, LMH_ERR_INVALID, "row_ptr[0] must be 0");
   const int64_t nnz = row_ptr_h[rows];
   LMH_CHECK(nnz >= 0 && (nnz == 0 || col_idx_h), LMH_ERR_INVALID, "bad nnz");
@@ -256,7 +257,7 @@ int lmh_matrix_create_csr(lmh_context *ctx, int64_t rows, const int32_t *row_ptr
     LMH_CHECK(rp[r + 1] >= rp[r], LMH_ERR_INVALID, "row_ptr not monotone at row %lld", (long long)r);
     max_row = std::max<int>(max_row, rp[r + 1] - rp[r]);
     for (int32_t k = rp[r]; k < rp[r + 1]; ++k) {
-      LMH_CHECK(col_idx_h[k] >= 0 && col_idx_h[k] < rows, LMH_ERR_INVALID, "column index out of range");
+      LMH_CHECK(col_idx_h[k] >= 0 && col_idx_h[k] < cols, LMH_ERR_INVALID, "column index out of range");
       LMH_CHECK(k == rp[r] || col_idx_h[k] > col_idx_h[k - 1], LMH_ERR_INVALID,
                 "columns of row %lld not strictly increasing", (long long)r);
     }
@@ -265,6 +266,7 @@ int lmh_matrix_create_csr(lmh_context *ctx, int64_t rows, const int32_t *row_ptr
   std::unique_ptr<lmh_matrix> A(new lmh_matrix);
   A->ctx = ctx;
   A->rows = rows;
+  A->cols = cols;
   A->nnz = nnz;
   A->max_row = max_row;
   LMH_HIP(hipMalloc((void **)&A->rp, sizeof(int32_t) * (rows + 1)));
@@ -277,6 +279,11 @@ int lmh_matrix_create_csr(lmh_context *ctx, int64_t rows, const int32_t *row_ptr
   LMH_TRY(build_row_blocks(A.get(), rp));
   *out = A.release();
   return LMH_OK;
+}
+
+int lmh_matrix_create_csr(lmh_context *ctx, int64_t rows, const int32_t *row_ptr_h, const int32_t *col_idx_h,
+                          lmh_matrix **out) {
+  return lmh_matrix_create_csr2(ctx, rows, rows, row_ptr_h, col_idx_h, out);
 }
 
 int lmh_matrix_destroy(lmh_matrix *A) {
@@ -302,6 +309,7 @@ int lmh_matrix_destroy(lmh_matrix *A) {
 }
 
 int64_t lmh_matrix_rows(const lmh_matrix *A) { return A ? A->rows : -1; }
+int64_t lmh_matrix_cols(const lmh_matrix *A) { return A ? A->cols : -1; }
 int64_t lmh_matrix_nnz(const lmh_matrix *A) { return A ? A->nnz : -1; }
 int lmh_matrix_n_colors(const lmh_matrix *A) { return A ? A->ncolors : -1; }
 

@@ -1,11 +1,15 @@
 // sparse_matrix_contract.C -- the SparseMatrix contract checks of the
 // reference's tests/numerics/sparse_matrix_test.h (setValues/testValues with
 // (i+1)(j+1), testTransposeNorms l1(A) = linf(A^T), testClone, get_diagonal,
-// add(a, X)) run on HipCsrMatrix, plus add_matrix on a DofMap-built pattern.
+// add(a, X)) run on HipCsrMatrix, plus add_matrix on a DofMap-built pattern,
+// and, given the directory of the reference's matrix fixtures as argv[1],
+// testReadMatlab (the .m and .m.gz files give the same norms), the PETSc
+// binary reads and a print_matlab/read round trip on rectangular matrices.
 // Prints one line per check and exits non-zero on any failure.
 #include <cmath>
 #include <cstdio>
 #include <numeric>
+#include <string>
 
 #include "libmesh/hip_csr_matrix.h"
 
@@ -17,7 +21,81 @@ static void check(bool ok, const char *what) {
   if (!ok) ++failures;
 }
 
-int main() {
+static bool same_entries(const SparseMatrix<Number> &A, const SparseMatrix<Number> &B, bool skip_zeros,
+                         double rtol = 0.0) {
+  if (A.m() != B.m() || A.n() != B.n()) return false;
+  for (numeric_index_type i = 0; i < A.m(); ++i) {
+    std::vector<numeric_index_type> ia, ib;
+    std::vector<Number> va, vb;
+    A.get_row(i, ia, va);
+    B.get_row(i, ib, vb);
+    std::vector<std::pair<numeric_index_type, Number>> a, b;
+    for (size_t k = 0; k < ia.size(); ++k)
+      if (!skip_zeros || va[k] != 0.0) a.emplace_back(ia[k], va[k]);
+    for (size_t k = 0; k < ib.size(); ++k)
+      if (!skip_zeros || vb[k] != 0.0) b.emplace_back(ib[k], vb[k]);
+    if (a.size() != b.size()) return false;
+    for (size_t k = 0; k < a.size(); ++k)
+      if (a[k].first != b[k].first || std::abs(a[k].second - b[k].second) > rtol * std::abs(a[k].second))
+        return false;
+  }
+  return true;
+}
+
+static void matrix_files(const std::string &dir) {
+  for (const char *g : {"1", "2", "4", "8"}) {
+    const std::string base = dir + "/geom_" + g + "_extraction_op.m";
+    HipCsrMatrix<Number> A, B;
+    A.read(base);
+    char what[160];
+    if (std::string(g) != "8") {
+      B.read(base + ".gz");
+      std::snprintf(what, sizeof(what), "testReadMatlab geom_%s: .m and .m.gz norms equal (%zu x %zu)", g,
+                    (size_t)A.m(), (size_t)A.n());
+      check(A.l1_norm() == B.l1_norm() && A.linfty_norm() == B.linfty_norm() && same_entries(A, B, false), what);
+    }
+    // print_matlab then read: the same nonzeros, exactly
+    const std::string out = std::string("roundtrip_") + g + ".m";
+    A.print_matlab(out);
+    HipCsrMatrix<Number> C;
+    C.read_matlab(out);
+    std::snprintf(what, sizeof(what), "print_matlab/read_matlab round trip geom_%s", g);
+    check(same_entries(A, C, true), what);
+    // rectangular SpMV on the device: y = A 1 equals the row sums
+    HipVector<Number> x(A.n()), y(A.m());
+    for (numeric_index_type j = 0; j < A.n(); ++j) x.set(j, 1.0);
+    x.close();
+    A.vector_mult(y, x);
+    bool ok = true;
+    for (numeric_index_type i = 0; i < A.m(); ++i) {
+      std::vector<numeric_index_type> idx;
+      std::vector<Number> val;
+      A.get_row(i, idx, val);
+      double s = 0;
+      for (Number v : val) s += v;
+      ok &= std::abs(y(i) - s) <= 1e-14 * std::max(1.0, std::abs(s));
+    }
+    std::snprintf(what, sizeof(what), "vector_mult on the %zu x %zu matrix geom_%s", (size_t)A.m(), (size_t)A.n(), g);
+    check(ok, what);
+  }
+  HipCsrMatrix<Number> M, P32, P64;
+  M.read(dir + "/geom_1_extraction_op.m");
+  P32.read(dir + "/geom_1_extraction_op.petsc32");
+  P64.read(dir + "/geom_1_extraction_op.petsc64");
+  // the reference's binary and text files differ in the last bit of some entries
+  check(same_entries(M, P32, true, 2.3e-16) && same_entries(M, P64, true, 2.3e-16) && same_entries(P32, P64, true),
+        "PETSc binary (32/64-bit) equals the Matlab file");
+  bool threw = false;
+  try {
+    HipCsrMatrix<Number> H;
+    H.read(dir + "/geom_1_extraction_op.h5");
+  } catch (const LogicError &) {
+    threw = true;
+  }
+  check(threw, "unsupported extension throws LogicError");
+}
+
+int main(int argc, char **argv) {
   try {
     // one rank: global_m = global_n = 1 * 5.75 - 1 = 4, locally dense block
     const numeric_index_type m = 4;
@@ -105,6 +183,8 @@ int main() {
       threw = true;
     }
     check(threw, "set outside the pattern throws LogicError");
+
+    if (argc > 1) matrix_files(argv[1]);
   } catch (const std::exception &e) {
     std::printf("FAIL exception: %s\n", e.what());
     return 1;

@@ -35,11 +35,11 @@ class HipCsrMatrix : public SparseMatrix<T> {
   std::unique_ptr<SparseMatrix<T>> clone() const override;
   void close() override;
   numeric_index_type m() const override { return _rows; }
-  numeric_index_type n() const override { return _rows; }
+  numeric_index_type n() const override { return _cols; }
   numeric_index_type row_start() const override { return 0; }
   numeric_index_type row_stop() const override { return _rows; }
   numeric_index_type col_start() const override { return 0; }
-  numeric_index_type col_stop() const override { return _rows; }
+  numeric_index_type col_stop() const override { return _cols; }
   void set(numeric_index_type i, numeric_index_type j, T value) override;
   void add(numeric_index_type i, numeric_index_type j, T value) override;
   void add_matrix(const DenseMatrix<T> &dm, const std::vector<numeric_index_type> &rows,
@@ -56,6 +56,11 @@ class HipCsrMatrix : public SparseMatrix<T> {
   void get_row(numeric_index_type i, std::vector<numeric_index_type> &indices, std::vector<T> &values) const override;
   void vector_mult(NumericVector<T> &dest, const NumericVector<T> &arg) const override;
   std::size_t n_nonzeros() const override { return (std::size_t)_nnz; }
+  // Matlab text (17 significant digits), Matlab text or PETSc binary in
+  void print_matlab(const std::string &name = "") const override;
+  void read(const std::string &filename) override;
+  void read_matlab(const std::string &filename) override;
+  void read_petsc_binary(const std::string &filename) override;
 
   // device handles for the GPU assembly and solver
   lmh_matrix *device_matrix() const { return _A; }
@@ -81,7 +86,7 @@ class HipCsrMatrix : public SparseMatrix<T> {
   lmh_mesh *_mesh = nullptr;
   lmh_dofmap *_dm = nullptr;
   lmh_matrix *_A = nullptr;
-  numeric_index_type _rows = 0;
+  numeric_index_type _rows = 0, _cols = 0;
   int64_t _nnz = 0;
   std::vector<int32_t> _rp, _ci;        // host copy of the pattern
   mutable std::vector<T> _vals;         // host mirror of the values

@@ -3,6 +3,8 @@
 // calls, and the defaults the drop-in keeps.
 #pragma once
 
+#include <string>
+
 #include <iostream>
 #include <memory>
 #include <vector>
@@ -61,6 +63,13 @@ class SparseMatrix {
   // SparseMatrix::vector_mult (sparse_matrix.C:237-253)
   virtual void vector_mult(NumericVector<T> &dest, const NumericVector<T> &arg) const = 0;
   virtual std::size_t n_nonzeros() const = 0;
+
+  // On-disk formats (sparse_matrix.C: print_matlab :398, read :686, read_matlab
+  // :1050, read_petsc_binary :1332)
+  virtual void print_matlab(const std::string &name = "") const { libmesh_not_implemented(); }
+  virtual void read(const std::string &filename) { libmesh_not_implemented(); }
+  virtual void read_matlab(const std::string &filename) { libmesh_not_implemented(); }
+  virtual void read_petsc_binary(const std::string &filename) { libmesh_not_implemented(); }
 
  protected:
   const DofMap *_dof_map = nullptr;

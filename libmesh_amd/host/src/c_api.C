@@ -4,7 +4,10 @@
 #include <string>
 
 #include "../../../include/lmesh_host.h"
+#include <fstream>
+
 #include "libmesh/dof_map.h"
+#include "libmesh/matrix_io.h"
 #include "libmesh/replicated_mesh.h"
 
 using namespace libMesh;
@@ -60,6 +63,43 @@ void lmesh_batch_recipe(uint64_t seed, int count, int nmin, int nmax, int *edge,
   for (int p = 0; p < count; ++p) {
     edge[p] = e[p];
     elem_type[p] = (int)t[p];
+  }
+}
+
+int lmesh_matrix_read(const char *filename, int64_t *m, int64_t *n, int64_t *nnz, int64_t *row, int64_t *col,
+                      double *val) {
+  try {
+    const MatrixIO::Entries E = MatrixIO::read(filename);
+    *m = (int64_t)E.m;
+    *n = (int64_t)E.n;
+    if (row && col && val) {
+      if (*nnz < (int64_t)E.ijv.size()) libmesh_error_msg("lmesh_matrix_read: output arrays too small");
+      for (std::size_t k = 0; k < E.ijv.size(); ++k) {
+        row[k] = (int64_t)std::get<0>(E.ijv[k]);
+        col[k] = (int64_t)std::get<1>(E.ijv[k]);
+        val[k] = std::get<2>(E.ijv[k]);
+      }
+    }
+    *nnz = (int64_t)E.ijv.size();
+    return 0;
+  } catch (const std::exception &e) {
+    g_lmesh_err = e.what();
+    return -1;
+  }
+}
+
+int lmesh_matrix_write_matlab(const char *filename, int64_t m, int64_t n, const int32_t *row_ptr,
+                              const int32_t *col_idx, const double *vals) {
+  try {
+    std::vector<int32_t> rp(row_ptr, row_ptr + m + 1), ci(col_idx, col_idx + row_ptr[m]);
+    std::vector<double> v(vals, vals + row_ptr[m]);
+    std::ofstream os(filename);
+    if (!os.good()) libmesh_error_msg("cannot write " << filename);
+    MatrixIO::write_matlab(os, (std::size_t)m, (std::size_t)n, rp, ci, v);
+    return 0;
+  } catch (const std::exception &e) {
+    g_lmesh_err = e.what();
+    return -1;
   }
 }
 

@@ -18,6 +18,10 @@
 
 #include "libmesh/hip_csr_matrix.h"
 
+#include <fstream>
+
+#include "libmesh/matrix_io.h"
+
 namespace libMesh {
 
 
@@ -38,6 +42,7 @@ void HipCsrMatrix<T>::clear() {
   _dm = nullptr;
   _mesh = nullptr;
   _rows = 0;
+  _cols = 0;
   _nnz = 0;
   _rp.clear();
   _ci.clear();
@@ -66,6 +71,7 @@ void HipCsrMatrix<T>::init(ParallelType) {
             "HipCsrMatrix::init (dof map)");
   hip_check(lmh_matrix_create(_ctx, _dm, &_A), "HipCsrMatrix::init (sparsity)");
   _rows = dm->n_dofs();
+  _cols = _rows;
   _nnz = lmh_matrix_nnz(_A);
   _rp.resize(_rows + 1);
   _ci.resize(_nnz);
@@ -79,9 +85,9 @@ void HipCsrMatrix<T>::init(ParallelType) {
 template <typename T>
 void HipCsrMatrix<T>::init(numeric_index_type m, numeric_index_type n, numeric_index_type, numeric_index_type,
                            numeric_index_type, numeric_index_type, numeric_index_type) {
-  if (m != n) libmesh_not_implemented();  // square operators only on this path
   clear();
   _rows = m;
+  _cols = n;
   this->_is_initialized = true;
   _closed = false;
 }
@@ -145,7 +151,7 @@ void HipCsrMatrix<T>::flush_entries() {
   }
   for (numeric_index_type r = 0; r < _rows; ++r) _rp[r + 1] += _rp[r];
   _nnz = (int64_t)_ci.size();
-  hip_check(lmh_matrix_create_csr(_ctx, _rows, _rp.data(), _ci.data(), &_A), "HipCsrMatrix::close (pattern)");
+  hip_check(lmh_matrix_create_csr2(_ctx, _rows, _cols, _rp.data(), _ci.data(), &_A), "HipCsrMatrix::close (pattern)");
   _vals_host_valid = true;
   _vals_host_dirty = true;
   _open.clear();
@@ -265,10 +271,10 @@ T HipCsrMatrix<T>::operator()(numeric_index_type i, numeric_index_type j) const 
 template <typename T>
 Real HipCsrMatrix<T>::l1_norm() const {
   pull_values();
-  std::vector<Real> col(_rows, 0.0);
+  std::vector<Real> col(_cols, 0.0);
   for (numeric_index_type r = 0; r < _rows; ++r)
     for (int32_t k = _rp[r]; k < _rp[r + 1]; ++k) col[_ci[k]] += std::abs(_vals[k]);
-  return _rows ? *std::max_element(col.begin(), col.end()) : 0.0;
+  return _cols ? *std::max_element(col.begin(), col.end()) : 0.0;
 }
 
 template <typename T>
@@ -308,7 +314,7 @@ void HipCsrMatrix<T>::get_transpose(SparseMatrix<T> &dest) const {
   if (!D) libmesh_error_msg("get_transpose: destination must be a HipCsrMatrix");
   pull_values();
   D->clear();
-  D->init(_rows, _rows, _rows, _rows);
+  D->init(_cols, _rows, _cols, _rows);
   for (numeric_index_type r = 0; r < _rows; ++r)
     for (int32_t k = _rp[r]; k < _rp[r + 1]; ++k) D->set((numeric_index_type)_ci[k], r, _vals[k]);
   D->close();
@@ -332,6 +338,7 @@ void HipCsrMatrix<T>::vector_mult(NumericVector<T> &dest, const NumericVector<T>
   const auto *a = dynamic_cast<const HipVector<T> *>(&arg);
   if (!d || !a) libmesh_error_msg("vector_mult: HipVector arguments required");
   if (!_closed) libmesh_error_msg("vector_mult before close()");
+  if (a->size() != _cols) libmesh_error_msg("vector_mult: argument has " << a->size() << " entries, need " << _cols);
   if (d->size() != _rows) d->init(_rows);
   hip_check(lmh_spmv(_A, (const double *)a->device_data(), (double *)d->device_data()), "vector_mult");
   d->mark_device_modified();
@@ -344,7 +351,7 @@ std::unique_ptr<SparseMatrix<T>> HipCsrMatrix<T>::zero_clone() const {
     c->attach_dof_map(*this->_dof_map);
     c->init();
   } else {
-    c->init(_rows, _rows, _rows, _rows);
+    c->init(_rows, _cols, _rows, _cols);
     for (numeric_index_type r = 0; r < _rows; ++r)
       for (int32_t k = _rp[r]; k < _rp[r + 1]; ++k) c->set(r, (numeric_index_type)_ci[k], T(0));
     c->close();
@@ -362,6 +369,46 @@ std::unique_ptr<SparseMatrix<T>> HipCsrMatrix<T>::clone() const {
   h->_vals_host_dirty = true;
   h->close();
   return c;
+}
+
+template <typename T>
+void HipCsrMatrix<T>::print_matlab(const std::string &name) const {
+  if (!_closed) libmesh_error_msg("HipCsrMatrix::print_matlab before close()");
+  pull_values();
+  std::vector<double> v(_vals.begin(), _vals.end());
+  if (name.empty()) {
+    MatrixIO::write_matlab(std::cout, _rows, _cols, _rp, _ci, v);
+    return;
+  }
+  std::ofstream os(name);
+  if (!os.good()) libmesh_error_msg("HipCsrMatrix::print_matlab: cannot write " << name);
+  MatrixIO::write_matlab(os, _rows, _cols, _rp, _ci, v);
+}
+
+// A read matrix takes the pattern of the file's entries (explicit zeros kept),
+// as the reference's read_matlab/read_petsc_binary init + set do.
+template <typename T>
+static void load_entries(HipCsrMatrix<T> &A, const MatrixIO::Entries &E) {
+  A.clear();
+  A.init(E.m, E.n, E.m, E.n);
+  for (const auto &e : E.ijv) A.set(std::get<0>(e), std::get<1>(e), (T)std::get<2>(e));
+  A.close();
+}
+
+template <typename T>
+void HipCsrMatrix<T>::read(const std::string &filename) {
+  load_entries(*this, MatrixIO::read(filename));
+}
+
+template <typename T>
+void HipCsrMatrix<T>::read_matlab(const std::string &filename) {
+  load_entries(*this, MatrixIO::read_matlab(filename));
+}
+
+template <typename T>
+void HipCsrMatrix<T>::read_petsc_binary(const std::string &filename) {
+  const bool wide = filename.size() >= 8 && filename.compare(filename.size() - 8, 8, ".petsc64") == 0;
+  load_entries(*this, MatrixIO::read_petsc_binary(filename, wide ? 8 : 4));
 }
 
 template class HipCsrMatrix<double>;

@@ -22,8 +22,9 @@ def run(args, tmp_path, expect=0):
 
 
 def test_sparse_matrix_contract(tmp_path):
-    out = run(["sparse_matrix_contract"], tmp_path)
+    out = run(["sparse_matrix_contract", os.path.join(ROOT, "tests", "golden", "matrices")], tmp_path)
     assert "FAIL" not in out and "0 failures" in out, out
+    assert "testReadMatlab geom_4" in out and "PETSc binary" in out, out
 
 
 def test_poisson_c0_dropin(tmp_path):
