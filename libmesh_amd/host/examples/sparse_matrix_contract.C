@@ -24,6 +24,13 @@ static void check(bool ok, const char *what) {
 static bool same_entries(const SparseMatrix<Number> &A, const SparseMatrix<Number> &B, bool skip_zeros,
                          double rtol = 0.0) {
   if (A.m() != B.m() || A.n() != B.n()) return false;
+  double amax = 0;
+  for (numeric_index_type i = 0; i < A.m(); ++i) {
+    std::vector<numeric_index_type> ia;
+    std::vector<Number> va;
+    A.get_row(i, ia, va);
+    for (Number v : va) amax = std::max(amax, std::abs(v));
+  }
   for (numeric_index_type i = 0; i < A.m(); ++i) {
     std::vector<numeric_index_type> ia, ib;
     std::vector<Number> va, vb;
@@ -35,9 +42,8 @@ static bool same_entries(const SparseMatrix<Number> &A, const SparseMatrix<Numbe
     for (size_t k = 0; k < ib.size(); ++k)
       if (!skip_zeros || vb[k] != 0.0) b.emplace_back(ib[k], vb[k]);
     if (a.size() != b.size()) return false;
-    for (size_t k = 0; k < a.size(); ++k)
-      if (a[k].first != b[k].first || std::abs(a[k].second - b[k].second) > rtol * std::abs(a[k].second))
-        return false;
+    for (size_t k = 0; k < a.size(); ++k)  // normwise: relative to the largest entry
+      if (a[k].first != b[k].first || std::abs(a[k].second - b[k].second) > rtol * amax) return false;
   }
   return true;
 }
