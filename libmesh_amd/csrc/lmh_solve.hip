@@ -685,6 +685,16 @@ __global__ __launch_bounds__(BLK) void k_refresh_fin(const double *__restrict__ 
   }
 }
 
+// x += xc (FP64), xc = 0
+__global__ __launch_bounds__(BLK) void k_fold(int64_t n, float *__restrict__ xc, double *__restrict__ x) {
+  int64_t lo, hi;
+  block_range(n, lo, hi);
+  for (int64_t i = lo + threadIdx.x; i < hi; i += BLK) {
+    x[i] += (double)xc[i];
+    xc[i] = 0.f;
+  }
+}
+
 // restarted refinement (mode 1): the correction solve starts over, p = dinv r
 __global__ __launch_bounds__(BLK) void k_restart_p(int64_t n, const float *__restrict__ dinv32,
                                                    const float *__restrict__ r32, float *__restrict__ p32,
@@ -755,7 +765,7 @@ static bool use_off16(const lmh_matrix *A) { return A->fp32_index && A->idx16_st
 
 static int ensure_work(lmh_matrix *A, bool mixed) {
   if (!A->w64) LMH_HIP(hipMalloc((void **)&A->w64, sizeof(double) * 4 * A->rows));
-  if (mixed && !A->w32) LMH_HIP(hipMalloc((void **)&A->w32, sizeof(float) * 4 * A->rows));
+  if (mixed && !A->w32) LMH_HIP(hipMalloc((void **)&A->w32, sizeof(float) * 5 * A->rows));
   return LMH_OK;
 }
 
@@ -909,7 +919,9 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
   LMH_TRY(ensure_off16(A));
   const bool c16 = use_off16(A);
   const int64_t n = A->rows;
-  float *r32 = A->w32, *p32 = r32 + n, *q32 = p32 + n, *d32 = q32 + n;
+  // xc: FP32 solution correction accumulated since the last refresh, folded
+  // into the FP64 x before each refresh
+  float *r32 = A->w32, *p32 = r32 + n, *q32 = p32 + n, *d32 = q32 + n, *xc = d32 + n;
   double *S = ctx->d_scal;
   double *part1 = ctx->d_part, *part2 = ctx->d_part + 2 * ctx->n_part;
   const int g = vec_grid(ctx);
@@ -921,11 +933,13 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
   LMH_TRY(ensure_diag(A));
   LMH_LAUNCH(ctx, "invdiag", (k_invdiag<float, float>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, n, A->diag,
              A->vals32, d32);
+  LMH_HIP(hipMemsetAsync(xc, 0, sizeof(float) * n, ctx->stream));
   LMH_LAUNCH(ctx, "mx_refresh_fp64", k_refresh, dim3(g), dim3(BLK), 0, 1, A->rb, A->nrb, A->rp, A->ci, A->vals, b_d,
              x_d, d32, r32, p32, part2, S, 0);
   LMH_LAUNCH(ctx, "fin", k_refresh_fin, dim3(1), dim3(BLK), 0, part2, g, 1, S, 0);
   LMH_TRY(check_launch("mixed setup"));
   auto refresh = [&](int bank) {
+    LMH_LAUNCH(ctx, "mx_fold", k_fold, dim3(g), dim3(BLK), 0, n, xc, x_d);
     LMH_LAUNCH(ctx, "mx_refresh_fp64", k_refresh, dim3(g), dim3(BLK), 0, 0, A->rb, A->nrb, A->rp, A->ci, A->vals, b_d,
                x_d, d32, r32, p32, part2, S, bank);
     LMH_LAUNCH(ctx, "fin", k_refresh_fin, dim3(1), dim3(BLK), 0, part2, g, 0, S, bank);
@@ -947,8 +961,8 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
       else
         LMH_LAUNCH(ctx, "mx_spmv_fp32", k_it_spmv<float>, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci,
                    A->vals32, p32, q32, part1, S, bank);
-      LMH_LAUNCH(ctx, "mx_update_fp32", (k_it_update<float, double>), dim3(g), dim3(BLK), 0, n, part1, g, d32, p32, q32,
-                 x_d, r32, part2, S, bank);
+      LMH_LAUNCH(ctx, "mx_update_fp32", (k_it_update<float, float>), dim3(g), dim3(BLK), 0, n, part1, g, d32, p32, q32,
+                 xc, r32, part2, S, bank);
       LMH_LAUNCH(ctx, "mx_pupdate_fp32", k_it_pupdate<float>, dim3(g), dim3(BLK), 0, n, part2, g, d32, r32, p32, S,
                  bank, 1);
     }
@@ -970,6 +984,7 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
     pending = true;
     slot ^= 1;
   }
+  LMH_LAUNCH(ctx, "mx_fold", k_fold, dim3(g), dim3(BLK), 0, n, xc, x_d);
   LMH_HIP(hipStreamSynchronize(ctx->stream));
   LMH_TRY(check_launch("mixed iterations"));
   for (auto &pl : poll)
