@@ -249,7 +249,7 @@ struct MfmaT<float> {
 constexpr int Q2N = 27, Q2Q = 27, Q2K = 81, Q2KS = 21;
 
 template <typename CT>
-__global__ __launch_bounds__(256) void k_poisson_hex27_mfma(const double *__restrict__ tab,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_poisson_hex27_mfma(const double *__restrict__ tab,
                                                             const double *__restrict__ xyz,
                                                             const int32_t *__restrict__ conn,
                                                             const int32_t *__restrict__ edofs,
@@ -270,24 +270,6 @@ __global__ __launch_bounds__(256) void k_poisson_hex27_mfma(const double *__rest
   for (int i = tid; i < Q2N * Q2Q; i += 256) s_phi[i] = tab[Q2Q + i];
   for (int i = tid; i < Q2N * Q2Q * 3; i += 256) s_dref[i] = tab[Q2Q + Q2N * Q2Q + i];
   __syncthreads();
-
-  // constant A operand: A[I][ks] = Gref^T[16I + l%16][4ks + l/16]
-  CT areg[2][Q2KS];
-  {
-    const int li = lane & 15, lk = lane >> 4;
-#pragma unroll
-    for (int I = 0; I < 2; ++I)
-#pragma unroll
-      for (int ks = 0; ks < Q2KS; ++ks) {
-        const int i = 16 * I + li, k = 4 * ks + lk;
-        double v = 0.0;
-        if (i < Q2N && k < Q2K) {
-          const int q = k / 3, r = k - 3 * (k / 3);
-          v = s_dref[(i * Q2Q + q) * 3 + r];
-        }
-        areg[I][ks] = (CT)v;
-      }
-  }
 
   double *X = s_X[wv];
   double *M = s_M[wv];
@@ -335,7 +317,7 @@ __global__ __launch_bounds__(256) void k_poisson_hex27_mfma(const double *__rest
       acc_t c00 = {0, 0, 0, 0}, c01 = {0, 0, 0, 0}, c11 = {0, 0, 0, 0};
       const int li = lane & 15, lk = lane >> 4;
       const int j0 = li, j1 = 16 + li;
-#pragma unroll
+#pragma unroll 1
       for (int ks = 0; ks < Q2KS; ++ks) {
         const int k = 4 * ks + lk;
         double b0 = 0.0, b1 = 0.0;
@@ -349,9 +331,17 @@ __global__ __launch_bounds__(256) void k_poisson_hex27_mfma(const double *__rest
             b1 = m0 * d1[0] + m1 * d1[1] + m2 * d1[2];
           }
         }
-        c00 = MfmaT<CT>::mma(areg[0][ks], (CT)b0, c00);
-        c01 = MfmaT<CT>::mma(areg[0][ks], (CT)b1, c01);
-        c11 = MfmaT<CT>::mma(areg[1][ks], (CT)b1, c11);
+        // constant A operand A[I][ks] = Gref^T[16I + l%16][4ks + l/16], read from the
+        // LDS reference-derivative table
+        CT a0 = 0, a1 = 0;
+        if (k < Q2K) {
+          const int q = k / 3, r = k - 3 * (k / 3);
+          a0 = (CT)s_dref[(j0 * Q2Q + q) * 3 + r];
+          if (j1 < Q2N) a1 = (CT)s_dref[(j1 * Q2Q + q) * 3 + r];
+        }
+        c00 = MfmaT<CT>::mma(a0, (CT)b0, c00);
+        c01 = MfmaT<CT>::mma(a0, (CT)b1, c01);
+        c11 = MfmaT<CT>::mma(a1, (CT)b1, c11);
       }
       // stage in LDS (row stride 28) with the mirrored (1,0) tile
 #pragma unroll

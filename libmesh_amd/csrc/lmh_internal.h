@@ -178,6 +178,7 @@ int matrix_create_impl(lmh_context *ctx, const lmh_dofmap *dm, bool colour, bool
 // G[r][c] = d xi_r / d x_c.  Returns det J; G is only meaningful when det > 0.
 __device__ inline double qp_map(const double *X, const double *dref, int n, int nq, int q, double G[3][3]) {
   double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // J[r][c] = d x_c / d xi_r
+#pragma unroll 4
   for (int a = 0; a < n; ++a) {
     const double *d = &dref[(a * nq + q) * 3];
     const double x = X[a * 3], y = X[a * 3 + 1], z = X[a * 3 + 2];
