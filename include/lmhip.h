@@ -90,6 +90,42 @@ int lmh_mesh_set_coords(lmh_mesh *m, const double *coords_h);
  * constrained_h: n_dofs bytes, nonzero = constrained (may be NULL). ---- */
 int lmh_dofmap_create(lmh_context *ctx, const lmh_mesh *m, int n_vars, int64_t n_dofs, const int32_t *elem_dofs_h,
                       const uint8_t *constrained_h, lmh_dofmap **out);
+
+/* ---- general constraint rows (DofMap::add_constraint_row,
+ * dof_map_constraints.C:2180-2208, then DofMap::process_constraints,
+ * :4341-4497).  Replaces the constraint set by the rows given at creation
+ * (empty rows, value 0) plus these n_rows rows, added in order: DoF dof_h[k]
+ * = rhs_h[k] + sum of coefs_h[t] * x[cols_h[t]] over t in
+ * [row_ptr_h[k], row_ptr_h[k+1]) (rhs_h NULL: 0).  Rows are then substituted
+ * until they refer to unconstrained DoFs only.  Matrices of this dofmap must be
+ * created after the call.  Assembly applies the rows with
+ * DofMap::heterogeneously_constrain_element_matrix_and_vector
+ * (dof_map_constraints.C:2518-2616) and the pattern couples each element's DoFs
+ * with the DoFs they are constrained by (find_connected_dofs, dof_map.C:2915).
+ * flags: LMH_CONSTRAINT_FORBID_OVERWRITE (error if a DoF is constrained
+ * already, the forbid_constraint_overwrite argument), LMH_CONSTRAINT_ERROR_ON_LOOP
+ * (DofMap::set_error_on_constraint_loop: check_for_constraint_loops,
+ * :4509-4597), LMH_CONSTRAINT_SYMMETRIC_ROWS (asymmetric_constraint_rows =
+ * false: a constrained row keeps only its diagonal and a zero rhs; use with
+ * CG and lmh_enforce_constraints).  A loop that survives substitution is
+ * always an error here.  Elements with general rows: at most 32 DoFs of their
+ * own and 64 once expanded. */
+enum {
+  LMH_CONSTRAINT_FORBID_OVERWRITE = 1,
+  LMH_CONSTRAINT_ERROR_ON_LOOP = 2,
+  LMH_CONSTRAINT_SYMMETRIC_ROWS = 4
+};
+int lmh_dofmap_set_constraints(lmh_dofmap *dm, int64_t n_rows, const int32_t *dof_h, const int32_t *row_ptr_h,
+                               const int32_t *cols_h, const double *coefs_h, const double *rhs_h, int flags);
+/* Processed rows: counts (any pointer may be NULL), then the rows ascending by
+ * DoF with entries ascending (cols_h, coefs_h, rhs_h may be NULL). */
+int lmh_dofmap_constraint_counts(const lmh_dofmap *dm, int64_t *n_rows, int64_t *n_entries, int64_t *n_general_elems);
+int lmh_dofmap_get_constraints(const lmh_dofmap *dm, int32_t *dof_h, int32_t *row_ptr_h, int32_t *cols_h,
+                               double *coefs_h, double *rhs_h);
+/* DofMap::enforce_constraints_exactly (dof_map_constraints.C:3088-3170):
+ * x_d = value_d + sum_j c_dj x_j on every constrained DoF (value omitted when
+ * homogeneous != 0). */
+int lmh_enforce_constraints(const lmh_dofmap *dm, double *x_d, int homogeneous);
 int lmh_dofmap_destroy(lmh_dofmap *dm);
 
 /* ---- sparsity + matrix: DofMap::compute_sparsity / SparsityPattern::Build
@@ -143,7 +179,10 @@ int lmh_vector_add_element_vectors(lmh_context *ctx, double *v_d, int64_t n_elem
  * 3 pi^2 sin(pi x) sin(pi y) sin(pi z).  precision LMH_FP64 builds the FP64
  * CSR; LMH_FP32 runs the element contraction in FP32 on MFMA and writes the
  * FP32 operator used by the mixed-precision inner solver (geometry stays FP64).
- * scatter: LMH_SCATTER_COLORED or LMH_SCATTER_GATHER. */
+ * scatter: LMH_SCATTER_COLORED or LMH_SCATTER_GATHER.  rhs_d may be NULL
+ * (matrix only).  The rhs of an FP32 assembly equals the FP64 one except for
+ * the K H terms of nonzero constraint values, formed from the FP32 element
+ * matrices there. */
 int lmh_assemble_poisson(lmh_matrix *A, const lmh_mesh *m, const lmh_dofmap *dm, int npts, const double *kappa_d,
                          int f_kind, double *rhs_d, int precision, int scatter);
 /* Element matrices/vectors only (no scatter), constrained if constrain != 0:

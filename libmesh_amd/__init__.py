@@ -24,6 +24,7 @@ LIB_HOST = os.path.join(HERE, "liblmesh.so")
 HEX8, HEX27, TET4, TET10 = 10, 12, 8, 9
 FP64, FP32 = 0, 1
 SCATTER_COLORED, SCATTER_GATHER = 0, 1
+CONSTRAINT_FORBID_OVERWRITE, CONSTRAINT_ERROR_ON_LOOP, CONSTRAINT_SYMMETRIC_ROWS = 1, 2, 4
 ALL_FACES = 0x3F
 CONVERGED_RTOL, DIVERGED_ITS = 2, -3
 
@@ -59,6 +60,10 @@ HIP_API = {
     "lmh_mesh_set_coords": (_I, [_P, _P]),
     "lmh_dofmap_create": (_I, [_P, _P, _I, _I64, _P, _P, C.POINTER(_P)]),
     "lmh_dofmap_destroy": (_I, [_P]),
+    "lmh_dofmap_set_constraints": (_I, [_P, _I64, _P, _P, _P, _P, _P, _I]),
+    "lmh_dofmap_constraint_counts": (_I, [_P, _P, _P, _P]),
+    "lmh_dofmap_get_constraints": (_I, [_P, _P, _P, _P, _P, _P]),
+    "lmh_enforce_constraints": (_I, [_P, _P, _I]),
     "lmh_matrix_create": (_I, [_P, _P, C.POINTER(_P)]),
     "lmh_matrix_create_csr": (_I, [_P, _I64, _P, _P, C.POINTER(_P)]),
     "lmh_matrix_destroy": (_I, [_P]),
@@ -349,6 +354,57 @@ class GpuProblem:
             self.nnz = L.lmh_matrix_nnz(self.matrix)
             self.n_colors = L.lmh_matrix_n_colors(self.matrix)
 
+    def set_constraints(self, rows, forbid_overwrite=False, error_on_loop=False, symmetric_rows=False):
+        """DofMap::add_constraint_row for each (dof, {dof: coefficient}, value)
+        of `rows` (in order) on top of the Dirichlet rows, then
+        process_constraints; the matrix is created again for the new pattern."""
+        rows = list(rows)
+        dofs = np.array([r[0] for r in rows], dtype=np.int32)
+        ptr = np.zeros(len(rows) + 1, dtype=np.int32)
+        cols, coefs = [], []
+        for k, (_, row, _) in enumerate(rows):
+            cols.extend(row.keys())
+            coefs.extend(row.values())
+            ptr[k + 1] = len(cols)
+        cols = np.array(cols if cols else [0], dtype=np.int32)
+        coefs = np.array(coefs if coefs else [0.0], dtype=np.float64)
+        rhs = np.array([r[2] for r in rows] if rows else [0.0], dtype=np.float64)
+        flags = ((CONSTRAINT_FORBID_OVERWRITE if forbid_overwrite else 0) |
+                 (CONSTRAINT_ERROR_ON_LOOP if error_on_loop else 0) |
+                 (CONSTRAINT_SYMMETRIC_ROWS if symmetric_rows else 0))
+        L = hip()
+        check(L.lmh_dofmap_set_constraints(self.dofmap, len(rows), _ptr(dofs) if rows else None, _ptr(ptr),
+                                           _ptr(cols), _ptr(coefs), _ptr(rhs), flags))
+        if self.matrix:
+            L.lmh_matrix_destroy(self.matrix)
+            self.matrix = _P()
+            check(L.lmh_matrix_create(self.ctx.h, self.dofmap, C.byref(self.matrix)))
+            self.nnz = L.lmh_matrix_nnz(self.matrix)
+            self.n_colors = L.lmh_matrix_n_colors(self.matrix)
+
+    def constraint_rows(self):
+        """{dof: ({dof: coefficient}, value)} after processing."""
+        nr, ne, ng = _I64(), _I64(), _I64()
+        L = hip()
+        check(L.lmh_dofmap_constraint_counts(self.dofmap, C.byref(nr), C.byref(ne), C.byref(ng)))
+        d = np.empty(max(nr.value, 1), dtype=np.int32)
+        ptr = np.empty(nr.value + 1, dtype=np.int32)
+        cols = np.empty(max(ne.value, 1), dtype=np.int32)
+        coefs = np.empty(max(ne.value, 1))
+        rhs = np.empty(max(nr.value, 1))
+        check(L.lmh_dofmap_get_constraints(self.dofmap, _ptr(d), _ptr(ptr), _ptr(cols), _ptr(coefs), _ptr(rhs)))
+        return {int(d[k]): ({int(cols[t]): float(coefs[t]) for t in range(ptr[k], ptr[k + 1])}, float(rhs[k]))
+                for k in range(nr.value)}
+
+    def n_general_elements(self):
+        ng = _I64()
+        check(hip().lmh_dofmap_constraint_counts(self.dofmap, None, None, C.byref(ng)))
+        return ng.value
+
+    def enforce_constraints(self, x_d, homogeneous=False):
+        """DofMap::enforce_constraints_exactly on a device vector."""
+        check(hip().lmh_enforce_constraints(self.dofmap, x_d.ptr, int(homogeneous)))
+
     @classmethod
     def from_problem(cls, ctx, p, build_matrix=True):
         return cls(ctx, p.coords, p.conn, p.elem_type, p.elem_dofs, p.n_dofs, p.constrained, p.n_vars, build_matrix)
@@ -375,8 +431,8 @@ class GpuProblem:
 
     def assemble_poisson(self, npts, kappa_d, f_kind, rhs_d, precision=FP64, scatter=SCATTER_GATHER):
         check(hip().lmh_assemble_poisson(self.matrix, self.mesh, self.dofmap, npts,
-                                         kappa_d.ptr if kappa_d is not None else None, f_kind, rhs_d.ptr, precision,
-                                         scatter))
+                                         kappa_d.ptr if kappa_d is not None else None, f_kind,
+                                         rhs_d.ptr if rhs_d is not None else None, precision, scatter))
 
     def element_poisson(self, npts, kappa_d, f_kind, Ke_d, Fe_d, constrain=True):
         check(hip().lmh_element_poisson(self.ctx.h, self.mesh, self.dofmap, npts,

@@ -670,6 +670,8 @@ extern "C" {
 int lmh_element_poisson(lmh_context *ctx, const lmh_mesh *m, const lmh_dofmap *dm, int npts, const double *kappa_d,
                         int f_kind, int constrain, double *Ke_d, double *Fe_d) {
   LMH_CHECK(ctx && m && dm && Ke_d && Fe_d, LMH_ERR_INVALID, "null argument");
+  LMH_CHECK(!constrain || dm->cr.n_general == 0, LMH_ERR_UNSUPPORTED,
+            "element matrices with general constraint rows grow past the element's DoFs: assemble the matrix");
   LMH_HIP(hipMemsetAsync(ctx->d_flags, 0, sizeof(int), ctx->stream));
   LMH_TRY(element_poisson_impl<double>(ctx, m, dm, npts, kappa_d, f_kind, constrain, Ke_d, Fe_d));
   return read_flag(ctx, "lmh_element_poisson");
@@ -677,9 +679,11 @@ int lmh_element_poisson(lmh_context *ctx, const lmh_mesh *m, const lmh_dofmap *d
 
 int lmh_assemble_poisson(lmh_matrix *A, const lmh_mesh *m, const lmh_dofmap *dm, int npts, const double *kappa_d,
                          int f_kind, double *rhs_d, int precision, int scatter) {
-  LMH_CHECK(A && m && dm && rhs_d, LMH_ERR_INVALID, "null argument");
+  LMH_CHECK(A && m && dm, LMH_ERR_INVALID, "null argument");
   LMH_CHECK(A->dm == dm && dm->mesh == m, LMH_ERR_INVALID, "matrix, dofmap and mesh do not belong together");
   LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
+  LMH_CHECK(A->cons_version == dm->cr.version, LMH_ERR_INVALID,
+            "the constraints changed after the matrix was created: create the matrix again");
   lmh_context *ctx = A->ctx;
   const int64_t ne = m->n_elem, dd = (int64_t)dm->dpe * dm->dpe;
   if (!A->Fe) LMH_HIP(hipMalloc((void **)&A->Fe, sizeof(double) * ne * dm->dpe));
@@ -687,14 +691,18 @@ int lmh_assemble_poisson(lmh_matrix *A, const lmh_mesh *m, const lmh_dofmap *dm,
   if (precision == LMH_FP64) {
     if (!A->Ke) LMH_HIP(hipMalloc((void **)&A->Ke, sizeof(double) * ne * dd));
     LMH_TRY(element_poisson_impl<double>(ctx, m, dm, npts, kappa_d, f_kind, 1, A->Ke, A->Fe));
+    LMH_TRY(constrain_general_impl<double>(A, A->Ke, A->Fe));
     LMH_TRY(scatter_impl<double>(A, A->Ke, A->Fe, A->vals, rhs_d, scatter));
+    LMH_TRY(scatter_extra_impl<double>(A, A->vals, rhs_d));
     A->have32 = false;
   } else {
     LMH_CHECK(precision == LMH_FP32, LMH_ERR_INVALID, "unknown precision %d", precision);
     if (!A->Ke32) LMH_HIP(hipMalloc((void **)&A->Ke32, sizeof(float) * ne * dd));
     if (!A->vals32) LMH_HIP(hipMalloc((void **)&A->vals32, sizeof(float) * (A->nnz + LMH_PAD)));
     LMH_TRY(element_poisson_impl<float>(ctx, m, dm, npts, kappa_d, f_kind, 1, A->Ke32, A->Fe));
+    LMH_TRY(constrain_general_impl<float>(A, A->Ke32, A->Fe));
     LMH_TRY(scatter_impl<float>(A, A->Ke32, A->Fe, A->vals32, rhs_d, scatter));
+    LMH_TRY(scatter_extra_impl<float>(A, A->vals32, rhs_d));
     A->have32 = true;
   }
   return read_flag(ctx, "lmh_assemble_poisson");
@@ -765,6 +773,7 @@ int lmh_vector_add_element_vectors(lmh_context *ctx, double *v_d, int64_t n_elem
 int lmh_qoi_box_derivative(const lmh_mesh *m, const lmh_dofmap *dm, int npts, double lo, double hi, double *dq_d) {
   LMH_CHECK(m && dm && dq_d, LMH_ERR_INVALID, "null argument");
   LMH_CHECK(dm->nv == 1, LMH_ERR_UNSUPPORTED, "QoI derivative needs one variable");
+  LMH_CHECK(dm->cr.n_general == 0, LMH_ERR_UNSUPPORTED, "QoI derivative with general constraint rows");
   lmh_context *ctx = m->ctx;
   const FETable *t = nullptr;
   LMH_TRY(fe_table(ctx, m->type, npts, &t));
@@ -790,6 +799,7 @@ int lmh_adjoint_kappa_gradient(const lmh_mesh *m, const lmh_dofmap *dm, int npts
   (void)kappa_d;  // dJ/dkappa_e does not depend on kappa_e for a per-element constant kappa
   LMH_CHECK(m && dm && u_d && lambda_d && grad_d, LMH_ERR_INVALID, "null argument");
   LMH_CHECK(dm->nv == 1, LMH_ERR_UNSUPPORTED, "adjoint gradient needs one variable");
+  LMH_CHECK(dm->cr.n_general == 0, LMH_ERR_UNSUPPORTED, "adjoint gradient with general constraint rows");
   lmh_context *ctx = m->ctx;
   const FETable *t = nullptr;
   LMH_TRY(fe_table(ctx, m->type, npts, &t));

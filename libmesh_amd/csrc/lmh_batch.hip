@@ -400,6 +400,7 @@ static int batch_setup(lmh_batch *b, lmh_context *ctx, int n_prob, const lmh_dof
     const lmh_dofmap *dm = dms[p];
     LMH_CHECK(dm && dm->ctx == ctx, LMH_ERR_INVALID, "problem %d: null DoF map or other context", p);
     LMH_CHECK(dm->nv == 3, LMH_ERR_INVALID, "problem %d: elasticity needs 3 variables per node", p);
+    LMH_CHECK(dm->cr.n_general == 0, LMH_ERR_UNSUPPORTED, "problem %d: general constraint rows", p);
     const int s = type_slot(dm->mesh->type);
     LMH_CHECK(s >= 0, LMH_ERR_UNSUPPORTED, "problem %d: element type %d", p, dm->mesh->type);
     LMH_CHECK(dm->mesh->n_elem * (int64_t)dm->dpe < INT32_MAX, LMH_ERR_UNSUPPORTED, "problem %d too large", p);

@@ -536,7 +536,12 @@ int lmh_dofmap_create(lmh_context *ctx, const lmh_mesh *m, int n_vars, int64_t n
     LMH_HIP(hipMemcpy(d->cons, constrained_h, n_dofs, hipMemcpyHostToDevice));
   else
     LMH_HIP(hipMemset(d->cons, 0, n_dofs));
+  d->h_cons0.assign(n_dofs, 0);
+  if (constrained_h)
+    for (int64_t t = 0; t < n_dofs; ++t) d->h_cons0[t] = constrained_h[t] != 0;
   LMH_TRY(build_incidence(ctx, d->edofs, m->n_elem, dpe, n_dofs, &d->inc_ptr, &d->inc));
+  // the constrained set as empty constraint rows
+  LMH_TRY(lmh_dofmap_set_constraints(d.get(), 0, nullptr, nullptr, nullptr, nullptr, nullptr, 0));
   LMH_HIP(hipStreamSynchronize(ctx->stream));
   *out = d.release();
   return LMH_OK;
@@ -549,6 +554,7 @@ int lmh_dofmap_destroy(lmh_dofmap *dm) {
   (void)hipFree(dm->cons);
   (void)hipFree(dm->inc_ptr);
   (void)hipFree(dm->inc);
+  (void)release_constraints(dm->cr);
   delete dm;
   return LMH_OK;
 }

@@ -99,16 +99,52 @@ struct lmh_mesh {
   int32_t *conn = nullptr;   // n_elem*npe
 };
 
+// General constraint rows after processing (DofMap::_dof_constraints with
+// _primal_constraint_values).  "Empty" rows (no entries, value 0: homogeneous
+// Dirichlet) are applied by the element kernels' mask; every element holding a
+// DoF with a non-empty row or a nonzero value is a "general element" whose
+// matrix goes through the dense constraint product C^T K C on its expanded DoF
+// list (its own DoFs, then the DoFs its rows refer to).
+struct lmh_constraints {
+  int64_t version = 0;          // bumped by every lmh_dofmap_set_constraints
+  bool asymmetric = true;       // asymmetric_constraint_rows
+  int64_t n_general = 0;        // rows with entries or a nonzero value
+  // all rows, ascending DoF, entries ascending (host mirror for queries)
+  std::vector<int32_t> dof, ptr, col;
+  std::vector<double> coef, val;
+  // the same rows on the device (enforce_constraints_exactly)
+  int32_t *d_dof = nullptr, *d_ptr = nullptr, *d_col = nullptr;
+  double *d_coef = nullptr, *d_val = nullptr;
+  // general elements
+  int64_t n_ge = 0;
+  int mx = 0;                   // expanded DoF count bound (<= 64)
+  int32_t *ge_elem = nullptr;   // n_ge element ids, ascending
+  int32_t *ge_m = nullptr;      // n_ge expanded sizes
+  int32_t *ge_dofs = nullptr;   // n_ge*mx expanded DoF lists, -1 padded
+  uint8_t *ge_flag = nullptr;   // n_ge*dpe: 1 = the element's DoF i is constrained
+  double *ge_val = nullptr;     // n_ge*dpe: its constraint value (H)
+  int32_t *ge_cptr = nullptr;   // n_ge*(mx+1): C by expanded column, absolute offsets
+  int32_t *ge_ci = nullptr;     // local row i of each C entry
+  double *ge_cv = nullptr;      // C entry
+  // DoF -> sorted list of (g*mx + a): expanded position a of general element g
+  std::vector<int32_t> h_xip;   // n_dofs+1 (host: pattern candidate bound)
+  int32_t *xip = nullptr, *xinc = nullptr;
+  int32_t *xrows = nullptr;     // rows with at least one such entry
+  int64_t n_xrows = 0;
+};
+
 struct lmh_dofmap {
   lmh_context *ctx = nullptr;
   const lmh_mesh *mesh = nullptr;
   int nv = 1, dpe = 0;
   int64_t n_dofs = 0;
   int32_t *edofs = nullptr;   // n_elem*dpe
-  uint8_t *cons = nullptr;    // n_dofs
+  uint8_t *cons = nullptr;    // n_dofs: 1 = constrained by an empty row (masked in the element kernels)
+  std::vector<uint8_t> h_cons0;  // the constrained set given at creation
   // incidence: dof -> sorted list of (e*dpe + i)
   int32_t *inc_ptr = nullptr;  // n_dofs+1
   int32_t *inc = nullptr;      // n_elem*dpe
+  lmh_constraints cr;
 };
 
 struct lmh_matrix {
@@ -136,6 +172,12 @@ struct lmh_matrix {
   // element buffers
   double *Ke = nullptr, *Fe = nullptr;
   float *Ke32 = nullptr;
+  // general-element matrices (constraint version they were built for, slot
+  // offsets of expanded entries, C^T K C, C^T (F - K H))
+  int64_t cons_version = 0;
+  uint16_t *offx = nullptr;   // n_ge*mx*mx
+  double *Kx = nullptr, *Fx = nullptr;
+  float *Kx32 = nullptr;
   // solver work vectors (allocated on first solve)
   double *w64 = nullptr;  // 4*rows
   float *w32 = nullptr;   // 5*rows
@@ -170,6 +212,14 @@ int fe_table(lmh_context *ctx, int type, int arg, const FETable **out);
 // CSR pattern, scatter map, (if colour) element colouring and (if fp64) zeroed
 // FP64 values for a DoF map
 int matrix_create_impl(lmh_context *ctx, const lmh_dofmap *dm, bool colour, bool fp64, lmh_matrix **out);
+// general constraints around the scatter: before it, the general elements'
+// matrices are replaced by their constrained, expanded forms (and zeroed in
+// Ke/Fe); after it, those are added into their rows
+template <typename T>
+int constrain_general_impl(lmh_matrix *A, T *Ke, double *Fe);
+template <typename T>
+int scatter_extra_impl(lmh_matrix *A, T *vals, double *rhs);
+int release_constraints(lmh_constraints &c);
 
 // ---- element map at one quadrature point ----------------------------------
 // From LDS coordinates X[a*3+c] and the reference derivative table

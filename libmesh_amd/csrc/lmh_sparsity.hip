@@ -15,24 +15,38 @@
 
 namespace lmh {
 
-// one wave (64 lanes) per row, rows grid-strided; cap = power of two >= max candidates
+// one wave (64 lanes) per row, rows grid-strided; cap = power of two >= max
+// candidates.  With general constraints (xip non-null) a row also takes the
+// expanded DoF lists (xd, mx per general element, -1 padded) of the general
+// elements it appears in (find_connected_dofs, dof_map.C:2915-2972).
 template <bool FILL>
 __global__ __launch_bounds__(64) void k_row_union(const int32_t *__restrict__ inc_ptr, const int32_t *__restrict__ inc,
                                                   const int32_t *__restrict__ edofs, int dpe, int64_t n_rows, int cap,
+                                                  const int32_t *__restrict__ xip, const int32_t *__restrict__ xinc,
+                                                  const int32_t *__restrict__ xd, int mx,
                                                   int32_t *__restrict__ rowlen, const int32_t *__restrict__ rp,
                                                   int32_t *__restrict__ ci) {
   extern __shared__ int32_t keys[];
   const int lane = threadIdx.x;
   for (int64_t r = blockIdx.x; r < n_rows; r += gridDim.x) {
     const int32_t b = inc_ptr[r], e = inc_ptr[r + 1];
-    const int ncand = (e - b) * dpe;
-    // gather candidates: incidence k covers element inc[k]/dpe
+    const int32_t xb = xip ? xip[r] : 0, xe = xip ? xip[r + 1] : 0;
+    const int nbase = (e - b) * dpe;
+    const int ncand = nbase + (xe - xb) * mx;
+    // gather candidates: incidence k covers element inc[k]/dpe; expanded
+    // incidence k covers general element xinc[k]/mx
     for (int t = lane; t < cap; t += 64) {
       int32_t v = INT_MAX;
-      if (t < ncand) {
+      if (t < nbase) {
         const int k = t / dpe, j = t - k * dpe;
         const int32_t el = inc[b + k] / dpe;
         v = edofs[(int64_t)el * dpe + j];
+      } else if (t < ncand) {
+        const int tt = t - nbase;
+        const int k = tt / mx, j = tt - k * mx;
+        const int32_t g = xinc[xb + k] / mx;
+        v = xd[(int64_t)g * mx + j];
+        if (v < 0) v = INT_MAX;
       }
       keys[t] = v;
     }
@@ -96,6 +110,36 @@ __global__ void k_scatter_offsets(const int32_t *__restrict__ edofs, int64_t n_e
     }
     if (lo >= rp[r + 1] || ci[lo] != c) atomicOr(err, 1);
     off[t] = (uint16_t)(lo - base);
+  }
+}
+
+// slot offset of every (g, a, b) of the general elements' expanded matrices
+// within row X_g[a]; 0 in the padding
+__global__ void k_xoffsets(const int32_t *__restrict__ xd, const int32_t *__restrict__ gm, int64_t n_ge, int mx,
+                           const int32_t *__restrict__ rp, const int32_t *__restrict__ ci, uint16_t *__restrict__ off,
+                           int *__restrict__ err) {
+  const int64_t mm = (int64_t)mx * mx, total = n_ge * mm;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = t / mm;
+    const int rem = (int)(t - g * mm);
+    const int a = rem / mx, bb = rem - a * mx;
+    const int m = gm[g];
+    uint16_t o = 0;
+    if (a < m && bb < m) {
+      const int32_t r = xd[g * mx + a], c = xd[g * mx + bb];
+      int32_t lo = rp[r], hi = rp[r + 1];
+      const int32_t base = lo;
+      while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (ci[mid] < c)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      if (lo >= rp[r + 1] || ci[lo] != c) atomicOr(err, 1);
+      o = (uint16_t)(lo - base);
+    }
+    off[t] = o;
   }
 }
 
@@ -174,19 +218,25 @@ int matrix_create_impl(lmh_context *ctx, const lmh_dofmap *dm, bool colour, bool
   // max candidates per row
   std::vector<int32_t> ip(n + 1);
   LMH_HIP(hipMemcpy(ip.data(), dm->inc_ptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost));
-  int64_t maxinc = 0;
-  for (int64_t d = 0; d < n; ++d) maxinc = std::max<int64_t>(maxinc, ip[d + 1] - ip[d]);
+  const lmh_constraints &cr = dm->cr;
+  const bool gen = cr.n_ge > 0;
+  A->cons_version = cr.version;
+  int64_t maxcand = 0;
+  for (int64_t d = 0; d < n; ++d)
+    maxcand = std::max<int64_t>(maxcand, (int64_t)(ip[d + 1] - ip[d]) * dpe +
+                                             (gen ? (int64_t)(cr.h_xip[d + 1] - cr.h_xip[d]) * cr.mx : 0));
   int cap = 64;
-  while (cap < maxinc * dpe) cap <<= 1;
-  LMH_CHECK(cap <= 16384, LMH_ERR_UNSUPPORTED, "row candidate count %lld too large", (long long)(maxinc * dpe));
+  while (cap < maxcand) cap <<= 1;
+  LMH_CHECK(cap <= 16384, LMH_ERR_UNSUPPORTED, "row candidate count %lld too large", (long long)maxcand);
   const size_t shm = sizeof(int32_t) * cap;
+  const int32_t *xip = gen ? cr.xip : nullptr;
 
   int32_t *rowlen = nullptr;
   LMH_HIP(hipMalloc((void **)&rowlen, sizeof(int32_t) * (n + 1)));
   LMH_HIP(hipMemsetAsync(rowlen, 0, sizeof(int32_t) * (n + 1), ctx->stream));
   const int g = (int)std::min<int64_t>(n, 65536);
   LMH_LAUNCH(ctx, "sparsity_count", k_row_union<false>, dim3(g), dim3(64), shm, dm->inc_ptr, dm->inc, dm->edofs, dpe,
-             n, cap, rowlen, (const int32_t *)nullptr, (int32_t *)nullptr);
+             n, cap, xip, cr.xinc, cr.ge_dofs, cr.mx, rowlen, (const int32_t *)nullptr, (int32_t *)nullptr);
   LMH_TRY(check_launch("k_row_union<count>"));
   LMH_HIP(hipMalloc((void **)&A->rp, sizeof(int32_t) * (n + 1)));
   int64_t nnz = 0;
@@ -195,7 +245,7 @@ int matrix_create_impl(lmh_context *ctx, const lmh_dofmap *dm, bool colour, bool
   LMH_HIP(hipMalloc((void **)&A->ci, sizeof(int32_t) * (nnz + LMH_PAD)));
   LMH_HIP(hipMemsetAsync(A->ci + nnz, 0, sizeof(int32_t) * LMH_PAD, ctx->stream));
   LMH_LAUNCH(ctx, "sparsity_fill", k_row_union<true>, dim3(g), dim3(64), shm, dm->inc_ptr, dm->inc, dm->edofs, dpe,
-             n, cap, (int32_t *)nullptr, (const int32_t *)A->rp, A->ci);
+             n, cap, xip, cr.xinc, cr.ge_dofs, cr.mx, (int32_t *)nullptr, (const int32_t *)A->rp, A->ci);
   LMH_TRY(check_launch("k_row_union<fill>"));
 
   // row length bound for the 16-bit scatter offsets
@@ -216,6 +266,16 @@ int matrix_create_impl(lmh_context *ctx, const lmh_dofmap *dm, bool colour, bool
   LMH_HIP(hipMemcpyAsync(&bad, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   LMH_HIP(hipStreamSynchronize(ctx->stream));
   LMH_CHECK(!bad, LMH_ERR_INVALID, "internal: element DoF pair missing from the sparsity pattern");
+  if (gen) {
+    const int64_t tot = cr.n_ge * cr.mx * cr.mx;
+    LMH_HIP(hipMalloc((void **)&A->offx, sizeof(uint16_t) * tot));
+    LMH_LAUNCH(ctx, "scatter_offsets_expanded", k_xoffsets, dim3(grid_for(tot, 256, 16384)), dim3(256), 0,
+               cr.ge_dofs, cr.ge_m, cr.n_ge, cr.mx, A->rp, A->ci, A->offx, ctx->d_flags);
+    LMH_TRY(check_launch("k_xoffsets"));
+    LMH_HIP(hipMemcpyAsync(&bad, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    LMH_HIP(hipStreamSynchronize(ctx->stream));
+    LMH_CHECK(!bad, LMH_ERR_INVALID, "internal: expanded DoF pair missing from the sparsity pattern");
+  }
 
   if (fp64) {
     LMH_HIP(hipMalloc((void **)&A->vals, sizeof(double) * (nnz + LMH_PAD)));
@@ -302,6 +362,10 @@ int lmh_matrix_destroy(lmh_matrix *A) {
   (void)hipFree(A->Ke);
   (void)hipFree(A->Fe);
   (void)hipFree(A->Ke32);
+  (void)hipFree(A->offx);
+  (void)hipFree(A->Kx);
+  (void)hipFree(A->Fx);
+  (void)hipFree(A->Kx32);
   (void)hipFree(A->w64);
   (void)hipFree(A->w32);
   delete A;

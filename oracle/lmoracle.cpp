@@ -13,6 +13,7 @@
 #include <cstring>
 #include <map>
 #include <random>
+#include <set>
 #include <utility>
 #include <vector>
 
@@ -547,6 +548,8 @@ double forcing(int f_kind, const double *x) {
 // ---------------------------------------------------------------------------
 // System: DoF map, constraint set, sparsity
 // ---------------------------------------------------------------------------
+using ConstraintRow = std::map<int32_t, double>;  // DofConstraintRow
+
 struct System {
   const Mesh *m = nullptr;
   int nv = 1, dpe = 0;
@@ -554,6 +557,14 @@ struct System {
   std::vector<int32_t> edofs;   // n_elem * dpe
   std::vector<uint8_t> cons;    // per dof
   std::vector<int32_t> rp, ci;  // CSR pattern
+  // DofMap::_dof_constraints and _primal_constraint_values; Dirichlet DoFs hold
+  // empty rows.  `general` is set once a row with entries or a nonzero value
+  // exists: assembly then follows the dense constraint-matrix path.
+  std::map<int32_t, ConstraintRow> crows;
+  std::map<int32_t, double> cvals;
+  bool general = false;
+  bool asymmetric = true;  // asymmetric_constraint_rows (dof_map.h default true)
+  bool is_constrained(int32_t d) const { return crows.count(d) != 0; }
 };
 
 // distribute_local_dofs_var_major (src/base/dof_map.C:1417-1530): nodes get a
@@ -601,6 +612,10 @@ void build_constraints(System &s, unsigned mask) {
         for (int v = 0; v < s.nv; ++v) s.cons[s.edofs[(size_t)e * s.dpe + v * m.npe + a]] = 1;
       }
     }
+  s.crows.clear();
+  s.cvals.clear();
+  for (i64 d = 0; d < s.ndofs; ++d)
+    if (s.cons[d]) s.crows[(int32_t)d];  // empty constraint row, value 0
 }
 
 // SparsityPattern::Build (src/base/sparsity_pattern.C:96-365): each row is the
@@ -665,6 +680,265 @@ void constrain(const System &s, i64 e, double *K, double *F) {
       if (F) F[i] = 0.;
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// General constraint rows
+// ---------------------------------------------------------------------------
+// DofMap::add_constraint_row (src/base/dof_map_constraints.C:2180-2208): store
+// the row and its value (insert_or_assign), refusing a constrained DoF when
+// forbid_constraint_overwrite is set.
+int add_constraint_row(System &s, int32_t dof, const ConstraintRow &row, double rhs, bool forbid) {
+  if (forbid && s.is_constrained(dof)) return -1;
+  s.crows[dof] = row;
+  s.cvals[dof] = rhs;
+  return 0;
+}
+
+// DofMap::check_for_constraint_loops (dof_map_constraints.C:4509-4597): the
+// expansion of process_constraints on a copy of the rows, failing when a row
+// being substituted refers to the DoF it constrains.
+bool constraint_loop(const System &s) {
+  std::map<int32_t, ConstraintRow> rows = s.crows;
+  std::set<int32_t> todo;
+  for (const auto &r : rows) todo.insert(r.first);
+  while (!todo.empty())
+    for (auto i = todo.begin(); i != todo.end();) {
+      ConstraintRow &row = rows[*i];
+      std::vector<int32_t> expand;
+      for (const auto &it : row)
+        if (it.first != *i && s.is_constrained(it.first)) {
+          todo.insert(it.first);
+          expand.push_back(it.first);
+        }
+      for (int32_t x : expand) {
+        const double c = row[x];
+        for (const auto &it : rows.at(x)) {
+          if (it.first == x) return true;  // "Constraint loop detected"
+          row[it.first] += it.second * c;
+        }
+        row.erase(x);
+      }
+      if (expand.empty())
+        i = todo.erase(i);
+      else
+        ++i;
+    }
+  return false;
+}
+
+// DofMap::process_constraints (dof_map_constraints.C:4341-4497): substitute
+// constrained DoFs inside every row (and their values into the row's value)
+// until rows refer only to unconstrained DoFs; values that become zero are
+// dropped from the value map.
+void process_constraints(System &s) {
+  std::set<int32_t> todo;
+  for (const auto &r : s.crows) todo.insert(r.first);
+  while (!todo.empty())
+    for (auto i = todo.begin(); i != todo.end();) {
+      ConstraintRow &row = s.crows[*i];
+      auto vit = s.cvals.find(*i);
+      double rhs = vit == s.cvals.end() ? 0.0 : vit->second;
+      std::vector<int32_t> expand;
+      for (const auto &it : row)
+        if (it.first != *i && s.is_constrained(it.first)) {
+          todo.insert(it.first);
+          expand.push_back(it.first);
+        }
+      for (int32_t x : expand) {
+        const double c = row[x];
+        for (const auto &it : s.crows.at(x)) row[it.first] += it.second * c;
+        auto sub = s.cvals.find(x);
+        if (sub != s.cvals.end()) rhs += sub->second * c;
+        row.erase(x);
+      }
+      if (rhs != 0.0)
+        s.cvals[*i] = rhs;
+      else
+        s.cvals.erase(*i);
+      if (expand.empty())
+        i = todo.erase(i);
+      else
+        ++i;
+    }
+  s.general = false;
+  for (const auto &r : s.crows) s.general |= !r.second.empty();
+  for (const auto &v : s.cvals) s.general |= v.second != 0.0;
+  s.cons.assign(s.ndofs, 0);
+  for (const auto &r : s.crows) s.cons[r.first] = 1;
+}
+
+// DofMap::find_connected_dofs (src/base/dof_map.C:2915-2972): add the DoFs
+// that constrained DoFs depend on, recursively; the result is sorted.
+void find_connected_dofs(const System &s, std::vector<int32_t> &dofs) {
+  std::set<int32_t> set(dofs.begin(), dofs.end());
+  bool done = true;
+  for (int32_t d : dofs) {
+    auto pos = s.crows.find(d);
+    if (pos == s.crows.end()) continue;
+    for (const auto &it : pos->second)
+      if (!set.count(it.first)) {
+        set.insert(it.first);
+        done = false;
+      }
+  }
+  if (!done) {
+    dofs.assign(set.begin(), set.end());
+    find_connected_dofs(s, dofs);
+  }
+}
+
+// SparsityPattern::Build with constraints (sparsity_pattern.C:96-147, 150-260):
+// each element's DoFs grow by find_connected_dofs, are sorted and made unique,
+// and every one of them couples to all of them.
+void build_sparsity_general(System &s) {
+  std::vector<std::vector<int32_t>> rows(s.ndofs);
+  for (i64 e = 0; e < s.m->n_elem; ++e) {
+    std::vector<int32_t> d(&s.edofs[(size_t)e * s.dpe], &s.edofs[(size_t)e * s.dpe] + s.dpe);
+    find_connected_dofs(s, d);
+    std::sort(d.begin(), d.end());
+    d.erase(std::unique(d.begin(), d.end()), d.end());
+    for (int32_t r : d) rows[r].insert(rows[r].end(), d.begin(), d.end());
+  }
+  s.rp.assign(s.ndofs + 1, 0);
+  for (i64 r = 0; r < s.ndofs; ++r) {
+    std::sort(rows[r].begin(), rows[r].end());
+    rows[r].erase(std::unique(rows[r].begin(), rows[r].end()), rows[r].end());
+    s.rp[r + 1] = s.rp[r] + (int32_t)rows[r].size();
+  }
+  s.ci.resize(s.rp[s.ndofs]);
+  for (i64 r = 0; r < s.ndofs; ++r) std::copy(rows[r].begin(), rows[r].end(), s.ci.begin() + s.rp[r]);
+}
+
+// Dense row-major matrix for the constraint algebra (DenseMatrix<Number>).
+struct Dense {
+  int m = 0, n = 0;
+  std::vector<double> a;
+  void resize(int mm, int nn) {
+    m = mm;
+    n = nn;
+    a.assign((size_t)m * n, 0.0);
+  }
+  double &operator()(int i, int j) { return a[(size_t)i * n + j]; }
+  double operator()(int i, int j) const { return a[(size_t)i * n + j]; }
+};
+
+// A <- A * B (DenseMatrix::right_multiply)
+void right_multiply(Dense &A, const Dense &B) {
+  Dense R;
+  R.resize(A.m, B.n);
+  for (int i = 0; i < A.m; ++i)
+    for (int j = 0; j < B.n; ++j) {
+      double t = 0.0;
+      for (int k = 0; k < A.n; ++k) t += A(i, k) * B(k, j);
+      R(i, j) = t;
+    }
+  A = R;
+}
+
+// A <- B^T * A (DenseMatrix::left_multiply_transpose)
+void left_multiply_transpose(Dense &A, const Dense &B) {
+  Dense R;
+  R.resize(B.n, A.n);
+  for (int i = 0; i < B.n; ++i)
+    for (int j = 0; j < A.n; ++j) {
+      double t = 0.0;
+      for (int k = 0; k < B.m; ++k) t += B(k, i) * A(k, j);
+      R(i, j) = t;
+    }
+  A = R;
+}
+
+// DofMap::build_constraint_matrix_and_vector (dof_map_constraints.C:3545-3660):
+// x_elem = C x_expanded + H.  The DoFs that constrained DoFs depend on and
+// that are not already present are appended (sorted); C(i,i) = 1 for a free
+// DoF, C(i, pos(j)) = c_ij and H(i) = value for a constrained one; recurses
+// while appended DoFs are constrained themselves.
+void build_constraint_matrix_and_vector(const System &s, Dense &C, std::vector<double> &H,
+                                        std::vector<int32_t> &dofs, bool recursive) {
+  std::set<int32_t> add;
+  bool any = false;
+  for (int32_t d : dofs) {
+    auto pos = s.crows.find(d);
+    if (pos == s.crows.end()) continue;
+    any = true;
+    for (const auto &it : pos->second) add.insert(it.first);
+  }
+  if (!any) return;
+  for (int32_t d : dofs) add.erase(d);
+  if (add.empty() && recursive) return;
+  const int old = (int)dofs.size();
+  dofs.insert(dofs.end(), add.begin(), add.end());
+  C.resize(old, (int)dofs.size());
+  H.assign(old, 0.0);
+  for (int i = 0; i < old; ++i) {
+    auto pos = s.crows.find(dofs[i]);
+    if (pos == s.crows.end()) {
+      C(i, i) = 1.0;
+      continue;
+    }
+    for (const auto &it : pos->second)
+      for (int j = 0; j < (int)dofs.size(); ++j)
+        if (dofs[j] == it.first) C(i, j) = it.second;
+    auto v = s.cvals.find(dofs[i]);
+    if (v != s.cvals.end()) H[i] = v->second;
+  }
+  Dense Cn;
+  std::vector<double> Hn;
+  build_constraint_matrix_and_vector(s, Cn, Hn, dofs, true);
+  if (C.n == Cn.m && Cn.n == (int)dofs.size()) {
+    for (int i = 0; i < C.m; ++i)  // H += C Hn
+      for (int k = 0; k < C.n; ++k) H[i] += C(i, k) * Hn[k];
+    right_multiply(C, Cn);
+  }
+}
+
+// DofMap::heterogeneously_constrain_element_matrix_and_vector
+// (dof_map_constraints.C:2518-2616): F <- C^T (F - K H), K <- C^T K C over the
+// expanded DoFs, then each constrained row i is zeroed with K(i,i) = 1 and, with
+// asymmetric_constraint_rows, K(i, pos(j)) = -c_ij and F(i) = its value
+// (without: F(i) = 0).  K (n x n row-major) and F grow to the expanded size;
+// dofs receives the expanded DoF list.
+void constrain_general(const System &s, std::vector<int32_t> &dofs, std::vector<double> &K, std::vector<double> &F) {
+  const int n = (int)dofs.size();
+  Dense C;
+  std::vector<double> H;
+  build_constraint_matrix_and_vector(s, C, H, dofs, false);
+  if (!(C.m == n && C.n == (int)dofs.size())) return;
+  Dense Km;
+  Km.resize(n, n);
+  Km.a = K;
+  std::vector<double> G(n);
+  for (int i = 0; i < n; ++i) {
+    double kh = 0.0;
+    for (int j = 0; j < n; ++j) kh += Km(i, j) * H[j];
+    G[i] = F[i] - kh;
+  }
+  const int m = C.n;
+  F.assign(m, 0.0);
+  for (int j = 0; j < m; ++j) {  // DenseMatrix::vector_mult_transpose
+    double t = 0.0;
+    for (int i = 0; i < n; ++i) t += C(i, j) * G[i];
+    F[j] = t;
+  }
+  left_multiply_transpose(Km, C);
+  right_multiply(Km, C);
+  for (int i = 0; i < m; ++i) {
+    auto pos = s.crows.find(dofs[i]);
+    if (pos == s.crows.end()) continue;
+    for (int j = 0; j < m; ++j) Km(i, j) = 0.0;
+    Km(i, i) = 1.0;
+    if (!s.asymmetric) {
+      F[i] = 0.0;
+      continue;
+    }
+    for (const auto &it : pos->second)
+      for (int j = 0; j < m; ++j)
+        if (dofs[j] == it.first) Km(i, j) = -it.second;
+    auto v = s.cvals.find(dofs[i]);
+    F[i] = v == s.cvals.end() ? 0.0 : v->second;
+  }
+  K = Km.a;
 }
 
 // EigenSparseMatrix::add_matrix (src/numerics/eigen_sparse_matrix.C:121-135,
@@ -844,6 +1118,79 @@ void orc_pattern(const orc_system *s, int32_t *rp, int32_t *ci) {
   std::memcpy(ci, s->s.ci.data(), s->s.ci.size() * 4);
 }
 
+int orc_add_constraint_row(orc_system *o, int32_t dof, int n, const int32_t *cols, const double *coefs, double rhs,
+                           int forbid) {
+  ConstraintRow row;
+  for (int k = 0; k < n; ++k) row[cols[k]] = coefs[k];
+  return add_constraint_row(o->s, dof, row, rhs, forbid != 0);
+}
+
+void orc_set_asymmetric_constraint_rows(orc_system *o, int on) { o->s.asymmetric = on != 0; }
+
+int orc_process_constraints(orc_system *o, int error_on_loop) {
+  if (error_on_loop && constraint_loop(o->s)) return -1;
+  process_constraints(o->s);
+  if (o->s.general)
+    build_sparsity_general(o->s);
+  else
+    build_sparsity(o->s);
+  return 0;
+}
+
+void orc_constraint_counts(const orc_system *o, int64_t *n_rows, int64_t *n_entries) {
+  *n_rows = (int64_t)o->s.crows.size();
+  int64_t k = 0;
+  for (const auto &r : o->s.crows) k += (int64_t)r.second.size();
+  *n_entries = k;
+}
+
+void orc_constraint_rows(const orc_system *o, int32_t *dofs, int32_t *rp, int32_t *cols, double *coefs, double *rhs) {
+  int64_t r = 0, k = 0;
+  rp[0] = 0;
+  for (const auto &row : o->s.crows) {
+    dofs[r] = row.first;
+    auto v = o->s.cvals.find(row.first);
+    rhs[r] = v == o->s.cvals.end() ? 0.0 : v->second;
+    for (const auto &it : row.second) {
+      cols[k] = it.first;
+      coefs[k] = it.second;
+      ++k;
+    }
+    rp[++r] = (int32_t)k;
+  }
+}
+
+int orc_constrain_element(const orc_system *o, int64_t e, const double *K, const double *F, int32_t *dofs_out,
+                          double *K_out, double *F_out, int cap) {
+  const System &s = o->s;
+  const int n = s.dpe;
+  std::vector<int32_t> d(&s.edofs[(size_t)e * n], &s.edofs[(size_t)e * n] + n);
+  std::vector<double> Kx(K, K + (size_t)n * n), Fx(F, F + n);
+  constrain_general(s, d, Kx, Fx);
+  const int m = (int)d.size();
+  if (m > cap) return -1;
+  std::copy(d.begin(), d.end(), dofs_out);
+  std::copy(Kx.begin(), Kx.end(), K_out);
+  std::copy(Fx.begin(), Fx.end(), F_out);
+  return m;
+}
+
+void orc_enforce_constraints(const orc_system *o, double *x, int homogeneous) {
+  // DofMap::enforce_constraints_exactly (dof_map_constraints.C:3088-3170):
+  // x_i = value_i + sum_j c_ij x_j on every constrained DoF, reading the
+  // vector as it was on entry
+  const std::vector<double> old(x, x + o->s.ndofs);
+  for (const auto &row : o->s.crows) {
+    double t = 0.0;
+    if (!homogeneous) {
+      auto v = o->s.cvals.find(row.first);
+      if (v != o->s.cvals.end()) t = v->second;
+    }
+    for (const auto &it : row.second) t += it.second * old[it.first];
+    x[row.first] = t;
+  }
+}
+
 int orc_element_poisson(const orc_system *os, int npts, const double *kappa, int f_kind, int64_t e0,
                         int64_t count, int constrain_flag, double *Ke, double *Fe) {
   const System &s = os->s;
@@ -870,6 +1217,17 @@ int orc_assemble_poisson(const orc_system *os, int npts, const double *kappa, in
   std::vector<double> K((size_t)n * n), F(n);
   for (i64 e = 0; e < s.m->n_elem; ++e) {
     if (!element_poisson(s, fe, e, kappa, f_kind, K.data(), F.data())) return -1;
+    if (s.general) {
+      std::vector<int32_t> d(&s.edofs[(size_t)e * n], &s.edofs[(size_t)e * n] + n);
+      std::vector<double> Kx(K), Fx(F);
+      constrain_general(s, d, Kx, Fx);
+      const int m = (int)d.size();
+      for (int i = 0; i < m; ++i) {
+        for (int j = 0; j < m; ++j) vals[slot(s, d[i], d[j])] += Kx[(size_t)i * m + j];
+        rhs[d[i]] += Fx[i];
+      }
+      continue;
+    }
     constrain(s, e, K.data(), F.data());
     scatter(s, e, K.data(), F.data(), vals, rhs);
   }

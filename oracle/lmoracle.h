@@ -59,6 +59,30 @@ void orc_elem_dofs(const orc_system *s, int32_t *dofs);       /* n_elem*dpe */
 void orc_constrained(const orc_system *s, uint8_t *flags);     /* n_dofs */
 void orc_pattern(const orc_system *s, int32_t *row_ptr, int32_t *col_idx);
 
+/* --- general constraint rows (dof_map_constraints.C:2180-2208 add_constraint_row,
+ *     4341-4497 process_constraints, 4509-4597 check_for_constraint_loops,
+ *     3545-3660 build_constraint_matrix_and_vector, 2518-2616 heterogeneous
+ *     element constraint; dof_map.C:2915-2972 find_connected_dofs) ---
+ * Dirichlet DoFs of orc_system_create hold empty rows.  add returns -1 when
+ * forbid is set and the DoF is constrained; process returns -1 on a loop when
+ * error_on_loop is set, and rebuilds the pattern (connected DoFs included). */
+int orc_add_constraint_row(orc_system *s, int32_t dof, int n, const int32_t *cols, const double *coefs, double rhs,
+                           int forbid);
+int orc_process_constraints(orc_system *s, int error_on_loop);
+/* asymmetric_constraint_rows of the element constraint (default 1). */
+void orc_set_asymmetric_constraint_rows(orc_system *s, int on);
+void orc_constraint_counts(const orc_system *s, int64_t *n_rows, int64_t *n_entries);
+void orc_constraint_rows(const orc_system *s, int32_t *dofs, int32_t *row_ptr, int32_t *cols, double *coefs,
+                         double *rhs);
+/* Constrained element matrix/vector of element e from its unconstrained K, F:
+ * returns the expanded size m (<= cap) with dofs_out[m], K_out[m*m], F_out[m];
+ * -1 if m > cap. */
+int orc_constrain_element(const orc_system *s, int64_t e, const double *K, const double *F, int32_t *dofs_out,
+                          double *K_out, double *F_out, int cap);
+/* x_i = value_i (0 if homogeneous) + sum_j c_ij x_j for every constrained DoF
+ * (enforce_constraints_exactly). */
+void orc_enforce_constraints(const orc_system *s, double *x, int homogeneous);
+
 /* --- assembly (introduction_ex4.C:313-560; fe.C:197-368; fe_map.C:958-1090,
  *     1167-1246; dof_map_constraints.C:2518-2616; eigen_sparse_matrix.C:121-135) ---
  * f_kind: 0 -> f = 1; 1 -> f = 3 pi^2 sin(pi x) sin(pi y) sin(pi z).

@@ -56,6 +56,13 @@ def lib():
             "orc_elem_dofs": (None, [_P, _P]),
             "orc_constrained": (None, [_P, _P]),
             "orc_pattern": (None, [_P, _P, _P]),
+            "orc_add_constraint_row": (C.c_int, [_P, C.c_int32, C.c_int, _P, _P, _D, C.c_int]),
+            "orc_process_constraints": (C.c_int, [_P, C.c_int]),
+            "orc_set_asymmetric_constraint_rows": (None, [_P, C.c_int]),
+            "orc_constraint_counts": (None, [_P, _P, _P]),
+            "orc_constraint_rows": (None, [_P, _P, _P, _P, _P, _P]),
+            "orc_constrain_element": (C.c_int, [_P, _I64, _P, _P, _P, _P, _P, C.c_int]),
+            "orc_enforce_constraints": (None, [_P, _P, C.c_int]),
             "orc_assemble_poisson": (C.c_int, [_P, C.c_int, _P, C.c_int, _P, _P]),
             "orc_element_poisson": (C.c_int, [_P, C.c_int, _P, C.c_int, _I64, _I64, C.c_int, _P, _P]),
             "orc_assemble_elasticity": (C.c_int, [_P, C.c_int, _D, _D, _D, _P, _P]),
@@ -179,6 +186,51 @@ class System:
         ci = np.empty(self.nnz, dtype=np.int32)
         lib().orc_pattern(self.h, _ptr(rp), _ptr(ci))
         return rp, ci
+
+    def add_constraint_row(self, dof, row, rhs=0.0, forbid_overwrite=True):
+        """DofMap::add_constraint_row; row is {dof: coefficient}."""
+        cols = np.array(list(row.keys()), dtype=np.int32)
+        coefs = np.array(list(row.values()), dtype=np.float64)
+        if lib().orc_add_constraint_row(self.h, int(dof), len(cols), _ptr(cols), _ptr(coefs), float(rhs),
+                                        int(forbid_overwrite)) != 0:
+            raise RuntimeError(f"DOF {dof} was already constrained")
+
+    def set_asymmetric_constraint_rows(self, on):
+        lib().orc_set_asymmetric_constraint_rows(self.h, int(on))
+
+    def process_constraints(self, error_on_loop=False):
+        if lib().orc_process_constraints(self.h, int(error_on_loop)) != 0:
+            raise RuntimeError("Constraint loop detected")
+        self.nnz = lib().orc_nnz(self.h)
+
+    def constraint_rows(self):
+        """{dof: ({dof: coefficient}, value)} after the last processing."""
+        nr, ne = C.c_int64(), C.c_int64()
+        lib().orc_constraint_counts(self.h, C.byref(nr), C.byref(ne))
+        d = np.empty(nr.value, dtype=np.int32)
+        rp = np.empty(nr.value + 1, dtype=np.int32)
+        cols = np.empty(max(ne.value, 1), dtype=np.int32)
+        coefs = np.empty(max(ne.value, 1))
+        rhs = np.empty(max(nr.value, 1))
+        lib().orc_constraint_rows(self.h, _ptr(d), _ptr(rp), _ptr(cols), _ptr(coefs), _ptr(rhs))
+        return {int(d[k]): ({int(cols[t]): float(coefs[t]) for t in range(rp[k], rp[k + 1])}, float(rhs[k]))
+                for k in range(nr.value)}
+
+    def constrain_element(self, e, K, F, cap=256):
+        K = np.ascontiguousarray(K, dtype=np.float64)
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        d = np.empty(cap, dtype=np.int32)
+        Ko = np.empty(cap * cap)
+        Fo = np.empty(cap)
+        m = lib().orc_constrain_element(self.h, int(e), _ptr(K), _ptr(F), _ptr(d), _ptr(Ko), _ptr(Fo), cap)
+        if m < 0:
+            raise RuntimeError("expanded element larger than cap")
+        return d[:m].copy(), Ko[:m * m].reshape(m, m).copy(), Fo[:m].copy()
+
+    def enforce_constraints(self, x, homogeneous=False):
+        x = np.array(x, dtype=np.float64)
+        lib().orc_enforce_constraints(self.h, _ptr(x), int(homogeneous))
+        return x
 
     def assemble_poisson(self, npts, kappa=None, f_kind=0):
         vals = np.empty(self.nnz)

@@ -1,0 +1,132 @@
+"""GPU parity of general constraint rows (lmh_dofmap_set_constraints and the
+constrained assembly) against the oracle's restatement of
+DofMap::add_constraint_row / process_constraints / the heterogeneous element
+constraint (dof_map_constraints.C:2180-2208, 4341-4497, 2518-2616).
+
+Bars: processed rows identical (same substitution arithmetic); sparsity
+pattern bit-exact; FP64 entries and rhs within 1e-12 normwise relative (the
+general elements' contributions are added after the others, a different
+summation order); FP32 operator (and the FP32 assembly's rhs, whose K H terms
+come from FP32 element matrices) within 2e-6; symmetric rows + CG + enforce
+within 1e-10 of the oracle's solution with its iteration count +-2.
+"""
+import numpy as np
+import pytest
+
+import libmesh_amd as L
+import oracle as O
+from constraint_cases import Y0, Y1, Z0, Z1, case_rows
+
+pytestmark = pytest.mark.gpu
+
+SEED_PERTURB = 0x5EED0001
+SEED_KAPPA = 0x5EED0002
+NPTS = {L.HEX8: 2, L.HEX27: 3, L.TET4: 3, L.TET10: 5}
+
+CASES = [
+    ("periodic_x", L.HEX27, 4, Z0 | Y0 | Y1 | Z1),
+    ("periodic_xy", L.HEX8, 6, Z0 | Z1),
+    ("random", L.TET10, 2, L.ALL_FACES),
+]
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def build(ctx, name, t, n, mask, symmetric=False):
+    p = L.Problem(n, t, 1, mask)
+    m = O.Mesh(n, t)
+    if t in (L.HEX8, L.HEX27):
+        p.perturb(SEED_PERTURB, 0.2)
+        m.perturb(SEED_PERTURB, 0.2)
+    s = O.System(m, 1, mask)
+    assert np.array_equal(p.elem_dofs, s.elem_dofs())
+    rows = case_rows(name, p.coords, p.conn, p.elem_dofs, p.constrained, n)
+    print(f"case {name} type {t} n {n}: {len(rows)} rows", flush=True)
+    for d, row, v in rows:
+        s.add_constraint_row(d, row, v)
+    s.set_asymmetric_constraint_rows(not symmetric)
+    s.process_constraints(error_on_loop=True)
+    g = L.GpuProblem.from_problem(ctx, p)
+    g.set_constraints(rows, error_on_loop=True, symmetric_rows=symmetric)
+    return p, m, s, g
+
+
+@pytest.mark.parametrize("name,t,n,mask", CASES)
+def test_rows_and_pattern(ctx, name, t, n, mask):
+    p, m, s, g = build(ctx, name, t, n, mask)
+    assert g.constraint_rows() == s.constraint_rows()
+    assert g.n_general_elements() > 0
+    rp, ci = s.pattern()
+    grp, gci = g.pattern()
+    assert g.nnz == s.nnz
+    assert np.array_equal(grp, rp) and np.array_equal(gci, ci)
+
+
+@pytest.mark.parametrize("scatter", [L.SCATTER_GATHER, L.SCATTER_COLORED])
+@pytest.mark.parametrize("name,t,n,mask", CASES)
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_constrained_assembly(ctx, name, t, n, mask, scatter, symmetric):
+    p, m, s, g = build(ctx, name, t, n, mask, symmetric)
+    kappa = L.lognormal(p.n_elem, SEED_KAPPA)
+    ov, orhs = s.assemble_poisson(NPTS[t], kappa, 1)
+    kd = L.DeviceArray(ctx, p.n_elem, data=kappa)
+    b = L.DeviceArray(ctx, p.n_dofs)
+    for _ in range(2):  # a second assembly must not see the first one's zeroed blocks
+        g.assemble_poisson(NPTS[t], kd, 1, b, L.FP64, scatter)
+        assert rel(g.values(), ov) < 1e-12
+        assert rel(b.download(), orhs) < 1e-12
+    g.assemble_poisson(NPTS[t], kd, 1, b, L.FP32, scatter)
+    assert rel(g.values_fp32().astype(np.float64), ov) < 2e-6
+    # exact but for the K H terms of nonzero values, formed from FP32 element matrices
+    assert rel(b.download(), orhs) < 2e-6
+
+
+@pytest.mark.parametrize("name,t,n,mask", CASES)
+def test_constrained_solve(ctx, name, t, n, mask):
+    p, m, s, g = build(ctx, name, t, n, mask, symmetric=True)
+    kappa = L.lognormal(p.n_elem, SEED_KAPPA)
+    ov, orhs = s.assemble_poisson(NPTS[t], kappa, 1)
+    rp, ci = s.pattern()
+    ox, oits, _ = O.cg_eigen(rp, ci, ov, orhs, 1e-10, 5000)
+    ox = s.enforce_constraints(ox)
+    kd = L.DeviceArray(ctx, p.n_elem, data=kappa)
+    b = L.DeviceArray(ctx, p.n_dofs)
+    g.assemble_poisson(NPTS[t], kd, 1, b, L.FP64, L.SCATTER_GATHER)
+    x = L.DeviceArray(ctx, p.n_dofs)
+    x.zero()
+    its, res, reason = g.cg(b, x, 1e-10)
+    assert reason == L.CONVERGED_RTOL and abs(its - oits) <= 2
+    g.enforce_constraints(x)
+    assert rel(x.download(), ox) < 1e-9
+    # mixed precision on the same symmetric system
+    g.assemble_poisson(NPTS[t], kd, 1, None, L.FP32, L.SCATTER_GATHER)  # operator only; FP64 b stays
+    x2 = L.DeviceArray(ctx, p.n_dofs)
+    x2.zero()
+    its2, res2, reason2, _ = g.mixed(b, x2, 1e-10)
+    assert reason2 == L.CONVERGED_RTOL
+    g.enforce_constraints(x2)
+    assert rel(x2.download(), ox) < 1e-8
+    # constrained entries satisfy their rows exactly
+    xs = x.download()
+    for d, (row, v) in s.constraint_rows().items():
+        assert abs(xs[d] - (v + sum(c * xs[j] for j, c in row.items()))) <= 1e-14 * max(1.0, abs(xs[d]))
+
+
+def test_constraint_errors(ctx):
+    p = L.Problem(2, L.HEX8, 1, 0)
+    g = L.GpuProblem.from_problem(ctx, p)
+    with pytest.raises(L.LmhError, match="loop"):
+        g.set_constraints([(0, {1: 1.0}, 0.0), (1, {0: 1.0}, 0.0)], error_on_loop=True)
+    with pytest.raises(L.LmhError, match="already constrained"):
+        g.set_constraints([(0, {1: 1.0}, 0.0), (0, {2: 1.0}, 0.0)], forbid_overwrite=True)
+    with pytest.raises(L.LmhError, match="itself"):
+        g.set_constraints([(3, {3: 1.0}, 0.0)])
+    with pytest.raises(L.LmhError, match="out of range"):
+        g.set_constraints([(3, {p.n_dofs: 1.0}, 0.0)])
+    # a valid set after the failures, then back to no rows at all
+    g.set_constraints([(0, {1: 0.5}, 0.25)])
+    assert g.constraint_rows() == {0: ({1: 0.5}, 0.25)}
+    g.set_constraints([])
+    assert g.constraint_rows() == {} and g.n_general_elements() == 0
