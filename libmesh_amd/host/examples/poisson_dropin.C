@@ -4,10 +4,17 @@
 //   sys.linear_solver = make_unique<HipLinearSolver>             (ImplicitSystem::linear_solver)
 //   sys.attach_assemble_object(HipAssembler(...))                (System::attach_assemble_object)
 // Usage: poisson_dropin n elem_type(10|12|8|9) out_prefix [perturb] [kappa] [sine] [mixed] [adjoint] [distort]
+//                      [periodic]
 // Writes <prefix>_u.bin (solution) and, with `adjoint`, <prefix>_grad.bin.
+// `periodic`: Dirichlet on the y and z faces only, u(x=1) = u(x=0) through
+// constraint rows from a System::Constraint object (plus one heterogeneous
+// row u = 0.3 on an x=0 face DoF), symmetric constraint rows, and
+// enforce_constraints_exactly after the solve.
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <map>
 #include <memory>
 #include <string>
 
@@ -17,6 +24,50 @@
 #include "libmesh/system.h"
 
 using namespace libMesh;
+
+// Periodic-style rows on the unit cube: every unconstrained DoF on x = 1 equals
+// the DoF on x = 0 at the same (y, z) (boundary nodes sit exactly on the
+// lattice).  A first row gives the first interior DoF of the x = 0 face the
+// value 0.3, so the rows of its partner chain through it.
+class PeriodicX : public System::Constraint {
+ public:
+  PeriodicX(System &sys, const MeshBase &mesh, unsigned cells) : _sys(sys), _mesh(mesh), _cells(cells) {}
+  void constrain() override {
+    DofMap &dm = _sys.get_dof_map();
+    std::vector<dof_id_type> node_dof(_mesh.n_nodes()), di;
+    for (dof_id_type e = 0; e < _mesh.n_elem(); ++e) {
+      dm.dof_indices(e, di);
+      for (unsigned a = 0; a < _mesh.nodes_per_elem(); ++a) node_dof[_mesh.node_id(e, a)] = di[a];
+    }
+    auto key = [&](const Point &p) {
+      return std::make_pair((long)std::lround(p.y * 2 * _cells), (long)std::lround(p.z * 2 * _cells));
+    };
+    std::map<std::pair<long, long>, dof_id_type> near;
+    bool valued = false;
+    for (dof_id_type k = 0; k < _mesh.n_nodes(); ++k) {
+      const Point p = _mesh.point(k);
+      if (std::abs(p.x) > 1e-12) continue;
+      near[key(p)] = node_dof[k];
+      const bool interior = p.y > 1e-9 && p.y < 1 - 1e-9 && p.z > 1e-9 && p.z < 1 - 1e-9;
+      if (!valued && interior && !dm.is_constrained_dof(node_dof[k])) {
+        dm.add_constraint_row(node_dof[k], DofConstraintRow(), 0.3, true);
+        valued = true;
+      }
+    }
+    for (dof_id_type k = 0; k < _mesh.n_nodes(); ++k) {
+      const Point p = _mesh.point(k);
+      if (std::abs(p.x - 1) > 1e-12 || dm.is_constrained_dof(node_dof[k])) continue;
+      DofConstraintRow row;
+      row[near.at(key(p))] = 1.0;
+      dm.add_constraint_row(node_dof[k], row, true);
+    }
+  }
+
+ private:
+  System &_sys;
+  const MeshBase &_mesh;
+  unsigned _cells;
+};
 
 static void write_bin(const std::string &path, const std::vector<double> &v) {
   std::ofstream f(path, std::ios::binary);
@@ -31,7 +82,8 @@ int main(int argc, char **argv) {
   const unsigned n = std::atoi(argv[1]);
   const ElemType type = (ElemType)std::atoi(argv[2]);
   const std::string prefix = argv[3];
-  bool perturb = false, kappa = false, sine = false, mixed = false, adjoint = false, distort = false;
+  bool perturb = false, kappa = false, sine = false, mixed = false, adjoint = false, distort = false,
+       periodic = false;
   for (int a = 4; a < argc; ++a) {
     perturb |= !std::strcmp(argv[a], "perturb");
     kappa |= !std::strcmp(argv[a], "kappa");
@@ -39,6 +91,7 @@ int main(int argc, char **argv) {
     mixed |= !std::strcmp(argv[a], "mixed");
     adjoint |= !std::strcmp(argv[a], "adjoint");
     distort |= !std::strcmp(argv[a], "distort");
+    periodic |= !std::strcmp(argv[a], "periodic");
   }
   try {
     Mesh mesh;
@@ -56,7 +109,15 @@ int main(int argc, char **argv) {
     LinearImplicitSystem &sys = es.add_system<LinearImplicitSystem>("Poisson");
     const Order order = (type == HEX8 || type == TET4) ? FIRST : SECOND;
     sys.add_variable("u", order, LAGRANGE);
-    sys.get_dof_map().add_dirichlet_boundary(DirichletBoundary({0, 1, 2, 3, 4, 5}, {0}));
+    PeriodicX periodic_rows(sys, mesh, n);
+    if (periodic) {
+      sys.get_dof_map().add_dirichlet_boundary(DirichletBoundary({0, 1, 3, 5}, {0}));
+      sys.get_dof_map().set_error_on_constraint_loop(true);
+      sys.get_dof_map().set_asymmetric_constraint_rows(false);
+      sys.attach_constraint_object(periodic_rows);
+    } else {
+      sys.get_dof_map().add_dirichlet_boundary(DirichletBoundary({0, 1, 2, 3, 4, 5}, {0}));
+    }
     sys.add_matrix("System Matrix", std::make_unique<HipCsrMatrix<Number>>());
     auto solver = std::make_unique<HipLinearSolver<Number>>();
     solver->set_mixed_precision(mixed);
@@ -72,6 +133,7 @@ int main(int argc, char **argv) {
     if (kappa) assembler.set_coefficient(MeshTools::lognormal_coefficients(mesh.n_elem(), 0x5EED0002));
 
     sys.solve();
+    if (periodic) sys.get_dof_map().enforce_constraints_exactly(sys);
     auto *lin = static_cast<HipLinearSolver<Number> *>(sys.linear_solver.get());
     std::printf("dofs %u iterations %u residual %.6e reason %d\n", sys.get_dof_map().n_dofs(),
                 sys.n_linear_iterations(), sys.final_linear_residual(), (int)lin->get_converged_reason());

@@ -47,6 +47,16 @@ class System {
   void attach_assemble_function(void (*fptr)(EquationSystems &, const std::string &));
   void attach_assemble_object(Assembly &assemble);
 
+  // System::Constraint (include/systems/system.h:232): user constraint rows,
+  // added with DofMap::add_constraint_row when the DoFs are distributed
+  // (DofMap::create_dof_constraints calls it, dof_map_constraints.C:1736-1800)
+  class Constraint {
+   public:
+    virtual ~Constraint() = default;
+    virtual void constrain() = 0;
+  };
+  void attach_constraint_object(Constraint &constrain) { _constrain_obj = &constrain; }
+
   // System::init -> init_data (dof distribution, vectors) + init_matrices (system.C:196-371)
   virtual void init();
   // System::assemble -> user_assembly (system.C:554-561, 2109-2119)
@@ -66,6 +76,7 @@ class System {
   std::unique_ptr<NumericVector<Number>> _rhs;
   void (*_assemble_fptr)(EquationSystems &, const std::string &) = nullptr;
   Assembly *_assemble_obj = nullptr;
+  Constraint *_constrain_obj = nullptr;
 };
 
 class ImplicitSystem : public System {

@@ -62,6 +62,22 @@ void DofMap::distribute_dofs(const MeshBase &mesh) {
           for (unsigned v : d.variables) _cons[_edofs[(size_t)e * _dpe + v * npe + a]] = 1;
         }
       }
+  _dirichlet_cons = _cons;
+  _rows.clear();
+}
+
+void DofMap::add_constraint_row(dof_id_type dof_number, const DofConstraintRow &constraint_row, Number constraint_rhs,
+                                bool forbid_constraint_overwrite) {
+  if (dof_number >= _n_dofs) libmesh_error_msg("add_constraint_row: DoF " << dof_number << " out of range");
+  if (forbid_constraint_overwrite && is_constrained_dof(dof_number))
+    libmesh_error_msg("ERROR: DOF " << dof_number << " was already constrained!");
+  if (constraint_row.count(dof_number))
+    libmesh_error_msg("Error: constraint_row for dof " << dof_number << " should not contain an entry for dof "
+                                                       << dof_number);
+  for (const auto &it : constraint_row)
+    if (it.first >= _n_dofs) libmesh_error_msg("add_constraint_row: DoF " << it.first << " out of range");
+  _rows.emplace_back(dof_number, constraint_row, constraint_rhs);
+  _cons[dof_number] = 1;
 }
 
 void DofMap::dof_indices(dof_id_type e, std::vector<dof_id_type> &di) const {

@@ -18,6 +18,8 @@
 
 #include "libmesh/hip_csr_matrix.h"
 
+#include <tuple>
+
 #include <fstream>
 
 #include "libmesh/matrix_io.h"
@@ -67,8 +69,29 @@ void HipCsrMatrix<T>::init(ParallelType) {
                             mesh.connectivity().data(), &_mesh),
             "HipCsrMatrix::init (mesh)");
   hip_check(lmh_dofmap_create(_ctx, _mesh, (int)dm->n_variables(), dm->n_dofs(), dm->elem_dofs().data(),
-                              dm->constrained().data(), &_dm),
+                              dm->dirichlet_constrained().data(), &_dm),
             "HipCsrMatrix::init (dof map)");
+  // constraint rows of the DofMap (processed on the device), then the pattern
+  // that couples each element's DoFs with the DoFs they are constrained by
+  const auto &rows = dm->added_constraint_rows();
+  const int flags = (dm->get_error_on_constraint_loop() ? LMH_CONSTRAINT_ERROR_ON_LOOP : 0) |
+                    (dm->asymmetric_constraint_rows() ? 0 : LMH_CONSTRAINT_SYMMETRIC_ROWS);
+  if (!rows.empty() || flags) {
+    std::vector<int32_t> dof, ptr{0}, col;
+    std::vector<double> coef, val;
+    for (const auto &r : rows) {
+      dof.push_back((int32_t)std::get<0>(r));
+      for (const auto &it : std::get<1>(r)) {
+        col.push_back((int32_t)it.first);
+        coef.push_back(it.second);
+      }
+      ptr.push_back((int32_t)col.size());
+      val.push_back(std::get<2>(r));
+    }
+    hip_check(lmh_dofmap_set_constraints(_dm, (int64_t)dof.size(), dof.data(), ptr.data(), col.data(), coef.data(),
+                                         val.data(), flags),
+              "HipCsrMatrix::init (constraint rows)");
+  }
   hip_check(lmh_matrix_create(_ctx, _dm, &_A), "HipCsrMatrix::init (sparsity)");
   _rows = dm->n_dofs();
   _cols = _rows;

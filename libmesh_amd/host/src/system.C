@@ -37,6 +37,7 @@ void System::attach_assemble_object(Assembly &a) {
 // System::init_data: distribute DoFs, size the vectors; then init_matrices
 void System::init() {
   _dof_map.distribute_dofs(_es.get_mesh());
+  if (_constrain_obj) _constrain_obj->constrain();
   auto sol = std::make_unique<HipVector<Number>>(_dof_map.n_dofs());
   solution = std::move(sol);
   _rhs = std::make_unique<HipVector<Number>>(_dof_map.n_dofs());

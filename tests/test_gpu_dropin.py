@@ -65,6 +65,36 @@ def test_poisson_c1_recipe_dropin_with_adjoint(tmp_path, mixed):
     assert np.abs(grad[elems] - fd).max() / np.abs(fd).max() < 1e-6
 
 
+def test_poisson_periodic_constraints_dropin(tmp_path):
+    """System::Constraint rows (u(x=1) = u(x=0) plus a heterogeneous row),
+    symmetric constraint rows, CG, enforce_constraints_exactly."""
+    from constraint_cases import Y0, Y1, Z0, Z1, periodic_rows, value_row_on
+
+    n = 4
+    out = run(["poisson_dropin", n, 12, tmp_path / "per", "perturb", "kappa", "sine", "periodic"], tmp_path)
+    u = np.fromfile(tmp_path / "per_u.bin")
+    m = O.Mesh(n, O.HEX27)
+    m.perturb(0x5EED0001, 0.2)
+    kappa = O.kappa_lognormal(m.n_elem, 0x5EED0002)
+    s = O.System(m, 1, Z0 | Y0 | Y1 | Z1)
+    taken = set()
+    args = (m.coords(), m.conn(), s.elem_dofs(), s.constrained())
+    rows = value_row_on(*args, taken, 0, 0.3) + periodic_rows(*args, n, (0,), taken)
+    for d, row, v in rows:
+        s.add_constraint_row(d, row, v)
+    s.set_asymmetric_constraint_rows(False)
+    s.process_constraints(error_on_loop=True)
+    v, b = s.assemble_poisson(3, kappa, 1)
+    rp, ci = s.pattern()
+    x, its, _ = O.cg_eigen(rp, ci, v, b, 1e-10, 20000)
+    x = s.enforce_constraints(x)
+    assert np.abs(u - x).max() <= 1e-9 * np.abs(x).max(), out
+    d0 = rows[0][0]
+    assert u[d0] == 0.3, out
+    gits = int(out.split("iterations ")[1].split()[0])
+    assert abs(gits - its) <= 2, out
+
+
 def test_degenerate_map_raises(tmp_path):
     """An inverted element raises DegenerateMap (fe_map.C:467-505) through the C ABI."""
     out = run(["poisson_dropin", 2, 12, tmp_path / "bad", "distort"], tmp_path, expect=3)
