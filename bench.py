@@ -80,6 +80,82 @@ def read_traffic(path):
     return None
 
 
+def periodic_x_rows(p, n):
+    """Rows u(x=1) = u(x=0) for the unconstrained DoFs of the x = 1 face of a
+    build_cube mesh with Dirichlet rows on the y and z faces (boundary nodes
+    sit on the lattice, so (y, z) * 2n are integers)."""
+    nd = np.full(p.n_nodes, -1, dtype=np.int64)
+    nd[p.conn.ravel()] = p.elem_dofs[:, :p.npe].ravel()
+    X = p.coords
+    key = np.rint(X[:, 1:] * 2 * n).astype(np.int64)
+    k1 = key[:, 0] * (2 * n + 1) + key[:, 1]
+    near = np.abs(X[:, 0]) < 1e-12
+    far = np.abs(X[:, 0] - 1.0) < 1e-12
+    lut = np.full((2 * n + 1) ** 2, -1, dtype=np.int64)
+    lut[k1[near]] = nd[near]
+    fd = nd[far]
+    keep = p.constrained[fd] == 0
+    return [(int(d), {int(c): 1.0}, 0.0) for d, c in zip(fd[keep], lut[k1[far][keep]])]
+
+
+def bench_constrained(args, L, ctx, kappa, log):
+    """The same problem with general constraint rows: Dirichlet on the y and z
+    faces, u(x=1) = u(x=0), symmetric constraint rows; assembly kernels and a
+    mixed solve followed by enforce_constraints_exactly."""
+    p = L.Problem(args.n, L.HEX27, 1, 1 | 2 | 8 | 32)
+    p.perturb(SEED_PERTURB, 0.2)
+    rows = periodic_x_rows(p, args.n)
+    g = L.GpuProblem.from_problem(ctx, p)
+    t0 = time.perf_counter()
+    g.set_constraints(rows, error_on_loop=True, symmetric_rows=True)
+    ctx.sync()
+    t_set = time.perf_counter() - t0
+    kd = L.DeviceArray(ctx, p.n_elem, data=kappa)
+    b = L.DeviceArray(ctx, p.n_dofs)
+    x = L.DeviceArray(ctx, p.n_dofs)
+    g.assemble_poisson(3, kd, 0, b, L.FP64)
+    ctx.sync()
+    reps = 3
+    T0 = time.perf_counter()
+    for _ in range(reps):
+        g.assemble_poisson(3, kd, 0, b, L.FP64)
+    ctx.sync()
+    t_asm = (time.perf_counter() - T0) / reps
+    ctx.profile_reset()
+    ctx.profile(True)
+    g.assemble_poisson(3, kd, 0, b, L.FP64)
+    ctx.sync()
+    ctx.profile(False)
+    kern = {}
+    for k in ("element_hex27_mfma_fp64", "constrain_general_fp64", "scatter_gather_fp64", "scatter_expanded_fp64"):
+        ms, c = ctx.profile_query(k)
+        kern[k] = round(ms * c, 4)
+    g.assemble_poisson(3, kd, 0, None, L.FP32)
+    x.zero()
+    ctx.sync()
+    T0 = time.perf_counter()
+    its, res, reason, st = g.mixed(b, x, args.rtol, args.inner, 0, 20000)
+    g.enforce_constraints(x)
+    ctx.sync()
+    t_solve = time.perf_counter() - T0
+    if reason != L.CONVERGED_RTOL:
+        raise RuntimeError(f"constrained solve did not converge: its={its} reason={reason}")
+    out = {
+        "rows": len(rows),
+        "general_elements": g.n_general_elements(),
+        "nnz": g.nnz,
+        "set_constraints_s": round(t_set, 3),
+        "assembly_fp64_ms": round(t_asm * 1e3, 3),
+        "assembly_kernels_ms": kern,
+        "mixed_solve_ms": round(t_solve * 1e3, 3),
+        "mixed_iterations": int(its),
+        "final_rel_residual": res,
+        "mdofs": round(p.n_dofs / (t_asm + t_solve) / 1e6, 4),
+    }
+    g.close()
+    return out
+
+
 def cpu_baseline(n_sample):
     """The oracle (C++ restatement of libMesh's FP64 path) on host cores, on a
     smaller instance of the same workload."""
@@ -380,6 +456,9 @@ def main():
     }
     log(f"adjoint: {adjoint}")
 
+    constrained = bench_constrained(args, L, ctx, kappa, log)
+    log(f"constrained: {constrained}")
+
     # assembly kernel timings (FP64 path, one profiled assembly)
     ctx.profile_reset()
     ctx.profile(True)
@@ -470,6 +549,7 @@ def main():
                                "avg_working_launch_ms": round(working_ms(results["fp64"]), 5)},
         "assembly_kernels_ms": {k: round(v, 4) for k, v in asm_k.items()},
         "adjoint": adjoint,
+        "constrained": constrained,
         "contraction_gflop": flops_contract / 1e9,
         "setup_s": round(t_setup, 3),
     }
