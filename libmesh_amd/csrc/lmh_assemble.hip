@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void k_poisson_hex8(const double *__restrict__
 // is the transpose of (0,1).  MFMA operand layout (16x16x4): lane l supplies
 // A[l%16][l/16] and B[l/16][l%16]; accumulator element r of lane l holds
 // D[drow(l/16, r)][l%16] (see MfmaT::drow).
-// The Gref^T operand is constant and held in registers for the whole kernel.
+// The constant Gref^T operand is read per k-step from the LDS derivative table.
 // ---------------------------------------------------------------------------
 template <typename CT>
 struct MfmaT;
