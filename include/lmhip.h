@@ -185,6 +185,16 @@ int lmh_vector_add_element_vectors(lmh_context *ctx, double *v_d, int64_t n_elem
  * matrices there. */
 int lmh_assemble_poisson(lmh_matrix *A, const lmh_mesh *m, const lmh_dofmap *dm, int npts, const double *kappa_d,
                          int f_kind, double *rhs_d, int precision, int scatter);
+/* Residual of the same physics at u (FEMSystem::assembly(get_residual = true,
+ * get_jacobian = false), fem_system.C:875-1010 with the residual of
+ * element_time_derivative, R_e = K_e u_e - F_e): element residuals from the
+ * unconstrained element matrices, C^T applied for the constraint rows
+ * (constrain_element_vector / constrain_element_matrix_and_vector,
+ * dof_map_constraints.C:2439-2516: constrained entries 0), added in element
+ * order into R_d (overwritten).  Uses the matrix's element buffers; general
+ * constraint rows are not supported here. */
+int lmh_assemble_residual_poisson(lmh_matrix *A, const lmh_mesh *m, const lmh_dofmap *dm, int npts,
+                                  const double *kappa_d, int f_kind, const double *u_d, double *R_d);
 /* Element matrices/vectors only (no scatter), constrained if constrain != 0:
  * Ke_d n_elem*dpe*dpe, Fe_d n_elem*dpe. */
 int lmh_element_poisson(lmh_context *ctx, const lmh_mesh *m, const lmh_dofmap *dm, int npts, const double *kappa_d,
@@ -198,6 +208,8 @@ int lmh_spmv_fp32(lmh_matrix *A, const float *x_d, float *y_d);
 /* r = b - A x in FP64 (LinearImplicitSystem::assembly residual, linear_implicit_system.C:367-381). */
 int lmh_residual(lmh_matrix *A, const double *b_d, const double *x_d, double *r_d);
 double lmh_norm2(lmh_context *ctx, const double *x_d, int64_t n);
+/* y += a x (NumericVector::add(a, v), numeric_vector.h). */
+int lmh_axpy(lmh_context *ctx, int64_t n, double a, const double *x_d, double *y_d);
 
 /* Jacobi-preconditioned CG, FP64 (EigenSparseLinearSolver::solve with CG +
  * JACOBI_PRECOND -> Eigen::internal::conjugate_gradient,

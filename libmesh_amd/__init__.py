@@ -64,6 +64,8 @@ HIP_API = {
     "lmh_dofmap_constraint_counts": (_I, [_P, _P, _P, _P]),
     "lmh_dofmap_get_constraints": (_I, [_P, _P, _P, _P, _P, _P]),
     "lmh_enforce_constraints": (_I, [_P, _P, _I]),
+    "lmh_assemble_residual_poisson": (_I, [_P, _P, _P, _I, _P, _I, _P, _P]),
+    "lmh_axpy": (_I, [_P, _I64, _D, _P, _P]),
     "lmh_matrix_create": (_I, [_P, _P, C.POINTER(_P)]),
     "lmh_matrix_create_csr": (_I, [_P, _I64, _P, _P, C.POINTER(_P)]),
     "lmh_matrix_destroy": (_I, [_P]),
@@ -433,6 +435,12 @@ class GpuProblem:
         check(hip().lmh_assemble_poisson(self.matrix, self.mesh, self.dofmap, npts,
                                          kappa_d.ptr if kappa_d is not None else None, f_kind,
                                          rhs_d.ptr if rhs_d is not None else None, precision, scatter))
+
+    def residual_poisson(self, npts, kappa_d, f_kind, u_d, R_d):
+        """FEMSystem residual R(u) = sum_e C^T (K_e u_e - F_e) of the Poisson physics."""
+        check(hip().lmh_assemble_residual_poisson(self.matrix, self.mesh, self.dofmap, npts,
+                                                  kappa_d.ptr if kappa_d is not None else None, f_kind, u_d.ptr,
+                                                  R_d.ptr))
 
     def element_poisson(self, npts, kappa_d, f_kind, Ke_d, Fe_d, constrain=True):
         check(hip().lmh_element_poisson(self.ctx.h, self.mesh, self.dofmap, npts,

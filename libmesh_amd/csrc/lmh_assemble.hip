@@ -492,6 +492,38 @@ __global__ void k_add_vec_gather(const int32_t *__restrict__ inc_ptr, const int3
   }
 }
 
+// Element residuals of the Poisson physics, R_e = K_e u_e - F_e from the
+// unconstrained element matrices, with C^T applied for empty constraint rows
+// (a constrained DoF's entry is 0); written over Fe.  One wave per element:
+// the element matrix is staged in LDS with coalesced loads, lane i forms row i.
+__global__ __launch_bounds__(256) void k_elem_residual(const double *__restrict__ Ke, double *__restrict__ Fe,
+                                                       const int32_t *__restrict__ edofs,
+                                                       const uint8_t *__restrict__ cons,
+                                                       const double *__restrict__ u, int dpe, int64_t ne) {
+  extern __shared__ double rsm[];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int dd = dpe * dpe;
+  double *K = rsm + wv * (dd + dpe), *ue = K + dd;
+  for (int64_t e = (int64_t)blockIdx.x * 4 + wv; e < ne; e += (int64_t)gridDim.x * 4) {
+    const double *src = Ke + e * dd;
+    for (int t = lane; t < dd; t += 64) K[t] = src[t];
+    const int32_t di = lane < dpe ? edofs[e * dpe + lane] : 0;
+    if (lane < dpe) ue[lane] = u[di];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane < dpe) {
+      double s = 0.0;
+      for (int j = 0; j < dpe; ++j) s += K[lane * dpe + j] * ue[j];
+      const int64_t t = e * dpe + lane;
+      Fe[t] = cons[di] ? 0.0 : s - Fe[t];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
 // ---------------------------------------------------------------------------
 // QoI derivative element vectors: Q_i = sum_{q: x_q in box} JxW_q phi_i(q)
 // ---------------------------------------------------------------------------
@@ -768,6 +800,28 @@ int lmh_vector_add_element_vectors(lmh_context *ctx, double *v_d, int64_t n_elem
   LMH_HIP(hipFree(ip));
   LMH_HIP(hipFree(lst));
   return LMH_OK;
+}
+
+int lmh_assemble_residual_poisson(lmh_matrix *A, const lmh_mesh *m, const lmh_dofmap *dm, int npts,
+                                  const double *kappa_d, int f_kind, const double *u_d, double *R_d) {
+  LMH_CHECK(A && m && dm && u_d && R_d, LMH_ERR_INVALID, "null argument");
+  LMH_CHECK(A->dm == dm && dm->mesh == m, LMH_ERR_INVALID, "matrix, dofmap and mesh do not belong together");
+  LMH_CHECK(dm->cr.n_general == 0, LMH_ERR_UNSUPPORTED, "residual assembly with general constraint rows");
+  LMH_CHECK(dm->dpe <= 32, LMH_ERR_UNSUPPORTED, "residual assembly needs at most 32 DoFs per element");
+  lmh_context *ctx = A->ctx;
+  const int64_t ne = m->n_elem, dd = (int64_t)dm->dpe * dm->dpe;
+  if (!A->Fe) LMH_HIP(hipMalloc((void **)&A->Fe, sizeof(double) * ne * dm->dpe));
+  if (!A->Ke) LMH_HIP(hipMalloc((void **)&A->Ke, sizeof(double) * ne * dd));
+  LMH_HIP(hipMemsetAsync(ctx->d_flags, 0, sizeof(int), ctx->stream));
+  LMH_TRY(element_poisson_impl<double>(ctx, m, dm, npts, kappa_d, f_kind, 0, A->Ke, A->Fe));
+  LMH_LAUNCH(ctx, "element_residual", k_elem_residual, dim3(grid_for((ne + 3) / 4, 1, ctx->cu_count * 8)), dim3(256),
+             sizeof(double) * 4 * ((size_t)dm->dpe * dm->dpe + dm->dpe), (const double *)A->Ke, A->Fe, dm->edofs,
+             dm->cons, u_d, dm->dpe, ne);
+  LMH_LAUNCH(ctx, "fill", k_fill<double>, dim3(grid_for(dm->n_dofs, 256, 8192)), dim3(256), 0, R_d, dm->n_dofs, 0.0);
+  LMH_LAUNCH(ctx, "add_element_vectors", k_add_vec_gather, dim3(grid_for(dm->n_dofs, 256, 8192)), dim3(256), 0,
+             dm->inc_ptr, dm->inc, dm->n_dofs, (const double *)A->Fe, R_d);
+  LMH_TRY(check_launch("residual"));
+  return read_flag(ctx, "lmh_assemble_residual_poisson");
 }
 
 int lmh_qoi_box_derivative(const lmh_mesh *m, const lmh_dofmap *dm, int npts, double lo, double hi, double *dq_d) {

@@ -316,6 +316,12 @@ __global__ __launch_bounds__(BLK) void k_fin_residual(const double *__restrict__
   }
 }
 
+// y += a x
+__global__ void k_axpy(int64_t n, double a, const double *__restrict__ x, double *__restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += a * x[i];
+}
+
 // Position of each row's diagonal entry (binary search of the sorted columns),
 // -1 when the row stores none.  Depends on the pattern only.
 __global__ void k_diag_index(int64_t n, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
@@ -822,6 +828,13 @@ double lmh_norm2(lmh_context *ctx, const double *x_d, int64_t n) {
     return NAN;
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) return NAN;
   return std::sqrt(v);
+}
+
+int lmh_axpy(lmh_context *ctx, int64_t n, double a, const double *x_d, double *y_d) {
+  LMH_CHECK(ctx && x_d && y_d && n >= 0, LMH_ERR_INVALID, "bad argument");
+  if (n == 0) return LMH_OK;
+  LMH_LAUNCH(ctx, "axpy", k_axpy, dim3(grid_for(n, 256, 8192)), dim3(256), 0, n, a, x_d, y_d);
+  return check_launch("k_axpy");
 }
 
 int lmh_matrix_make_fp32(lmh_matrix *A) {

@@ -32,6 +32,12 @@ class HipAssembler : public System::Assembly {
 
   void assemble() override;
 
+  // The same physics as the FEMSystem residual/Jacobian pair (fem_system.C:875-1010):
+  // R(u) = sum_e C^T (K_e u_e - F_e) into R, and the constrained Jacobian K into
+  // the system matrix (its rhs is not written).
+  void assemble_residual(const NumericVector<Number> &u, NumericVector<Number> &R);
+  void assemble_jacobian();
+
   // dJ/dkappa_e for J = integral of u over [lo,hi]^3, from the forward
   // solution u and the adjoint solution z (adjoint_qoi_parameter_sensitivity,
   // src/systems/implicit_system.C:517-623, for per-element parameters)

@@ -27,6 +27,10 @@ class NumericVector {
   virtual Real l2_norm() const = 0;
   virtual void localize(std::vector<T> &v) const = 0;
   virtual std::unique_ptr<NumericVector<T>> clone() const = 0;
+  // NumericVector::zero_clone: same size, zero entries
+  virtual std::unique_ptr<NumericVector<T>> zero_clone() const = 0;
+  // NumericVector::add(a, v): this += a v
+  virtual void add(T a, const NumericVector<T> &v) = 0;
 };
 
 template <typename T>
@@ -48,6 +52,8 @@ class HipVector : public NumericVector<T> {
   Real l2_norm() const override;
   void localize(std::vector<T> &v) const override;
   std::unique_ptr<NumericVector<T>> clone() const override;
+  std::unique_ptr<NumericVector<T>> zero_clone() const override;
+  void add(T a, const NumericVector<T> &v) override;
 
   // device storage (valid after close())
   T *device_data();

@@ -40,6 +40,30 @@ void HipAssembler::assemble() {
   b->mark_device_modified();
 }
 
+void HipAssembler::assemble_residual(const NumericVector<Number> &u_in, NumericVector<Number> &R_in) {
+  auto *A = dynamic_cast<HipCsrMatrix<Number> *>(&_system.get_system_matrix());
+  const auto *u = dynamic_cast<const HipVector<Number> *>(&u_in);
+  auto *R = dynamic_cast<HipVector<Number> *>(&R_in);
+  if (!A || !u || !R) libmesh_error_msg("assemble_residual: HipCsrMatrix / HipVector required");
+  if (!A->device_matrix()) libmesh_error_msg("assemble_residual: matrix not initialised from the DofMap");
+  const int arg = _qrule.kernel_arg(_mesh.elem_type());
+  hip_check(lmh_assemble_residual_poisson(A->device_matrix(), A->device_mesh(), A->device_dofmap(), arg, _kappa_d,
+                                          (int)_f, (const double *)u->device_data(), (double *)R->device_data()),
+            "HipAssembler::assemble_residual");
+  R->mark_device_modified();
+}
+
+void HipAssembler::assemble_jacobian() {
+  auto *A = dynamic_cast<HipCsrMatrix<Number> *>(&_system.get_system_matrix());
+  if (!A || !A->device_matrix()) libmesh_error_msg("assemble_jacobian: HipCsrMatrix initialised from the DofMap required");
+  const int arg = _qrule.kernel_arg(_mesh.elem_type());
+  hip_check(lmh_assemble_poisson(A->device_matrix(), A->device_mesh(), A->device_dofmap(), arg, _kappa_d, (int)_f,
+                                 nullptr, LMH_FP64, _scatter),
+            "HipAssembler::assemble_jacobian");
+  if (_fp32) hip_check(lmh_matrix_make_fp32(A->device_matrix()), "HipAssembler::assemble_jacobian (FP32 operator)");
+  A->mark_device_modified();
+}
+
 void HipAssembler::qoi_box_derivative(Real lo, Real hi, NumericVector<Number> &dq_in) {
   auto *A = dynamic_cast<HipCsrMatrix<Number> *>(&_system.get_system_matrix());
   auto *dq = dynamic_cast<HipVector<Number> *>(&dq_in);

@@ -108,6 +108,25 @@ std::unique_ptr<NumericVector<T>> HipVector<T>::clone() const {
 }
 
 template <typename T>
+std::unique_ptr<NumericVector<T>> HipVector<T>::zero_clone() const {
+  std::unique_ptr<HipVector<T>> c(new HipVector<T>(0));
+  c->init(_n);
+  c->zero();
+  c->close();
+  return c;
+}
+
+template <typename T>
+void HipVector<T>::add(T a, const NumericVector<T> &v) {
+  const auto *hv = dynamic_cast<const HipVector<T> *>(&v);
+  if (!hv || hv->size() != _n) libmesh_error_msg("HipVector::add(a, v): HipVector of the same size required");
+  close();
+  hip_check(lmh_axpy(_ctx, (int64_t)_n, (double)a, (const double *)hv->device_data(), (double *)_d),
+            "HipVector::add");
+  mark_device_modified();
+}
+
+template <typename T>
 T *HipVector<T>::device_data() {
   close();
   return _d;

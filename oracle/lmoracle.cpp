@@ -1234,6 +1234,32 @@ int orc_assemble_poisson(const orc_system *os, int npts, const double *kappa, in
   return 0;
 }
 
+// FEMSystem::assembly(get_residual) for the Poisson physics
+// (fem_system.C:875-1010; element_time_derivative of a Laplace-type physics:
+// R_e(i) = sum_qp JxW (kappa grad u . grad phi_i - f phi_i) = (K_e u_e - F_e)(i)),
+// then constrain_element_vector's C^T (dof_map_constraints.C:2439-2516: empty
+// rows give 0 at constrained DoFs), added in element order.
+int orc_residual_poisson(const orc_system *os, int npts, const double *kappa, int f_kind, const double *u,
+                         double *R) {
+  const System &s = os->s;
+  std::fill(R, R + s.ndofs, 0.);
+  FEValues fe;
+  fe.init(s.m->type, npts);
+  const int n = s.dpe;
+  std::vector<double> K((size_t)n * n), F(n);
+  for (i64 e = 0; e < s.m->n_elem; ++e) {
+    if (!element_poisson(s, fe, e, kappa, f_kind, K.data(), F.data())) return -1;
+    const int32_t *d = &s.edofs[(size_t)e * n];
+    for (int i = 0; i < n; ++i) {
+      if (s.cons[d[i]]) continue;
+      double r = 0.;
+      for (int j = 0; j < n; ++j) r += K[(size_t)i * n + j] * u[d[j]];
+      R[d[i]] += r - F[i];
+    }
+  }
+  return 0;
+}
+
 int orc_element_elasticity(const orc_system *os, int npts, double E, double nu, double fz, int64_t e0,
                            int64_t count, int constrain_flag, double *Ke, double *Fe) {
   const System &s = os->s;

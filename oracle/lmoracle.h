@@ -74,6 +74,9 @@ void orc_set_asymmetric_constraint_rows(orc_system *s, int on);
 void orc_constraint_counts(const orc_system *s, int64_t *n_rows, int64_t *n_entries);
 void orc_constraint_rows(const orc_system *s, int32_t *dofs, int32_t *row_ptr, int32_t *cols, double *coefs,
                          double *rhs);
+/* FEMSystem residual of the Poisson physics at u (Dirichlet rows only):
+ * R = sum_e C^T (K_e u_e - F_e), constrained entries 0.  Returns 0 or -1. */
+int orc_residual_poisson(const orc_system *s, int npts, const double *kappa, int f_kind, const double *u, double *R);
 /* Constrained element matrix/vector of element e from its unconstrained K, F:
  * returns the expanded size m (<= cap) with dofs_out[m], K_out[m*m], F_out[m];
  * -1 if m > cap. */

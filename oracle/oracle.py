@@ -63,6 +63,7 @@ def lib():
             "orc_constraint_rows": (None, [_P, _P, _P, _P, _P, _P]),
             "orc_constrain_element": (C.c_int, [_P, _I64, _P, _P, _P, _P, _P, C.c_int]),
             "orc_enforce_constraints": (None, [_P, _P, C.c_int]),
+            "orc_residual_poisson": (C.c_int, [_P, C.c_int, _P, C.c_int, _P, _P]),
             "orc_assemble_poisson": (C.c_int, [_P, C.c_int, _P, C.c_int, _P, _P]),
             "orc_element_poisson": (C.c_int, [_P, C.c_int, _P, C.c_int, _I64, _I64, C.c_int, _P, _P]),
             "orc_assemble_elasticity": (C.c_int, [_P, C.c_int, _D, _D, _D, _P, _P]),
@@ -231,6 +232,15 @@ class System:
         x = np.array(x, dtype=np.float64)
         lib().orc_enforce_constraints(self.h, _ptr(x), int(homogeneous))
         return x
+
+    def residual_poisson(self, npts, u, kappa=None, f_kind=0):
+        """FEMSystem residual R(u) of the Poisson physics (Dirichlet rows only)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        R = np.empty(self.n_dofs)
+        k = None if kappa is None else np.ascontiguousarray(kappa, dtype=np.float64)
+        if lib().orc_residual_poisson(self.h, npts, None if k is None else _ptr(k), f_kind, _ptr(u), _ptr(R)) != 0:
+            raise RuntimeError("degenerate map")
+        return R
 
     def assemble_poisson(self, npts, kappa=None, f_kind=0):
         vals = np.empty(self.nnz)

@@ -225,3 +225,30 @@ def test_enforce_constraints_rows():
                 assert abs(y[i] - want) <= 1e-15 * max(1.0, abs(want))
             else:
                 assert y[i] == x[i]
+
+
+@pytest.mark.parametrize("t,n", [(O.HEX27, 3), (O.TET10, 2), (O.HEX8, 4)])
+def test_fem_residual(t, n):
+    """The FEMSystem residual restatement: for any u, R equals the unconstrained
+    global K u - F on unconstrained rows and 0 on constrained ones; for u that
+    vanishes on constrained DoFs it equals A u - b of the constrained assembly."""
+    m, s = _system(n, t, 0x3F)
+    npts = 3 if t == O.HEX27 else 2
+    kappa = O.kappa_lognormal(m.n_elem, 3)
+    u = np.random.default_rng(2).standard_normal(s.n_dofs)
+    R = s.residual_poisson(npts, u, kappa, 1)
+    full = O.System(m, 1, 0)
+    rpf, cif = full.pattern()
+    vf, Ff = full.assemble_poisson(npts, kappa, 1)
+    Kf = sp.csr_matrix((vf, cif, rpf), shape=(s.n_dofs,) * 2)
+    g = Kf @ u - Ff
+    cons = s.constrained().astype(bool)
+    scale = np.abs(Kf).max() * np.abs(u).max()
+    assert np.abs(R[~cons] - g[~cons]).max() <= 1e-13 * scale
+    assert not R[cons].any()
+    u[cons] = 0.0
+    R0 = s.residual_poisson(npts, u, kappa, 1)
+    v, b = s.assemble_poisson(npts, kappa, 1)
+    rp, ci = s.pattern()
+    A = sp.csr_matrix((v, ci, rp), shape=(s.n_dofs,) * 2)
+    assert np.abs(R0 - (A @ u - b)).max() <= 1e-13 * scale

@@ -98,3 +98,24 @@ def test_poisson_periodic_constraints_dropin(tmp_path):
 def test_degenerate_map_raises(tmp_path):
     """An inverted element raises DegenerateMap (fe_map.C:467-505) through the C ABI."""
     out = run(["poisson_dropin", 2, 12, tmp_path / "bad", "distort"], tmp_path, expect=3)
+
+
+def test_fem_system_newton_dropin(tmp_path):
+    """FEMSystem + SteadySolver + NewtonSolver with the device physics: one
+    Newton step solves the linear problem; the solution matches the oracle."""
+    n = 4
+    out = run(["fem_system_dropin", n, 12, tmp_path / "fem", "perturb", "kappa", "sine"], tmp_path)
+    u = np.fromfile(tmp_path / "fem_u.bin")
+    m = O.Mesh(n, O.HEX27)
+    m.perturb(0x5EED0001, 0.2)
+    kappa = O.kappa_lognormal(m.n_elem, 0x5EED0002)
+    s = O.System(m)
+    v, b = s.assemble_poisson(3, kappa, 1)
+    rp, ci = s.pattern()
+    x, _, _ = O.cg_eigen(rp, ci, v, b, 1e-14, 20000)
+    assert np.abs(u - x).max() <= 1e-9 * np.abs(x).max(), out
+    toks = out.split()
+    newton, result = int(toks[toks.index("newton") + 1]), int(toks[toks.index("result") + 1])
+    assert newton == 1 and result & 4, out  # CONVERGED_RELATIVE_RESIDUAL
+    R = s.residual_poisson(3, u, kappa, 1)
+    assert np.linalg.norm(R) <= 1e-9 * np.linalg.norm(b), out

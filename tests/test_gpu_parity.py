@@ -337,3 +337,19 @@ def test_c1_full_size_against_oracle(ctx):
     # both within the error a 1e-10 residual permits of the oracle's solution
     for xs in (x64, xm):
         assert np.linalg.norm(xs - ox) / np.linalg.norm(ox) < 1e-7
+
+
+@pytest.mark.parametrize("t,n", [(L.HEX27, 4), (L.HEX8, 6), (L.TET10, 2)])
+def test_fem_residual(ctx, t, n):
+    """FEMSystem residual R(u) = sum_e C^T (K_e u_e - F_e) at a u that does not
+    satisfy the constraints (so constrained columns of K_e count)."""
+    p, m, s = make(n, t, perturb=t != L.TET10)
+    kappa = L.lognormal(p.n_elem, SEED_KAPPA)
+    u = np.random.default_rng(3).standard_normal(p.n_dofs)
+    oR = s.residual_poisson(NPTS[t], u, kappa, 1)
+    g = L.GpuProblem.from_problem(ctx, p)
+    kd = L.DeviceArray(ctx, p.n_elem, data=kappa)
+    ud = L.DeviceArray(ctx, p.n_dofs, data=u)
+    Rd = L.DeviceArray(ctx, p.n_dofs)
+    g.residual_poisson(NPTS[t], kd, 1, ud, Rd)
+    assert rel(Rd.download(), oR) < 1e-12
