@@ -15,6 +15,9 @@ element constraint, enforce_constraints_exactly) before it checks the GPU.
   solution     : symmetric rows + CG + enforce_constraints_exactly equals the
                  direct solution of the asymmetric system and solves the
                  globally reduced system
+  FEM residual : the FEMSystem residual equals the global unconstrained K u - F
+                 (scipy) on free rows, 0 on constrained rows, and A u - b when u
+                 vanishes on constrained DoFs
 """
 import numpy as np
 import pytest
