@@ -2,8 +2,11 @@
 // Poisson physics -div(kappa grad u) = f evaluated on the GPU (residual and
 // Jacobian through FEMSystem::assembly), the way FEMSystem applications
 // (adjoints_ex1/ex2, fem_system_ex*) drive assembly and solve.
-// Usage: fem_system_dropin n elem_type(10|12|8|9) out_prefix [perturb] [kappa] [sine] [rtol]
-// Writes <prefix>_u.bin; prints Newton and linear iteration counts.
+// Usage: fem_system_dropin n elem_type(10|12|8|9) out_prefix [perturb] [kappa] [sine] [rtol=R] [adjoint]
+// Writes <prefix>_u.bin; prints Newton and linear iteration counts.  `adjoint`:
+// adjoint_qoi_parameter_sensitivity of J = integral of u over [0.25,0.75]^3 for
+// every per-element kappa through the FEMSystem's Jacobian, written to
+// <prefix>_grad.bin.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -27,12 +30,13 @@ int main(int argc, char **argv) {
   const unsigned n = std::atoi(argv[1]);
   const ElemType type = (ElemType)std::atoi(argv[2]);
   const std::string prefix = argv[3];
-  bool perturb = false, kappa = false, sine = false;
+  bool perturb = false, kappa = false, sine = false, adjoint = false;
   Real rtol = 1e-9;
   for (int a = 4; a < argc; ++a) {
     perturb |= !std::strcmp(argv[a], "perturb");
     kappa |= !std::strcmp(argv[a], "kappa");
     sine |= !std::strcmp(argv[a], "sine");
+    adjoint |= !std::strcmp(argv[a], "adjoint");
     if (!std::strncmp(argv[a], "rtol=", 5)) rtol = std::atof(argv[a] + 5);
   }
   try {
@@ -64,6 +68,16 @@ int main(int argc, char **argv) {
     sys.solution->localize(u);
     std::ofstream f(prefix + "_u.bin", std::ios::binary);
     f.write((const char *)u.data(), sizeof(double) * u.size());
+    if (adjoint) {
+      // the system matrix holds the Jacobian of the last Newton step
+      HipVector<Number> dq(sys.get_dof_map().n_dofs()), z(sys.get_dof_map().n_dofs());
+      physics.qoi_box_derivative(0.25, 0.75, dq);
+      const auto r = sys.adjoint_solve(z, dq);
+      const std::vector<Real> g = physics.kappa_sensitivities(*sys.solution, z);
+      std::printf("adjoint iterations %u residual %.6e\n", r.first, r.second);
+      std::ofstream fg(prefix + "_grad.bin", std::ios::binary);
+      fg.write((const char *)g.data(), sizeof(double) * g.size());
+    }
   } catch (const std::exception &e) {
     std::fprintf(stderr, "error: %s\n", e.what());
     return 1;

@@ -104,7 +104,7 @@ def test_fem_system_newton_dropin(tmp_path):
     """FEMSystem + SteadySolver + NewtonSolver with the device physics: one
     Newton step solves the linear problem; the solution matches the oracle."""
     n = 4
-    out = run(["fem_system_dropin", n, 12, tmp_path / "fem", "perturb", "kappa", "sine"], tmp_path)
+    out = run(["fem_system_dropin", n, 12, tmp_path / "fem", "perturb", "kappa", "sine", "adjoint"], tmp_path)
     u = np.fromfile(tmp_path / "fem_u.bin")
     m = O.Mesh(n, O.HEX27)
     m.perturb(0x5EED0001, 0.2)
@@ -119,3 +119,9 @@ def test_fem_system_newton_dropin(tmp_path):
     assert newton == 1 and result & 4, out  # CONVERGED_RELATIVE_RESIDUAL
     R = s.residual_poisson(3, u, kappa, 1)
     assert np.linalg.norm(R) <= 1e-9 * np.linalg.norm(b), out
+    # adjoint_qoi_parameter_sensitivity through the FEMSystem's Jacobian
+    grad = np.fromfile(tmp_path / "fem_grad.bin")
+    dq = s.qoi_box_derivative(3)
+    zt, _, _ = O.cg_eigen(rp, ci, v, dq, 1e-14, 20000)
+    exact = s.adjoint_exact(3, kappa, x, zt)
+    assert np.abs(grad - exact).max() / np.abs(exact).max() < 1e-6, out
