@@ -22,9 +22,12 @@ constexpr int64_t LMH_PAD = 8;
 #ifndef LMH_RB_CAP
 #define LMH_RB_CAP 2048
 #endif
-// workgroups per CU of the vector/SpMV kernels of the solvers
+// workgroups per CU of the solvers' SpMV-type kernels and vector kernels
+#ifndef LMH_SPMV_GRID_PER_CU
+#define LMH_SPMV_GRID_PER_CU 16
+#endif
 #ifndef LMH_GRID_PER_CU
-#define LMH_GRID_PER_CU 8
+#define LMH_GRID_PER_CU 4
 #endif
 
 // ---- errors ---------------------------------------------------------------

@@ -599,12 +599,12 @@ __global__ __launch_bounds__(BLK) void k_cg_init(int64_t n, const double *__rest
 }
 
 // the early exits of ConjugateGradient.h:44-60; rr, bb from k_residual's partials
-__global__ __launch_bounds__(BLK) void k_cg_init_fin(const double *__restrict__ part_res,
-                                                     const double *__restrict__ part_rz, int G, double tol,
+__global__ __launch_bounds__(BLK) void k_cg_init_fin(const double *__restrict__ part_res, int Gres,
+                                                     const double *__restrict__ part_rz, int Grz, double tol,
                                                      double *__restrict__ S, double *__restrict__ x, int64_t n) {
   double t[2], u[1];
-  reduce_all<2>(part_res, G, t);
-  reduce_all<1>(part_rz, G, u);
+  reduce_all<2>(part_res, Gres, t);
+  reduce_all<1>(part_rz, Grz, u);
   const double rr = t[0], bb = t[1];
   if (bb == 0.0)  // b = 0: x = 0 (this kernel runs with a full grid for that case)
     for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) x[i] = 0.0;
@@ -736,6 +736,14 @@ static int vec_grid(lmh_context *ctx) {
   return g < ctx->n_part ? g : ctx->n_part;
 }
 
+// SpMV-type kernels (row-block streams) want more resident waves than the
+// vector kernels: their grid is sized separately; every kernel that reduces
+// partials is told the producer's grid.
+static int spmv_grid(lmh_context *ctx) {
+  const int g = ctx->cu_count * LMH_SPMV_GRID_PER_CU;
+  return g < ctx->n_part ? g : ctx->n_part;
+}
+
 // compressed FP32-operator columns, built once per pattern
 static int ensure_off16(lmh_matrix *A) {
   if (A->idx16_state != 0 || !A->fp32_index) return LMH_OK;
@@ -785,7 +793,7 @@ int lmh_spmv(lmh_matrix *A, const double *x_d, double *y_d) {
   LMH_CHECK(A && x_d && y_d, LMH_ERR_INVALID, "null argument");
   LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
   lmh_context *ctx = A->ctx;
-  const int g = vec_grid(ctx);
+  const int g = spmv_grid(ctx);
   LMH_LAUNCH(ctx, "spmv_fp64", (k_spmv<double, double, double, double>), dim3(g),
                                           dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals, x_d, y_d);
   return check_launch("k_spmv");
@@ -795,7 +803,7 @@ int lmh_spmv_fp32(lmh_matrix *A, const float *x_d, float *y_d) {
   LMH_CHECK(A && x_d && y_d, LMH_ERR_INVALID, "null argument");
   LMH_CHECK(A->have32 && A->vals32, LMH_ERR_INVALID, "matrix has no FP32 operator");
   lmh_context *ctx = A->ctx;
-  const int g = vec_grid(ctx);
+  const int g = spmv_grid(ctx);
   LMH_TRY(ensure_off16(A));
   if (use_off16(A)) {
     LMH_LAUNCH(ctx, "spmv_fp32", k_spmv16, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->off16, A->rbmeta,
@@ -811,7 +819,7 @@ int lmh_residual(lmh_matrix *A, const double *b_d, const double *x_d, double *r_
   LMH_CHECK(A && b_d && x_d && r_d, LMH_ERR_INVALID, "null argument");
   LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
   lmh_context *ctx = A->ctx;
-  const int g = vec_grid(ctx);
+  const int g = spmv_grid(ctx);
   LMH_LAUNCH(ctx, "residual_fp64", k_residual, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp,
                                           A->ci, A->vals, b_d, x_d, r_d, ctx->d_part);
   LMH_LAUNCH(ctx, "fin", k_fin_residual, dim3(1), dim3(BLK), 0, ctx->d_part, g, ctx->d_scal + 48);
@@ -866,14 +874,14 @@ int lmh_cg_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int
   double *r = A->w64, *p = r + n, *q = p + n, *dinv = q + n;
   double *S = ctx->d_scal;
   double *part1 = ctx->d_part, *part2 = ctx->d_part + 2 * ctx->n_part;  // 1 and 2 values per block
-  const int g = vec_grid(ctx);
+  const int g = vec_grid(ctx), gs = spmv_grid(ctx);
   LMH_TRY(ensure_diag(A));
   LMH_LAUNCH(ctx, "invdiag", (k_invdiag<double, double>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, n, A->diag,
              A->vals, dinv);
-  LMH_LAUNCH(ctx, "residual_fp64", k_residual, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals, b_d, x_d,
-             r, part2);
+  LMH_LAUNCH(ctx, "residual_fp64", k_residual, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals, b_d,
+             x_d, r, part2);
   LMH_LAUNCH(ctx, "cg_init", k_cg_init, dim3(g), dim3(BLK), 0, n, dinv, r, p, part1);
-  LMH_LAUNCH(ctx, "cg_init", k_cg_init_fin, dim3(g), dim3(BLK), 0, part2, part1, g, rtol, S, x_d, n);
+  LMH_LAUNCH(ctx, "cg_init", k_cg_init_fin, dim3(g), dim3(BLK), 0, part2, gs, part1, g, rtol, S, x_d, n);
   LMH_TRY(check_launch("cg setup"));
   const int batch = 4;
   Poll poll[2];
@@ -885,10 +893,10 @@ int lmh_cg_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int
     const int64_t nb = std::min<int64_t>(batch, maxits - K);
     for (int64_t i = 0; i < nb; ++i, ++K) {
       const int bank = (int)(K & 1);
-      LMH_LAUNCH(ctx, "cg_spmv_fp64", k_it_spmv<double>, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals,
-                 p, q, part1, S, bank);
-      LMH_LAUNCH(ctx, "cg_update_fp64", (k_it_update<double, double>), dim3(g), dim3(BLK), 0, n, part1, g, dinv, p, q,
-                 x_d, r, part2, S, bank);
+      LMH_LAUNCH(ctx, "cg_spmv_fp64", k_it_spmv<double>, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci,
+                 A->vals, p, q, part1, S, bank);
+      LMH_LAUNCH(ctx, "cg_update_fp64", (k_it_update<double, double>), dim3(g), dim3(BLK), 0, n, part1, gs, dinv, p,
+                 q, x_d, r, part2, S, bank);
       LMH_LAUNCH(ctx, "cg_pupdate_fp64", k_it_pupdate<double>, dim3(g), dim3(BLK), 0, n, part2, g, dinv, r, p, S, bank,
                  0);
     }
@@ -937,7 +945,7 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
   float *r32 = A->w32, *p32 = r32 + n, *q32 = p32 + n, *d32 = q32 + n, *xc = d32 + n;
   double *S = ctx->d_scal;
   double *part1 = ctx->d_part, *part2 = ctx->d_part + 2 * ctx->n_part;
-  const int g = vec_grid(ctx);
+  const int g = vec_grid(ctx), gs = spmv_grid(ctx);
   double init[40] = {0};
   init[G_THR] = rtol * rtol;
   init[G_DELTA2] = inner_rtol * inner_rtol;
@@ -947,15 +955,15 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
   LMH_LAUNCH(ctx, "invdiag", (k_invdiag<float, float>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, n, A->diag,
              A->vals32, d32);
   LMH_HIP(hipMemsetAsync(xc, 0, sizeof(float) * n, ctx->stream));
-  LMH_LAUNCH(ctx, "mx_refresh_fp64", k_refresh, dim3(g), dim3(BLK), 0, 1, A->rb, A->nrb, A->rp, A->ci, A->vals, b_d,
-             x_d, d32, r32, p32, part2, S, 0);
-  LMH_LAUNCH(ctx, "fin", k_refresh_fin, dim3(1), dim3(BLK), 0, part2, g, 1, S, 0);
+  LMH_LAUNCH(ctx, "mx_refresh_fp64", k_refresh, dim3(gs), dim3(BLK), 0, 1, A->rb, A->nrb, A->rp, A->ci, A->vals,
+             b_d, x_d, d32, r32, p32, part2, S, 0);
+  LMH_LAUNCH(ctx, "fin", k_refresh_fin, dim3(1), dim3(BLK), 0, part2, gs, 1, S, 0);
   LMH_TRY(check_launch("mixed setup"));
   auto refresh = [&](int bank) {
     LMH_LAUNCH(ctx, "mx_fold", k_fold, dim3(g), dim3(BLK), 0, n, xc, x_d);
-    LMH_LAUNCH(ctx, "mx_refresh_fp64", k_refresh, dim3(g), dim3(BLK), 0, 0, A->rb, A->nrb, A->rp, A->ci, A->vals, b_d,
-               x_d, d32, r32, p32, part2, S, bank);
-    LMH_LAUNCH(ctx, "fin", k_refresh_fin, dim3(1), dim3(BLK), 0, part2, g, 0, S, bank);
+    LMH_LAUNCH(ctx, "mx_refresh_fp64", k_refresh, dim3(gs), dim3(BLK), 0, 0, A->rb, A->nrb, A->rp, A->ci, A->vals,
+               b_d, x_d, d32, r32, p32, part2, S, bank);
+    LMH_LAUNCH(ctx, "fin", k_refresh_fin, dim3(1), dim3(BLK), 0, part2, gs, 0, S, bank);
     if (mode == 1) LMH_LAUNCH(ctx, "mx_restart", k_restart_p, dim3(g), dim3(BLK), 0, n, d32, r32, p32, S, bank);
   };
   const int batch = 4;
@@ -969,13 +977,13 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
     for (int64_t i = 0; i < nb; ++i, ++K) {
       const int bank = (int)(K & 1);
       if (c16)
-        LMH_LAUNCH(ctx, "mx_spmv_fp32", k_it_spmv16, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->off16, A->rbmeta,
-                   A->vals32, p32, q32, part1, S, bank);
+        LMH_LAUNCH(ctx, "mx_spmv_fp32", k_it_spmv16, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->off16,
+                   A->rbmeta, A->vals32, p32, q32, part1, S, bank);
       else
-        LMH_LAUNCH(ctx, "mx_spmv_fp32", k_it_spmv<float>, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci,
+        LMH_LAUNCH(ctx, "mx_spmv_fp32", k_it_spmv<float>, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci,
                    A->vals32, p32, q32, part1, S, bank);
-      LMH_LAUNCH(ctx, "mx_update_fp32", (k_it_update<float, float>), dim3(g), dim3(BLK), 0, n, part1, g, d32, p32, q32,
-                 xc, r32, part2, S, bank);
+      LMH_LAUNCH(ctx, "mx_update_fp32", (k_it_update<float, float>), dim3(g), dim3(BLK), 0, n, part1, gs, d32, p32,
+                 q32, xc, r32, part2, S, bank);
       LMH_LAUNCH(ctx, "mx_pupdate_fp32", k_it_pupdate<float>, dim3(g), dim3(BLK), 0, n, part2, g, d32, r32, p32, S,
                  bank, 1);
     }
@@ -1008,9 +1016,9 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
   bool conv = Bk[B_DONE] != 0.0;
   if (!conv) {
     // iteration limit: report the FP64 true residual of the current x
-    LMH_LAUNCH(ctx, "residual_fp64", k_residual, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals, b_d, x_d,
-               A->w64, part2);
-    LMH_LAUNCH(ctx, "fin", k_fin_residual, dim3(1), dim3(BLK), 0, part2, g, S + G_RRTRUE);
+    LMH_LAUNCH(ctx, "residual_fp64", k_residual, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals, b_d,
+               x_d, A->w64, part2);
+    LMH_LAUNCH(ctx, "fin", k_fin_residual, dim3(1), dim3(BLK), 0, part2, gs, S + G_RRTRUE);
     LMH_TRY(check_launch("mixed finish"));
   }
   LMH_HIP(hipMemcpy(gl, S + G_THR, sizeof(gl), hipMemcpyDeviceToHost));
