@@ -340,7 +340,7 @@ int lmh_context_create(int device, lmh_context **out) {
   hipDeviceProp_t prop;
   LMH_HIP(hipGetDeviceProperties(&prop, device));
   c->cu_count = prop.multiProcessorCount;
-  c->n_part = 8192;
+  c->n_part = 16384;
   LMH_HIP(hipMalloc((void **)&c->d_scal, 64 * sizeof(double)));
   LMH_HIP(hipMemset(c->d_scal, 0, 64 * sizeof(double)));
   LMH_HIP(hipMalloc((void **)&c->d_part, (size_t)c->n_part * 4 * sizeof(double)));

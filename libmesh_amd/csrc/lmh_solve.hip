@@ -740,8 +740,9 @@ static int vec_grid(lmh_context *ctx) {
 // vector kernels: their grid is sized separately; every kernel that reduces
 // partials is told the producer's grid.
 static int spmv_grid(lmh_context *ctx) {
-  const int g = ctx->cu_count * LMH_SPMV_GRID_PER_CU;
-  return g < ctx->n_part ? g : ctx->n_part;
+  // k_refresh publishes 3 values per workgroup into the second half (2 * n_part)
+  const int g = ctx->cu_count * LMH_SPMV_GRID_PER_CU, cap = 2 * ctx->n_part / 3;
+  return g < cap ? g : cap;
 }
 
 // compressed FP32-operator columns, built once per pattern
