@@ -47,6 +47,8 @@ struct BScat {
   double *rhs;
   int64_t rows;
   int dpe, w;  // element DoFs, lanes per row (16, 32 or 64)
+  int stride;  // LDS accumulator row of each sub-group: the problem's longest row, rounded up to 4
+  int pad;
 };
 
 struct BRTile {
@@ -54,6 +56,9 @@ struct BRTile {
 };
 
 constexpr int kBatchBlock = 256;
+#ifndef LMH_BATCH_SCAT_PER_CU
+#define LMH_BATCH_SCAT_PER_CU 32
+#endif
 constexpr int kBatchLds = 32 * 1024;  // bytes of LDS per element-kernel block
 constexpr int kRowsPerTile = 128;
 
@@ -211,14 +216,17 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_elasticity(const BProb *_
 constexpr int kScR = 8;      // contributions per lane per chunk
 constexpr int kScChunk = 64; // incidences per chunk at most
 
-__global__ __launch_bounds__(64) void k_batch_scatter(const BScat *__restrict__ probs,
-                                                      const BRTile *__restrict__ tiles, int ntiles, int stride) {
+#ifndef LMH_BATCH_SCAT_WAVES
+#define LMH_BATCH_SCAT_WAVES 6
+#endif
+__global__ __launch_bounds__(64, LMH_BATCH_SCAT_WAVES) void k_batch_scatter(const BScat *__restrict__ probs,
+                                                      const BRTile *__restrict__ tiles, int ntiles) {
   extern __shared__ __align__(16) unsigned char scsm[];
   const int lane = threadIdx.x;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const BRTile tl = tiles[t];
     const BScat P = probs[tl.p];
-    const int dpe = P.dpe, W = P.w, G = 64 / W;
+    const int dpe = P.dpe, W = P.w, G = 64 / W, stride = P.stride;
     const int sg = lane / W, l = lane - sg * W;
     // per sub-group: acc[stride] floats | element ids [kScChunk] | Fe [kScChunk]
     unsigned char *base = scsm + (size_t)sg * (stride * 4 + kScChunk * 12);
@@ -311,6 +319,7 @@ struct lmh_batch {
   int lpe[4] = {16, 128, 16, 32};   // lanes per element
   int ppt[4] = {4, 4, 4, 4};        // passes (groups of concurrent elements) per tile
   int max_row = 0;
+  size_t scat_lds = 0;
   int grid_elem = 0, grid_scat = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -319,10 +328,11 @@ struct lmh_batch {
 
 namespace lmh {
 
-// LDS of the scatter: 4 sub-groups at most, each an accumulator row (stride
-// floats, a multiple of 4) plus a chunk's element ids and Fe values
+// LDS of the scatter: 64/W sub-groups, each an accumulator row (stride floats,
+// a multiple of 4) plus a chunk's element ids and Fe values; the launch takes
+// the largest need over the problems
 static int scatter_stride(int max_row) { return (max_row + 3) & ~3; }
-static size_t scatter_lds(int max_row) { return 4 * ((size_t)scatter_stride(max_row) * 4 + kScChunk * 12); }
+static size_t scatter_lds(const BScat &s) { return (size_t)(64 / s.w) * ((size_t)s.stride * 4 + kScChunk * 12); }
 
 static bool config_fits(int n, int nq, int L) {
   return L >= 16 && L <= kBatchBlock && (kBatchBlock / L) * slot_bytes(n, nq) <= kBatchLds;
@@ -377,8 +387,8 @@ static int build_graph(lmh_batch *b, double E, double nu, double fz) {
     hipLaunchKernelGGL(k_batch_elasticity, dim3(b->grid_elem), dim3(kBatchBlock), kBatchLds, ctx->stream,
                        (const BProb *)b->dprob, (const BTile *)b->dtile, b->ntile, l1, l2, fz, ctx->d_flags);
   if (b->nrt > 0)
-    hipLaunchKernelGGL(k_batch_scatter, dim3(b->grid_scat), dim3(64), scatter_lds(b->max_row), ctx->stream,
-                       (const BScat *)b->dscat, (const BRTile *)b->drt, b->nrt, scatter_stride(b->max_row));
+    hipLaunchKernelGGL(k_batch_scatter, dim3(b->grid_scat), dim3(64), b->scat_lds, ctx->stream,
+                       (const BScat *)b->dscat, (const BRTile *)b->drt, b->nrt);
   const hipError_t ce = hipStreamEndCapture(ctx->stream, &b->graph);
   LMH_HIP(ce);
   LMH_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
@@ -454,7 +464,8 @@ static int batch_setup(lmh_batch *b, lmh_context *ctx, int n_prob, const lmh_dof
     lmh_matrix *A = b->mats[p];
     const int dpe = b->dms[p]->dpe;
     sc[p] = BScat{b->dms[p]->inc_ptr, b->dms[p]->inc, A->rp, A->off, A->Ke32, A->Fe, A->vals32, b->rhs[p],
-                  A->rows, dpe, dpe <= 16 ? 16 : dpe <= 32 ? 32 : 64};
+                  A->rows, dpe, dpe <= 16 ? 16 : dpe <= 32 ? 32 : 64, scatter_stride(A->max_row), 0};
+    b->scat_lds = std::max(b->scat_lds, scatter_lds(sc[p]));
     for (int64_t r = 0; r < A->rows; r += kRowsPerTile)
       rt.push_back(BRTile{p, (int32_t)r, (int32_t)std::min<int64_t>(A->rows, r + kRowsPerTile), 0});
   }
@@ -484,7 +495,14 @@ static int batch_setup(lmh_batch *b, lmh_context *ctx, int n_prob, const lmh_dof
   int occ = 0;
   LMH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_batch_elasticity, kBatchBlock, kBatchLds));
   b->grid_elem = ctx->cu_count * std::max(1, occ);
-  b->grid_scat = ctx->cu_count * 16;
+  // one-wave workgroups, as many per CU as can be resident at once (registers
+  // and LDS): the tiles are dealt round-robin, so a second round of
+  // workgroups would only wait (measured: 24 resident per CU 71 ms, 32 launched 84 ms)
+  {
+    int socc = 0;
+    LMH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&socc, k_batch_scatter, 64, b->scat_lds));
+    b->grid_scat = ctx->cu_count * std::max(1, std::min(socc, (int)LMH_BATCH_SCAT_PER_CU));
+  }
   std::vector<BTile> tiles;
   make_tiles(b, 0xFu, b->lpe, b->ppt, tiles);
   LMH_TRY(upload_tiles(b, tiles));
