@@ -668,12 +668,15 @@ static int element_poisson_impl(lmh_context *ctx, const lmh_mesh *m, const lmh_d
   return check_launch("k_poisson_generic");
 }
 
+#ifndef LMH_SCATTER_PER_CU
+#define LMH_SCATTER_PER_CU 256
+#endif
 template <typename T>
 static int scatter_impl(lmh_matrix *A, const T *Ke, const double *Fe, T *vals, double *rhs, int scatter) {
   lmh_context *ctx = A->ctx;
   const lmh_dofmap *dm = A->dm;
   if (scatter == LMH_SCATTER_GATHER) {
-    const int g = (int)std::min<int64_t>(A->rows, (int64_t)ctx->cu_count * 64);
+    const int g = (int)std::min<int64_t>(A->rows, (int64_t)ctx->cu_count * LMH_SCATTER_PER_CU);
     LMH_LAUNCH(ctx, sizeof(T) == 8 ? "scatter_gather_fp64" : "scatter_gather_fp32", k_scatter_gather<T>, dim3(g),
                dim3(64), sizeof(T) * A->max_row, dm->inc_ptr, dm->inc, dm->dpe, A->rows, A->rp, A->off, Ke, Fe, vals,
                rhs);
