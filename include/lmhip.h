@@ -219,6 +219,15 @@ int lmh_axpy(lmh_context *ctx, int64_t n, double a, const double *x_d, double *y
 int lmh_cg_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int64_t maxits, int64_t *iters,
                  double *rel_res, int *reason);
 
+/* Jacobi-preconditioned BiCGSTAB, FP64 (EigenSparseLinearSolver::solve with
+ * BICGSTAB + JACOBI_PRECOND -> Eigen::BiCGSTAB, eigen_sparse_linear_solver.C:175-205,
+ * BiCGSTAB.h:28-112), for nonsymmetric matrices such as those with asymmetric
+ * constraint rows.  Same contract as lmh_cg_solve: stops when
+ * |r|^2 <= rtol^2 |b|^2; b = 0 gives x = 0.  Eigen's restart when (r0, r)
+ * vanishes relative to |r0|^2 is reported as LMH_DIVERGED_BREAKDOWN. */
+int lmh_bicgstab_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int64_t maxits, int64_t *iters,
+                       double *rel_res, int *reason);
+
 /* Mixed precision: FP32 Jacobi-CG on the FP32 operator inside FP64 refinement
  * of the FP64 residual r = b - A x.  mode 0: reliable updates (the FP32
  * recurrence continues; r and x are refreshed in FP64 whenever the FP32

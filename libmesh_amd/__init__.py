@@ -66,6 +66,7 @@ HIP_API = {
     "lmh_enforce_constraints": (_I, [_P, _P, _I]),
     "lmh_assemble_residual_poisson": (_I, [_P, _P, _P, _I, _P, _I, _P, _P]),
     "lmh_axpy": (_I, [_P, _I64, _D, _P, _P]),
+    "lmh_bicgstab_solve": (_I, [_P, _P, _P, _D, _I64, C.POINTER(_I64), C.POINTER(_D), C.POINTER(_I)]),
     "lmh_matrix_create": (_I, [_P, _P, C.POINTER(_P)]),
     "lmh_matrix_create_csr": (_I, [_P, _I64, _P, _P, C.POINTER(_P)]),
     "lmh_matrix_destroy": (_I, [_P]),
@@ -453,6 +454,14 @@ class GpuProblem:
         reason = _I()
         check(hip().lmh_cg_solve(self.matrix, b_d.ptr, x_d.ptr, rtol, maxits, C.byref(its), C.byref(res),
                                  C.byref(reason)))
+        return its.value, res.value, reason.value
+
+    def bicgstab(self, b_d, x_d, rtol, maxits=5000):
+        its = _I64()
+        res = _D()
+        reason = _I()
+        check(hip().lmh_bicgstab_solve(self.matrix, b_d.ptr, x_d.ptr, rtol, maxits, C.byref(its), C.byref(res),
+                                       C.byref(reason)))
         return its.value, res.value, reason.value
 
     def make_fp32(self):

@@ -183,6 +183,7 @@ struct lmh_matrix {
   float *Kx32 = nullptr;
   // solver work vectors (allocated on first solve)
   double *w64 = nullptr;  // 4*rows
+  double *wbcg = nullptr;  // 9*rows (BiCGSTAB)
   float *w32 = nullptr;   // 5*rows
 };
 

@@ -857,6 +857,186 @@ int lmh_matrix_make_fp32(lmh_matrix *A) {
   return check_launch("k_round32");
 }
 
+// ---------------------------------------------------------------------------
+// BiCGSTAB with the Jacobi preconditioner, FP64 (Eigen BiCGSTAB.h:28-112 as
+// EigenSparseLinearSolver::solve runs it for BICGSTAB + JACOBI_PRECOND).  One
+// iteration is five kernels: p/y update (reduces |r|^2 and (r0, r), tests
+// convergence) | v = A y + (r0, v) | s, z | t = A z + (t, t), (t, s) | x, r
+// update + |r|^2, (r0, r).  rho, alpha, w live in the banks next to the CG
+// fields; each is written by block 0 of one kernel and read by later kernels.
+// ---------------------------------------------------------------------------
+enum { B_RHO = 8, B_ALPHA, B_W };
+enum { G_R0SQ = 40 };
+
+// r0 = r, p = v = 0; partials (r.r, r0.r)
+__global__ __launch_bounds__(BLK) void k_bcg_init(int64_t n, const double *__restrict__ r, double *__restrict__ r0,
+                                                  double *__restrict__ p, double *__restrict__ v,
+                                                  double *__restrict__ part) {
+  int64_t lo, hi;
+  block_range(n, lo, hi);
+  double acc[2] = {0, 0};
+  for (int64_t i = lo + threadIdx.x; i < hi; i += BLK) {
+    const double ri = r[i];
+    r0[i] = ri;
+    p[i] = 0.0;
+    v[i] = 0.0;
+    acc[0] += ri * ri;
+    acc[1] += ri * ri;
+  }
+  publish<2>(acc, part);
+}
+
+// |b|^2 == 0: x = 0 and done; tol2 = tol^2 |b|^2, |r0|^2; rho = alpha = w = 1
+__global__ __launch_bounds__(BLK) void k_bcg_init_fin(const double *__restrict__ part_res, int Gres, double tol,
+                                                      double *__restrict__ S, double *__restrict__ x, int64_t n) {
+  double t[2];
+  reduce_all<2>(part_res, Gres, t);
+  const double rr = t[0], bb = t[1];
+  if (bb == 0.0)
+    for (int64_t i = blockIdx.x * (int64_t)BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) x[i] = 0.0;
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  S[G_THR] = tol * tol * bb;
+  S[G_BB] = bb;
+  S[G_R0SQ] = rr;
+  double *B0 = S;
+  for (int k = 0; k < B_SIZE; ++k) B0[k] = 0.0;
+  B0[B_RHO] = B0[B_ALPHA] = B0[B_W] = 1.0;
+  B0[B_RR] = rr;
+  if (bb == 0.0) {
+    B0[B_RR] = 0.0;
+    B0[B_DONE] = 1;
+    B0[B_REASON] = LMH_CONVERGED_RTOL;
+  }
+}
+
+// top of the loop: rr = |r|^2, rho = (r0, r); stop when rr <= tol2; the
+// restart of Eigen's loop (|rho| < eps^2 |r0|^2) is reported as a breakdown;
+// else beta = (rho / rho_old)(alpha / w), p = r + beta (p - w v), y = dinv p
+__global__ __launch_bounds__(BLK) void k_bcg_p(int64_t n, const double *__restrict__ part_in, int G,
+                                               const double *__restrict__ dinv, const double *__restrict__ r,
+                                               const double *__restrict__ v, double *__restrict__ p,
+                                               double *__restrict__ y, double *__restrict__ S, int bank) {
+  const double *Sb = S + bank * B_SIZE;
+  double *Sn = S + (bank ^ 1) * B_SIZE;
+  if (Sb[B_DONE] != 0.0) {
+    if (blockIdx.x == 0 && threadIdx.x < B_SIZE) Sn[threadIdx.x] = Sb[threadIdx.x];
+    return;
+  }
+  double t[2];
+  reduce_all<2>(part_in, G, t);
+  const double rr = t[0], rho = t[1];
+  const double eps = 2.220446049250313e-16;
+  bool done = true;
+  int reason = 0;
+  if (!isfinite(rr))
+    reason = LMH_DIVERGED_NANORINF;
+  else if (rr <= S[G_THR])
+    reason = LMH_CONVERGED_RTOL;
+  else if (fabs(rho) < eps * eps * S[G_R0SQ])
+    reason = LMH_DIVERGED_BREAKDOWN;
+  else
+    done = false;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int k = 0; k < B_SIZE; ++k) Sn[k] = Sb[k];
+    Sn[B_RR] = rr;
+    Sn[B_DONE] = done ? 1.0 : 0.0;
+    Sn[B_REASON] = reason;
+    Sn[B_RHO] = done ? Sb[B_RHO] : rho;
+  }
+  if (done) return;
+  const double beta = (rho / Sb[B_RHO]) * (Sb[B_ALPHA] / Sb[B_W]), w = Sb[B_W];
+  int64_t lo, hi;
+  block_range(n, lo, hi);
+  for (int64_t i = lo + threadIdx.x; i < hi; i += BLK) {
+    const double pi = r[i] + beta * (p[i] - w * v[i]);
+    p[i] = pi;
+    y[i] = dinv[i] * pi;
+  }
+}
+
+// v = A y; partials (r0, v)
+__global__ __launch_bounds__(BLK) void k_bcg_spmv_v(const int32_t *__restrict__ rb, int64_t nrb,
+                                                    const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                    const double *__restrict__ vals, const double *__restrict__ y,
+                                                    const double *__restrict__ r0, double *__restrict__ v,
+                                                    double *__restrict__ part, const double *__restrict__ S,
+                                                    int nbank) {
+  if (S[nbank * B_SIZE + B_DONE] != 0.0) return;
+  double acc[1] = {0};
+  rowblock_spmv<double, double, double>(rb, nrb, rp, ci, vals, y, [&](int32_t row, double sv) {
+    v[row] = sv;
+    acc[0] += r0[row] * sv;
+  });
+  publish<1>(acc, part);
+}
+
+// alpha = rho / (r0, v); s = r - alpha v; z = dinv s
+__global__ __launch_bounds__(BLK) void k_bcg_s(int64_t n, const double *__restrict__ part_in, int G,
+                                               const double *__restrict__ dinv, const double *__restrict__ r,
+                                               const double *__restrict__ v, double *__restrict__ s,
+                                               double *__restrict__ z, double *__restrict__ S, int nbank) {
+  double *Sn = S + nbank * B_SIZE;
+  if (Sn[B_DONE] != 0.0) return;
+  double t[1];
+  reduce_all<1>(part_in, G, t);
+  const double alpha = Sn[B_RHO] / t[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) Sn[B_ALPHA] = alpha;
+  int64_t lo, hi;
+  block_range(n, lo, hi);
+  for (int64_t i = lo + threadIdx.x; i < hi; i += BLK) {
+    const double si = r[i] - alpha * v[i];
+    s[i] = si;
+    z[i] = dinv[i] * si;
+  }
+}
+
+// t = A z; partials (t, t), (t, s)
+__global__ __launch_bounds__(BLK) void k_bcg_spmv_t(const int32_t *__restrict__ rb, int64_t nrb,
+                                                    const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                    const double *__restrict__ vals, const double *__restrict__ z,
+                                                    const double *__restrict__ s, double *__restrict__ t,
+                                                    double *__restrict__ part, const double *__restrict__ S,
+                                                    int nbank) {
+  if (S[nbank * B_SIZE + B_DONE] != 0.0) return;
+  double acc[2] = {0, 0};
+  rowblock_spmv<double, double, double>(rb, nrb, rp, ci, vals, z, [&](int32_t row, double tv) {
+    t[row] = tv;
+    acc[0] += tv * tv;
+    acc[1] += tv * s[row];
+  });
+  publish<2>(acc, part);
+}
+
+// w = (t, s) / (t, t) (0 when (t, t) = 0); x += alpha y + w z; r = s - w t;
+// partials |r|^2, (r0, r)
+__global__ __launch_bounds__(BLK) void k_bcg_update(int64_t n, const double *__restrict__ part_in, int G,
+                                                    const double *__restrict__ y, const double *__restrict__ z,
+                                                    const double *__restrict__ s, const double *__restrict__ t,
+                                                    const double *__restrict__ r0, double *__restrict__ x,
+                                                    double *__restrict__ r, double *__restrict__ part,
+                                                    double *__restrict__ S, int nbank) {
+  double *Sn = S + nbank * B_SIZE;
+  if (Sn[B_DONE] != 0.0) return;
+  double tt[2];
+  reduce_all<2>(part_in, G, tt);
+  const double w = tt[0] > 0.0 ? tt[1] / tt[0] : 0.0, alpha = Sn[B_ALPHA];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    Sn[B_W] = w;
+    Sn[B_ITS] = Sn[B_ITS] + 1.0;
+  }
+  int64_t lo, hi;
+  block_range(n, lo, hi);
+  double acc[2] = {0, 0};
+  for (int64_t i = lo + threadIdx.x; i < hi; i += BLK) {
+    x[i] += alpha * y[i] + w * z[i];
+    const double ri = s[i] - w * t[i];
+    r[i] = ri;
+    acc[0] += ri * ri;
+    acc[1] += r0[i] * ri;
+  }
+  publish<2>(acc, part);
+}
+
 // Poll slots in pinned memory: the two banks are copied after every batch.
 struct Poll {
   hipEvent_t ev = nullptr;
@@ -923,6 +1103,77 @@ int lmh_cg_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int
   const double *Bk = h + (K & 1) * B_SIZE;
   const bool conv = Bk[B_DONE] != 0.0;
   if (iters) *iters = conv ? (int64_t)Bk[B_ITS] : maxits;
+  if (rel_res) *rel_res = gbb > 0 ? std::sqrt(Bk[B_RR] / gbb) : 0.0;
+  if (reason) *reason = conv ? (int)Bk[B_REASON] : LMH_DIVERGED_ITS;
+  return LMH_OK;
+}
+
+int lmh_bicgstab_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int64_t maxits, int64_t *iters,
+                       double *rel_res, int *reason) {
+  LMH_CHECK(A && b_d && x_d, LMH_ERR_INVALID, "null argument");
+  LMH_CHECK(rtol >= 0 && maxits >= 0, LMH_ERR_INVALID, "bad tolerance or iteration limit");
+  LMH_CHECK(A->rows == A->cols, LMH_ERR_INVALID, "BiCGSTAB needs a square matrix");
+  LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
+  lmh_context *ctx = A->ctx;
+  const int64_t n = A->rows;
+  if (!A->wbcg) LMH_HIP(hipMalloc((void **)&A->wbcg, sizeof(double) * 9 * std::max<int64_t>(1, n)));
+  double *r = A->wbcg, *r0 = r + n, *p = r0 + n, *v = p + n, *y = v + n, *z = y + n, *sv = z + n, *tv = sv + n,
+         *dinv = tv + n;
+  double *S = ctx->d_scal;
+  double *part1 = ctx->d_part, *part2 = ctx->d_part + 2 * ctx->n_part;
+  const int g = vec_grid(ctx), gs = spmv_grid(ctx);
+  LMH_TRY(ensure_diag(A));
+  LMH_LAUNCH(ctx, "invdiag", (k_invdiag<double, double>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, n, A->diag,
+             A->vals, dinv);
+  LMH_LAUNCH(ctx, "residual_fp64", k_residual, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals, b_d,
+             x_d, r, part2);
+  LMH_LAUNCH(ctx, "bcg_init", k_bcg_init, dim3(g), dim3(BLK), 0, n, (const double *)r, r0, p, v, part1);
+  LMH_LAUNCH(ctx, "bcg_init", k_bcg_init_fin, dim3(g), dim3(BLK), 0, part2, gs, rtol, S, x_d, n);
+  LMH_TRY(check_launch("bicgstab setup"));
+  const int batch = 4;
+  Poll poll[2];
+  double *snap = ctx->h_pinned;
+  int64_t K = 0;
+  int slot = 0;
+  bool pending = false, done = false;
+  auto iteration = [&](int bank) {
+    const int nb = bank ^ 1;
+    LMH_LAUNCH(ctx, "bcg_p", k_bcg_p, dim3(g), dim3(BLK), 0, n, part1, g, dinv, r, v, p, y, S, bank);
+    LMH_LAUNCH(ctx, "bcg_spmv_fp64", k_bcg_spmv_v, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals, y,
+               r0, v, part1, S, nb);
+    LMH_LAUNCH(ctx, "bcg_s", k_bcg_s, dim3(g), dim3(BLK), 0, n, part1, gs, dinv, r, v, sv, z, S, nb);
+    LMH_LAUNCH(ctx, "bcg_spmv_fp64", k_bcg_spmv_t, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals, z,
+               sv, tv, part2, S, nb);
+    LMH_LAUNCH(ctx, "bcg_update", k_bcg_update, dim3(g), dim3(BLK), 0, n, part2, gs, y, z, sv, tv, r0, x_d, r, part1,
+               S, nb);
+  };
+  while (K < maxits && !done) {
+    const int64_t nb = std::min<int64_t>(batch, maxits - K);
+    for (int64_t i = 0; i < nb; ++i, ++K) iteration((int)(K & 1));
+    LMH_HIP(hipMemcpyAsync(snap + 32 * slot, S, sizeof(double) * 2 * B_SIZE, hipMemcpyDeviceToHost, ctx->stream));
+    if (!poll[slot].ev) poll[slot].ev = get_event(ctx);
+    LMH_HIP(hipEventRecord(poll[slot].ev, ctx->stream));
+    poll[slot].end = K;
+    if (pending) {
+      const int o = slot ^ 1;
+      LMH_HIP(hipEventSynchronize(poll[o].ev));
+      done = snap[32 * o + (poll[o].end & 1) * B_SIZE + B_DONE] != 0.0;
+    }
+    pending = true;
+    slot ^= 1;
+  }
+  // the loop test after the last iteration (Eigen: while (|r|^2 > tol2 && i < maxIters))
+  LMH_LAUNCH(ctx, "bcg_p", k_bcg_p, dim3(g), dim3(BLK), 0, n, part1, g, dinv, r, v, p, y, S, (int)(K & 1));
+  LMH_HIP(hipStreamSynchronize(ctx->stream));
+  LMH_TRY(check_launch("bicgstab iterations"));
+  for (auto &pl : poll)
+    if (pl.ev) ctx->event_pool.push_back(pl.ev);
+  double h[2 * B_SIZE], gbb = 0;
+  LMH_HIP(hipMemcpy(h, S, sizeof(h), hipMemcpyDeviceToHost));
+  LMH_HIP(hipMemcpy(&gbb, S + G_BB, sizeof(double), hipMemcpyDeviceToHost));
+  const double *Bk = h + ((K + 1) & 1) * B_SIZE;
+  const bool conv = Bk[B_DONE] != 0.0;
+  if (iters) *iters = (int64_t)Bk[B_ITS];
   if (rel_res) *rel_res = gbb > 0 ? std::sqrt(Bk[B_RR] / gbb) : 0.0;
   if (reason) *reason = conv ? (int)Bk[B_REASON] : LMH_DIVERGED_ITS;
   return LMH_OK;

@@ -367,6 +367,7 @@ int lmh_matrix_destroy(lmh_matrix *A) {
   (void)hipFree(A->Fx);
   (void)hipFree(A->Kx32);
   (void)hipFree(A->w64);
+  (void)hipFree(A->wbcg);
   (void)hipFree(A->w32);
   delete A;
   return LMH_OK;

@@ -4,12 +4,14 @@
 //   sys.linear_solver = make_unique<HipLinearSolver>             (ImplicitSystem::linear_solver)
 //   sys.attach_assemble_object(HipAssembler(...))                (System::attach_assemble_object)
 // Usage: poisson_dropin n elem_type(10|12|8|9) out_prefix [perturb] [kappa] [sine] [mixed] [adjoint] [distort]
-//                      [periodic]
+//                      [periodic] [bicgstab]
 // Writes <prefix>_u.bin (solution) and, with `adjoint`, <prefix>_grad.bin.
 // `periodic`: Dirichlet on the y and z faces only, u(x=1) = u(x=0) through
 // constraint rows from a System::Constraint object (plus one heterogeneous
 // row u = 0.3 on an x=0 face DoF), symmetric constraint rows, and
-// enforce_constraints_exactly after the solve.
+// enforce_constraints_exactly after the solve.  `periodic bicgstab`: the
+// default asymmetric constraint rows instead, solved by Jacobi-BiCGSTAB
+// (LinearSolver::set_solver_type(BICGSTAB)).
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -83,7 +85,7 @@ int main(int argc, char **argv) {
   const ElemType type = (ElemType)std::atoi(argv[2]);
   const std::string prefix = argv[3];
   bool perturb = false, kappa = false, sine = false, mixed = false, adjoint = false, distort = false,
-       periodic = false;
+       periodic = false, bicgstab = false;
   for (int a = 4; a < argc; ++a) {
     perturb |= !std::strcmp(argv[a], "perturb");
     kappa |= !std::strcmp(argv[a], "kappa");
@@ -92,6 +94,7 @@ int main(int argc, char **argv) {
     adjoint |= !std::strcmp(argv[a], "adjoint");
     distort |= !std::strcmp(argv[a], "distort");
     periodic |= !std::strcmp(argv[a], "periodic");
+    bicgstab |= !std::strcmp(argv[a], "bicgstab");
   }
   try {
     Mesh mesh;
@@ -113,7 +116,7 @@ int main(int argc, char **argv) {
     if (periodic) {
       sys.get_dof_map().add_dirichlet_boundary(DirichletBoundary({0, 1, 3, 5}, {0}));
       sys.get_dof_map().set_error_on_constraint_loop(true);
-      sys.get_dof_map().set_asymmetric_constraint_rows(false);
+      sys.get_dof_map().set_asymmetric_constraint_rows(bicgstab);
       sys.attach_constraint_object(periodic_rows);
     } else {
       sys.get_dof_map().add_dirichlet_boundary(DirichletBoundary({0, 1, 2, 3, 4, 5}, {0}));
@@ -121,6 +124,7 @@ int main(int argc, char **argv) {
     sys.add_matrix("System Matrix", std::make_unique<HipCsrMatrix<Number>>());
     auto solver = std::make_unique<HipLinearSolver<Number>>();
     solver->set_mixed_precision(mixed);
+    if (bicgstab) solver->set_solver_type(BICGSTAB);
     sys.linear_solver = std::move(solver);
     es.linear_solver_tolerance = 1e-10;
     es.linear_solver_maximum_iterations = 20000;
@@ -133,7 +137,7 @@ int main(int argc, char **argv) {
     if (kappa) assembler.set_coefficient(MeshTools::lognormal_coefficients(mesh.n_elem(), 0x5EED0002));
 
     sys.solve();
-    if (periodic) sys.get_dof_map().enforce_constraints_exactly(sys);
+    if (periodic && !bicgstab) sys.get_dof_map().enforce_constraints_exactly(sys);
     auto *lin = static_cast<HipLinearSolver<Number> *>(sys.linear_solver.get());
     std::printf("dofs %u iterations %u residual %.6e reason %d\n", sys.get_dof_map().n_dofs(),
                 sys.n_linear_iterations(), sys.final_linear_residual(), (int)lin->get_converged_reason());

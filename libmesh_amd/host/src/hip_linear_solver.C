@@ -1,11 +1,12 @@
-// hip_linear_solver.C -- LinearSolver backend: Jacobi-CG on the GPU.
+// hip_linear_solver.C -- LinearSolver backend: Jacobi-CG (FP64 or mixed) and
+// Jacobi-BiCGSTAB (FP64) on the GPU.
 #include "libmesh/hip_linear_solver.h"
 
 namespace libMesh {
 
 template <typename T>
 void HipLinearSolver<T>::init(const char *) {
-  if (this->_solver_type != CG) libmesh_not_implemented();
+  if (this->_solver_type != CG && this->_solver_type != BICGSTAB) libmesh_not_implemented();
   if (this->_preconditioner_type != JACOBI_PRECOND) libmesh_not_implemented();
   this->_is_initialized = true;
 }
@@ -40,7 +41,13 @@ std::pair<unsigned int, Real> HipLinearSolver<T>::solve(SparseMatrix<T> &matrix_
   int64_t its = 0;
   double res = 0;
   int reason = 0;
-  if (_mixed) {
+  if (this->_solver_type == BICGSTAB) {
+    // EigenSparseLinearSolver::solve, case BICGSTAB + JACOBI_PRECOND (eigen_sparse_linear_solver.C:175-205)
+    if (_mixed) libmesh_not_implemented();
+    hip_check(lmh_bicgstab_solve(A->device_matrix(), (const double *)b->device_data(), (double *)x->device_data(),
+                                 rtol, maxits, &its, &res, &reason),
+              "HipLinearSolver::solve (BiCGSTAB)");
+  } else if (_mixed) {
     hip_check(lmh_matrix_make_fp32(A->device_matrix()), "HipLinearSolver (FP32 operator)");
     double stats[3];
     hip_check(lmh_mixed_solve(A->device_matrix(), (const double *)b->device_data(), (double *)x->device_data(), rtol,
@@ -61,6 +68,9 @@ template <typename T>
 std::pair<unsigned int, Real> HipLinearSolver<T>::adjoint_solve(SparseMatrix<T> &matrix, NumericVector<T> &solution,
                                                                 NumericVector<T> &rhs, std::optional<double> tol,
                                                                 std::optional<unsigned int> m_its) {
+  // the system matrices of this path are symmetric for CG; BiCGSTAB serves the
+  // nonsymmetric ones, whose transpose solve is not provided
+  if (this->_solver_type == BICGSTAB) libmesh_not_implemented();
   return solve(matrix, solution, rhs, tol, m_its);
 }
 

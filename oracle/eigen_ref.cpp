@@ -38,3 +38,21 @@ extern "C" void ref_eigen_spmv(int64_t n, const int32_t *rp, const int32_t *ci, 
   Eigen::Map<Eigen::VectorXd> yy(y, n);
   yy.noalias() = A * xx;
 }
+
+// BICGSTAB + JACOBI_PRECOND of EigenSparseLinearSolver::solve
+// (eigen_sparse_linear_solver.C:175-205): BiCGSTAB<EigenSM, DiagonalPreconditioner>.
+extern "C" int ref_eigen_bicgstab(int64_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *b,
+                                  double *x, double tol, int64_t maxits, int64_t *iters, double *err) {
+  Eigen::Map<const EigenSM> view((Eigen::Index)n, (Eigen::Index)n, (Eigen::Index)rp[n], rp, ci, v);
+  EigenSM A(view);
+  Eigen::Map<const Eigen::VectorXd> bb(b, n);
+  Eigen::VectorXd xx = Eigen::Map<const Eigen::VectorXd>(x, n);
+  Eigen::BiCGSTAB<EigenSM, Eigen::DiagonalPreconditioner<double>> solver(A);
+  solver.setMaxIterations((Eigen::Index)maxits);
+  solver.setTolerance(tol);
+  xx = solver.solveWithGuess(bb, xx);
+  std::memcpy(x, xx.data(), sizeof(double) * n);
+  *iters = solver.iterations();
+  *err = solver.error();
+  return (int)solver.info();
+}

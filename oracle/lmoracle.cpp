@@ -1355,6 +1355,63 @@ void orc_cg_eigen(int64_t n, const int32_t *rp, const int32_t *ci, const double 
   *rel_err = std::sqrt(residualNorm2 / rhsNorm2);
 }
 
+// Eigen::BiCGSTAB with DiagonalPreconditioner as EigenSparseLinearSolver::solve
+// runs it for BICGSTAB + JACOBI_PRECOND (eigen_sparse_linear_solver.C:175-205;
+// Eigen 3.3.9 IterativeLinearSolvers/BiCGSTAB.h:28-112): restart with r0 = r when
+// |r0.r| < eps^2 |r0|^2 (the first restart also resets the iteration count);
+// stops when |r|^2 <= tol^2 |b|^2.  Returns the iteration count; rel_err =
+// |r|/|b|.
+void orc_bicgstab_eigen(int64_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *b, double *x,
+                        double tol, int64_t maxits, int64_t *iters, double *rel_err) {
+  std::vector<double> invd(n, 1.0);
+  for (i64 r = 0; r < n; ++r)
+    for (i64 k = rp[r]; k < rp[r + 1]; ++k)
+      if (ci[k] == r && v[k] != 0.0) invd[r] = 1.0 / v[k];
+  std::vector<double> bb(b, b + n), xx(x, x + n), r(n), r0(n), vv(n, 0.), p(n, 0.), y(n), z(n), s(n), t(n), tmp(n);
+  orc_spmv(n, rp, ci, v, x, tmp.data());
+  for (i64 i = 0; i < n; ++i) r[i] = bb[i] - tmp[i];
+  r0 = r;
+  double r0_sqnorm = dotv(r0, r0);
+  const double rhs_sqnorm = dotv(bb, bb);
+  if (rhs_sqnorm == 0) {
+    std::fill(x, x + n, 0.);
+    *iters = 0;
+    *rel_err = 0;
+    return;
+  }
+  double rho = 1, alpha = 1, w = 1;
+  const double tol2 = tol * tol * rhs_sqnorm;
+  const double eps = 2.220446049250313e-16, eps2 = eps * eps;
+  i64 i = 0, restarts = 0;
+  while (dotv(r, r) > tol2 && i < maxits) {
+    const double rho_old = rho;
+    rho = dotv(r0, r);
+    if (std::fabs(rho) < eps2 * r0_sqnorm) {
+      orc_spmv(n, rp, ci, v, xx.data(), tmp.data());
+      for (i64 k = 0; k < n; ++k) r[k] = bb[k] - tmp[k];
+      r0 = r;
+      rho = r0_sqnorm = dotv(r, r);
+      if (restarts++ == 0) i = 0;
+    }
+    const double beta = (rho / rho_old) * (alpha / w);
+    for (i64 k = 0; k < n; ++k) p[k] = r[k] + beta * (p[k] - w * vv[k]);
+    for (i64 k = 0; k < n; ++k) y[k] = invd[k] * p[k];
+    orc_spmv(n, rp, ci, v, y.data(), vv.data());
+    alpha = rho / dotv(r0, vv);
+    for (i64 k = 0; k < n; ++k) s[k] = r[k] - alpha * vv[k];
+    for (i64 k = 0; k < n; ++k) z[k] = invd[k] * s[k];
+    orc_spmv(n, rp, ci, v, z.data(), t.data());
+    const double tmpn = dotv(t, t);
+    w = tmpn > 0 ? dotv(t, s) / tmpn : 0.0;
+    for (i64 k = 0; k < n; ++k) xx[k] += alpha * y[k] + w * z[k];
+    for (i64 k = 0; k < n; ++k) r[k] = s[k] - w * t[k];
+    ++i;
+  }
+  std::memcpy(x, xx.data(), n * 8);
+  *iters = i;
+  *rel_err = std::sqrt(dotv(r, r) / rhs_sqnorm);
+}
+
 // Laspack CGIter with JacobiPrecond, Omega = 1 (itersolv.C:377-476, precond.C:26-46):
 // RTCResult stops when |r| < eps |b| (rtc.C:48-72), checked before each iteration.
 // LaspackLinearSolver::solve zeroes the initial guess (laspack_linear_solver.C:130).

@@ -70,6 +70,7 @@ def lib():
             "orc_element_elasticity": (C.c_int, [_P, C.c_int, _D, _D, _D, _I64, _I64, C.c_int, _P, _P]),
             "orc_cg_eigen": (None, [_I64, _P, _P, _P, _P, _P, _D, _I64, _P, _P]),
             "orc_cg_laspack": (None, [_I64, _P, _P, _P, _P, _P, _D, _I64, _P, _P]),
+            "orc_bicgstab_eigen": (None, [_I64, _P, _P, _P, _P, _P, _D, _I64, _P, _P]),
             "orc_spmv": (None, [_I64, _P, _P, _P, _P, _P]),
             "orc_qoi_box_derivative": (C.c_int, [_P, C.c_int, _D, _D, _P]),
             "orc_qoi_box": (_D, [_P, C.c_int, _D, _D, _P]),
@@ -98,6 +99,8 @@ def ref():
         R = C.CDLL(path)
         R.ref_eigen_cg.restype = C.c_int
         R.ref_eigen_cg.argtypes = [_I64, _P, _P, _P, _P, _P, _D, _I64, _P, _P]
+        R.ref_eigen_bicgstab.restype = C.c_int
+        R.ref_eigen_bicgstab.argtypes = [_I64, _P, _P, _P, _P, _P, _D, _I64, _P, _P]
         R.ref_eigen_spmv.restype = None
         R.ref_eigen_spmv.argtypes = [_I64, _P, _P, _P, _P, _P]
         _ref = R
@@ -323,6 +326,17 @@ def cg_eigen(rp, ci, vals, b, tol=1e-12, maxits=5000, x0=None):
 
 def cg_laspack(rp, ci, vals, b, tol=1e-12, maxits=5000):
     return _cg(lib().orc_cg_laspack, rp, ci, vals, b, tol, maxits)
+
+
+def bicgstab_eigen(rp, ci, vals, b, tol=1e-12, maxits=5000, x0=None):
+    return _cg(lib().orc_bicgstab_eigen, rp, ci, vals, b, tol, maxits, x0)
+
+
+def ref_bicgstab_eigen(rp, ci, vals, b, tol=1e-12, maxits=5000, x0=None):
+    R = ref()
+    if R is None:
+        return None
+    return _cg(R.ref_eigen_bicgstab, rp, ci, vals, b, tol, maxits, x0)
 
 
 def ref_cg_eigen(rp, ci, vals, b, tol=1e-12, maxits=5000, x0=None):

@@ -255,3 +255,25 @@ def test_fem_residual(t, n):
     rp, ci = s.pattern()
     A = sp.csr_matrix((v, ci, rp), shape=(s.n_dofs,) * 2)
     assert np.abs(R0 - (A @ u - b)).max() <= 1e-13 * scale
+
+
+@pytest.mark.parametrize("name,t,n,mask", CASES)
+def test_bicgstab_restatement(name, t, n, mask):
+    """The BiCGSTAB restatement against the reference's own vendored Eigen
+    3.3.9 BiCGSTAB (oracle/_ref) and a direct solve, on the asymmetric
+    constraint-row systems that CG cannot solve."""
+    m, s = _build(name, t, n, mask, asym=True)
+    npts = 3 if t == O.HEX27 else 2
+    kappa = O.kappa_lognormal(m.n_elem, 3)
+    vals, rhs = s.assemble_poisson(npts, kappa, 1)
+    rp, ci = s.pattern()
+    x, its, err = O.bicgstab_eigen(rp, ci, vals, rhs, 1e-12, 5000)
+    assert err <= 1e-12
+    xd = spla.spsolve(sp.csr_matrix((vals, ci, rp), shape=(s.n_dofs,) * 2).tocsc(), rhs)
+    assert np.abs(x - xd).max() <= 1e-9 * np.abs(xd).max()
+    ref = O.ref_bicgstab_eigen(rp, ci, vals, rhs, 1e-12, 5000)
+    if ref is None:
+        pytest.skip("oracle/_ref not built (needs the reference tree)")
+    xr, itr, errr = ref
+    assert abs(its - itr) <= 2 and errr <= 1e-12
+    assert np.abs(x - xr).max() <= 1e-9 * np.abs(xr).max()

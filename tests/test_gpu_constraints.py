@@ -130,3 +130,46 @@ def test_constraint_errors(ctx):
     assert g.constraint_rows() == {0: ({1: 0.5}, 0.25)}
     g.set_constraints([])
     assert g.constraint_rows() == {} and g.n_general_elements() == 0
+
+
+@pytest.mark.parametrize("name,t,n,mask", CASES)
+def test_bicgstab_asymmetric_rows(ctx, name, t, n, mask):
+    """The reference's default asymmetric constraint rows give a nonsymmetric
+    system: Jacobi-BiCGSTAB on the GPU against the oracle's Eigen BiCGSTAB
+    restatement (iterations +-2, solution 1e-9)."""
+    p, m, s, g = build(ctx, name, t, n, mask, symmetric=False)
+    kappa = L.lognormal(p.n_elem, SEED_KAPPA)
+    ov, orhs = s.assemble_poisson(NPTS[t], kappa, 1)
+    rp, ci = s.pattern()
+    ox, oits, oerr = O.bicgstab_eigen(rp, ci, ov, orhs, 1e-10, 5000)
+    kd = L.DeviceArray(ctx, p.n_elem, data=kappa)
+    b = L.DeviceArray(ctx, p.n_dofs)
+    g.assemble_poisson(NPTS[t], kd, 1, b, L.FP64, L.SCATTER_GATHER)
+    x = L.DeviceArray(ctx, p.n_dofs)
+    x.zero()
+    its, res, reason = g.bicgstab(b, x, 1e-10)
+    assert reason == L.CONVERGED_RTOL and abs(its - oits) <= 2 and res <= 1e-10
+    assert rel(x.download(), ox) < 1e-8
+    # the asymmetric rows already hold the constraints
+    xs = x.download()
+    for d, (row, v) in s.constraint_rows().items():
+        assert abs(xs[d] - (v + sum(c * xs[j] for j, c in row.items()))) <= 1e-8 * max(1.0, np.abs(xs).max())
+
+
+def test_bicgstab_limits(ctx):
+    """Zero rhs: x = 0 and 0 iterations; an iteration limit reports DIVERGED_ITS
+    with the residual of the last iterate."""
+    p = L.Problem(3, L.HEX27)
+    g = L.GpuProblem.from_problem(ctx, p)
+    kd = L.DeviceArray(ctx, p.n_elem, data=np.ones(p.n_elem))
+    b = L.DeviceArray(ctx, p.n_dofs)
+    g.assemble_poisson(3, kd, 0, b)
+    x = L.DeviceArray(ctx, p.n_dofs)
+    x.zero()
+    its, res, reason = g.bicgstab(b, x, 1e-14, 3)
+    assert its == 3 and reason == L.DIVERGED_ITS and res > 1e-14
+    z = L.DeviceArray(ctx, p.n_dofs)
+    z.zero()
+    x.upload(np.ones(p.n_dofs))
+    its, res, reason = g.bicgstab(z, x, 1e-10)
+    assert its == 0 and reason == L.CONVERGED_RTOL and not x.download().any()
