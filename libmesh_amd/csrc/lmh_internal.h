@@ -161,12 +161,15 @@ struct lmh_matrix {
   double *vals = nullptr;
   float *vals32 = nullptr;
   bool have32 = false;
-  // FP32 operator column compression: per row block {b0, base1, s0, -} and a
-  // 16-bit offset per entry, col = off + (off <= s0 ? b0 : base1)
-  int fp32_index = 1;         // 1: use the compressed columns when they fit
-  int idx16_state = 0;        // 0 not built, 1 built, -1 does not fit
-  uint16_t *off16 = nullptr;  // nnz
-  int4 *rbmeta = nullptr;     // nrb
+  // FP32 operator column compression (x windows): each row block lists its
+  // distinct columns once, sorted, as 16-bit offsets woff16[w0 .. w0+nu),
+  // col = off + (off <= s0 ? b0 : base1); every entry stores the 16-bit index
+  // of its column in that list.  rbmeta[bk] = {b0, base1, s0 | nu << 16, w0}.
+  int fp32_index = 1;          // 1: use the compressed columns when they fit
+  int idx16_state = 0;         // 0 not built, 1 built, -1 does not fit
+  uint16_t *widx16 = nullptr;  // nnz: window index of each entry's column
+  uint16_t *woff16 = nullptr;  // sum of nu over the row blocks
+  int4 *rbmeta = nullptr;      // nrb
   uint16_t *off = nullptr;  // n_elem*dpe*dpe slot offsets within rows
   int32_t *diag = nullptr;  // rows: position of the diagonal entry, -1 if absent
   int ncolors = 0;

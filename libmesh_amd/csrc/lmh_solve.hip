@@ -192,43 +192,60 @@ __device__ inline void rowblock_spmv_c(const int32_t *__restrict__ rb, int64_t n
   }
 }
 
-// FP32 values with compressed columns: the 16-bit column offsets of a block
-// are read as aligned 8-entry chunks (one 16-byte load per thread instead of
-// eight 2-byte loads) and passed through LDS so that gathers and products keep
-// the consecutive-entries-per-lane layout of rowblock_spmv_c (same sums, same
-// order).  Row blocks hold at most RB_CAP - 8 entries, so a block's chunks
-// (its start rounded down to 8 entries) fit one per thread.
+// FP32 values with x windows.  A row block of ~2000 entries references only a
+// few hundred distinct columns (about 300 for Q2 hexahedra, under 520 for every
+// Lagrange type here), so the block's x values are gathered once, in sorted
+// column order, into an LDS window, and each entry reads its x there through a
+// 16-bit window index.  The sorted gather touches each x cache line about once
+// per block instead of once per entry that uses it.  Values are read as 16-byte
+// chunks and the window indices as 8-entry chunks, both from the block's start
+// rounded down to 8 entries; the indices pass through LDS, so every product
+// lands at its entry's slot and the rows are reduced exactly as in
+// rowblock_spmv_c (same products, same summation order, bit-identical sums).
+// Row blocks hold at most RB_CAP - 8 entries, so a block's chunks fit one
+// 8-entry index chunk and two 4-entry value chunks per thread.
 typedef uint32_t vu4 __attribute__((ext_vector_type(4)));
+typedef float vf4 __attribute__((ext_vector_type(4)));
+constexpr int UCAP = 1024;  // distinct columns of one row block (x window)
 template <typename Epi>
 __device__ inline void rowblock_spmv16(const int32_t *__restrict__ rb, int64_t nrb, const int32_t *__restrict__ rp,
-                                       const uint16_t *__restrict__ off, const int4 *__restrict__ meta,
-                                       const float *__restrict__ v, const float *__restrict__ x, Epi &&epi) {
+                                       const uint16_t *__restrict__ widx, const uint16_t *__restrict__ woff,
+                                       const int4 *__restrict__ meta, const float *__restrict__ v,
+                                       const float *__restrict__ x, Epi &&epi) {
   __shared__ float prod[RB_CAP];
+  __shared__ float xw[UCAP];
   __shared__ int32_t rowbuf[2][BLK];  // double-buffered: written before the first barrier
-  __shared__ vu4 colbuf[BLK];  // 8 offsets per thread
+  __shared__ vu4 colbuf[BLK];         // 8 window indices per thread
   const uint16_t *cb = (const uint16_t *)colbuf;
-  constexpr int PER = RB_CAP / BLK;
-  static_assert(PER == 8, "one 8-entry offset chunk per thread");
+  constexpr int PER = RB_CAP / BLK, UPER = UCAP / BLK;
+  static_assert(PER == 8, "one 8-entry index chunk per thread");
   int64_t lo, hi;
   block_range(nrb, lo, hi);
   const int tid = threadIdx.x;
   int32_t r0 = 0, r1 = 0, k0 = 0, cnt = 0, rpn = 0;
   int4 m{};
-  float vv[PER];
+  vf4 vv[PER / 4];  // values of entries a0 + 4 tid + 4 BLK j + e, a0 = block start rounded down to 8
+  uint32_t uo[UPER];
   vu4 ov;
+  // branch-free loads: out-of-range slots re-read a valid address and are dropped
   auto fetch = [&](int64_t bk) {
+    m = meta[bk];
     r0 = rb[bk];
     r1 = rb[bk + 1];
     k0 = rp[r0];
     cnt = rp[r1] - k0;
-    m = meta[bk];
     rpn = rp[r0 + (tid <= r1 - r0 ? tid : 0)];
-    const int32_t a0 = k0 & ~7, span = (k0 - a0) + cnt;
-    ov = __builtin_nontemporal_load((const vu4 *)(off + a0 + 8 * (8 * tid < span ? tid : 0)));
+    const int32_t a0 = k0 & ~7, span = (k0 - a0) + cnt, nu = m.z >> 16;
+    ov = __builtin_nontemporal_load((const vu4 *)(widx + a0 + 8 * (8 * tid < span ? tid : 0)));
 #pragma unroll
-    for (int j = 0; j < PER; ++j) {
-      const int32_t t = tid + j * BLK;
-      vv[j] = __builtin_nontemporal_load(&v[k0 + (t < cnt ? t : 0)]);
+    for (int j = 0; j < PER / 4; ++j) {
+      const int32_t c = 4 * tid + 4 * BLK * j;  // chunks starting past the block re-read chunk 0
+      vv[j] = __builtin_nontemporal_load((const vf4 *)(v + a0 + (c < span ? c : 0)));
+    }
+#pragma unroll
+    for (int j = 0; j < UPER; ++j) {
+      const int32_t u = tid + j * BLK;
+      uo[j] = __builtin_nontemporal_load(&woff[m.w + (u < nu ? u : 0)]);
     }
   };
   if (lo < hi) fetch(lo);
@@ -238,24 +255,31 @@ __device__ inline void rowblock_spmv16(const int32_t *__restrict__ rb, int64_t n
     buf ^= 1;
     const int32_t R0 = r0, R1 = r1, K0 = k0, CNT = cnt, RPN = rpn, lead = K0 & 7;
     const int4 M = m;
+    const int32_t NU = M.z >> 16, S0 = M.z & 0xffff;
     const bool staged = R1 - R0 < BLK;
+    float xu[UPER];
+    vf4 cv[PER / 4];
+#pragma unroll
+    for (int j = 0; j < UPER; ++j) {
+      const int32_t u = tid + j * BLK, o = (int32_t)uo[j];
+      xu[j] = x[u < NU ? o + (o <= S0 ? M.x : M.y) : 0];
+    }
     colbuf[tid] = ov;
+#pragma unroll
+    for (int j = 0; j < PER / 4; ++j) cv[j] = vv[j];
+    if (bk + 1 < hi) fetch(bk + 1);  // the next block's loads fly during this one
     if (staged && tid <= R1 - R0) rows[tid] = RPN - K0;
-    __syncthreads();  // offsets and row pointers visible; the previous block's rows are reduced
-    float xx[PER], cv[PER];
 #pragma unroll
-    for (int j = 0; j < PER; ++j) {
-      const int32_t t = tid + j * BLK;
-      const int32_t o = t < CNT ? cb[t + lead] : 0;
-      xx[j] = x[t < CNT ? o + (o <= M.z ? M.x : M.y) : 0];
-      cv[j] = vv[j];
-    }
-    if (bk + 1 < hi) fetch(bk + 1);
+    for (int j = 0; j < UPER; ++j)
+      if (tid + j * BLK < NU) xw[tid + j * BLK] = xu[j];
+    __syncthreads();  // window, indices and row pointers visible; the previous block's rows are reduced
 #pragma unroll
-    for (int j = 0; j < PER; ++j) {
-      const int32_t t = tid + j * BLK;
-      if (t < CNT) prod[t] = cv[j] * xx[j];
-    }
+    for (int j = 0; j < PER / 4; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int32_t pos = 4 * tid + 4 * BLK * j + e, t = pos - lead;
+        if (t >= 0 && t < CNT) prod[t] = cv[j][e] * xw[cb[pos]];
+      }
     __syncthreads();
     const int g = tid >> 3, sub = tid & 7;
     for (int32_t r = R0 + g; r < R1; r += BLK / 8) {
@@ -407,13 +431,14 @@ __global__ __launch_bounds__(BLK) void k_it_spmv(const int32_t *__restrict__ rb,
 // q = A32 p on the compressed-column FP32 operator; partials (p, q).  Held to
 // 64 VGPRs (8 waves per SIMD): the pipelined loop needs every resident wave.
 __global__ __launch_bounds__(BLK, 8) void k_it_spmv16(const int32_t *__restrict__ rb, int64_t nrb,
-                                                   const int32_t *__restrict__ rp, const uint16_t *__restrict__ off,
-                                                   const int4 *__restrict__ meta, const float *__restrict__ v,
-                                                   const float *__restrict__ p, float *__restrict__ q,
-                                                   double *__restrict__ part, const double *__restrict__ S, int bank) {
+                                                   const int32_t *__restrict__ rp, const uint16_t *__restrict__ widx,
+                                                   const uint16_t *__restrict__ woff, const int4 *__restrict__ meta,
+                                                   const float *__restrict__ v, const float *__restrict__ p,
+                                                   float *__restrict__ q, double *__restrict__ part,
+                                                   const double *__restrict__ S, int bank) {
   if (S[bank * B_SIZE + B_DONE] != 0.0) return;
   double acc[1] = {0};
-  rowblock_spmv16(rb, nrb, rp, off, meta, v, p, [&](int32_t r, float s) {
+  rowblock_spmv16(rb, nrb, rp, widx, woff, meta, v, p, [&](int32_t r, float s) {
     q[r] = s;
     acc[0] += (double)p[r] * (double)s;
   });
@@ -421,36 +446,46 @@ __global__ __launch_bounds__(BLK, 8) void k_it_spmv16(const int32_t *__restrict_
 }
 
 __global__ __launch_bounds__(BLK, 8) void k_spmv16(const int32_t *__restrict__ rb, int64_t nrb,
-                                                const int32_t *__restrict__ rp, const uint16_t *__restrict__ off,
-                                                const int4 *__restrict__ meta, const float *__restrict__ v,
-                                                const float *__restrict__ x, float *__restrict__ y) {
-  rowblock_spmv16(rb, nrb, rp, off, meta, v, x, [&](int32_t r, float s) { y[r] = s; });
+                                                const int32_t *__restrict__ rp, const uint16_t *__restrict__ widx,
+                                                const uint16_t *__restrict__ woff, const int4 *__restrict__ meta,
+                                                const float *__restrict__ v, const float *__restrict__ x,
+                                                float *__restrict__ y) {
+  rowblock_spmv16(rb, nrb, rp, widx, woff, meta, v, x, [&](int32_t r, float s) { y[r] = s; });
 }
 
-// Build the compressed columns of one row block per workgroup: sort the
-// block's columns in LDS, split at the largest gap into two clusters of spans
-// s0, s1; they fit when s0 + s1 < 65535 (or the whole block spans < 65536).
-__global__ __launch_bounds__(BLK) void k_build_off16(const int32_t *__restrict__ rb, int64_t nrb,
-                                                     const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                                                     uint16_t *__restrict__ off, int4 *__restrict__ meta,
-                                                     int *__restrict__ bad) {
-  __shared__ int32_t keys[RB_CAP];
+// Build the x windows, one row block per workgroup: sort the block's
+// (column, position) pairs in LDS; its distinct columns in ascending order form
+// the window.  They are split at the largest gap into two clusters of spans s0,
+// s1, which fit 16-bit offsets when s0 + s1 < 65535 (or the whole block spans
+// < 65536).  COUNT: store each block's window size in wsz.  Otherwise wsz holds
+// their exclusive prefix sums (the window starts): write the window offsets,
+// each entry's window index and the block's meta.  A block that does not fit
+// (more than UCAP distinct columns, too wide a span, too many entries) raises
+// *bad.
+template <bool COUNT>
+__global__ __launch_bounds__(BLK) void k_build_windows(const int32_t *__restrict__ rb, int64_t nrb,
+                                                       const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                       int32_t *__restrict__ wsz, uint16_t *__restrict__ widx,
+                                                       uint16_t *__restrict__ woff, int4 *__restrict__ meta,
+                                                       int *__restrict__ bad) {
+  __shared__ int64_t keys[RB_CAP];
   __shared__ int32_t gv[BLK], gi[BLK];
+  constexpr int PER = RB_CAP / BLK;
   const int tid = threadIdx.x;
   for (int64_t bk = blockIdx.x; bk < nrb; bk += gridDim.x) {
     const int32_t k0 = rp[rb[bk]], cnt = rp[rb[bk + 1]] - k0;
-    if ((k0 & 7) + cnt > RB_CAP) {  // does not fit one 8-entry offset chunk per thread
+    if ((k0 & 7) + cnt > RB_CAP) {  // does not fit one 8-entry index chunk per thread
       if (tid == 0) atomicOr(bad, 1);
       continue;
     }
-    for (int t = tid; t < RB_CAP; t += BLK) keys[t] = t < cnt ? ci[k0 + t] : INT_MAX;
+    for (int t = tid; t < RB_CAP; t += BLK) keys[t] = t < cnt ? ((int64_t)ci[k0 + t] << 12) | t : INT64_MAX;
     __syncthreads();
     for (int k = 2; k <= RB_CAP; k <<= 1)
       for (int j = k >> 1; j > 0; j >>= 1) {
         for (int i = tid; i < RB_CAP; i += BLK) {
           const int pj = i ^ j;
           if (pj > i) {
-            const int32_t a = keys[i], c = keys[pj];
+            const int64_t a = keys[i], c = keys[pj];
             if ((a > c) == ((i & k) == 0)) {
               keys[i] = c;
               keys[pj] = a;
@@ -459,13 +494,18 @@ __global__ __launch_bounds__(BLK) void k_build_off16(const int32_t *__restrict__
         }
         __syncthreads();
       }
-    // largest gap between consecutive sorted columns
-    int32_t bestg = -1, besti = 0;
-    for (int i = tid; i + 1 < cnt; i += BLK) {
-      const int32_t g = keys[i + 1] - keys[i];
-      if (g > bestg) {
-        bestg = g;
-        besti = i;
+    auto col = [&](int i) { return (int32_t)(keys[i] >> 12); };
+    // thread tid owns the sorted entries PER*tid .. PER*tid + PER - 1
+    int32_t nfl = 0, bestg = -1, besti = 0;
+    for (int e = 0; e < PER; ++e) {
+      const int i = PER * tid + e;
+      if (i < cnt && (i == 0 || col(i) != col(i - 1))) ++nfl;
+      if (i + 1 < cnt) {
+        const int32_t g = col(i + 1) - col(i);
+        if (g > bestg) {
+          bestg = g;
+          besti = i;
+        }
       }
     }
     gv[tid] = bestg;
@@ -478,25 +518,48 @@ __global__ __launch_bounds__(BLK) void k_build_off16(const int32_t *__restrict__
       }
       __syncthreads();
     }
-    const int32_t cmin = cnt > 0 ? keys[0] : 0, cmax = cnt > 0 ? keys[cnt - 1] : 0;
+    const int32_t cmin = cnt > 0 ? col(0) : 0, cmax = cnt > 0 ? col(cnt - 1) : 0;
     int32_t b0 = cmin, b1 = cmin, s0 = 65535, split = INT_MAX;
     bool ok = true;
     if (cmax - cmin > 65535) {
-      const int32_t glo = keys[gi[0]], ghi = keys[gi[0] + 1];
+      const int32_t glo = col(gi[0]), ghi = col(gi[0] + 1);
       s0 = glo - cmin;
       const int32_t s1 = cmax - ghi;
       ok = s0 + s1 + 1 <= 65535;
       b1 = ghi - (s0 + 1);
       split = glo;
     }
+    __syncthreads();  // gi[0] read by every thread before gv is reused
+    // window size and each thread's first window index: scan of the counts
+    gv[tid] = nfl;
+    __syncthreads();
+    for (int d = 1; d < BLK; d <<= 1) {
+      const int32_t add = tid >= d ? gv[tid - d] : 0;
+      __syncthreads();
+      gv[tid] += add;
+      __syncthreads();
+    }
+    const int32_t nu = gv[BLK - 1], before = gv[tid] - nfl;
+    ok = ok && nu <= UCAP;
     if (!ok) {
       if (tid == 0) atomicOr(bad, 1);
+    } else if (COUNT) {
+      if (tid == 0) wsz[bk] = nu;
     } else {
-      for (int t = tid; t < cnt; t += BLK) {
-        const int32_t c = ci[k0 + t];
-        off[k0 + t] = (uint16_t)(c <= split ? c - b0 : c - b1);
+      const int32_t w0 = wsz[bk];
+      int32_t u = before - 1;  // window index of the last distinct column before this thread's entries
+      for (int e = 0; e < PER; ++e) {
+        const int i = PER * tid + e;
+        if (i < cnt) {
+          const int32_t c = col(i);
+          if (i == 0 || c != col(i - 1)) {
+            ++u;
+            woff[w0 + u] = (uint16_t)(c <= split ? c - b0 : c - b1);
+          }
+          widx[k0 + (int32_t)(keys[i] & 4095)] = (uint16_t)u;
+        }
       }
-      if (tid == 0) meta[bk] = make_int4(b0, b1, s0, 0);
+      if (tid == 0) meta[bk] = make_int4(b0, b1, s0 | (nu << 16), w0);
     }
     __syncthreads();
   }
@@ -745,24 +808,53 @@ static int spmv_grid(lmh_context *ctx) {
   return g < cap ? g : cap;
 }
 
-// compressed FP32-operator columns, built once per pattern
+// x windows of the FP32 operator, built once per pattern: window sizes, their
+// prefix sums on the host, then offsets and indices
 static int ensure_off16(lmh_matrix *A) {
   if (A->idx16_state != 0 || !A->fp32_index) return LMH_OK;
   lmh_context *ctx = A->ctx;
-  LMH_HIP(hipMalloc((void **)&A->off16, sizeof(uint16_t) * (A->nnz + LMH_PAD)));
-  LMH_HIP(hipMalloc((void **)&A->rbmeta, sizeof(int4) * (A->nrb ? A->nrb : 1)));
+  const int64_t nrb = A->nrb;
+  const dim3 grid(grid_for(nrb, 1, 16384));
+  struct Tmp {
+    int32_t *p = nullptr;
+    ~Tmp() { (void)hipFree(p); }
+  } wsz;
+  LMH_HIP(hipMalloc((void **)&wsz.p, sizeof(int32_t) * (nrb + 1)));
   LMH_HIP(hipMemsetAsync(ctx->d_flags, 0, sizeof(int), ctx->stream));
-  LMH_LAUNCH(ctx, "build_off16", k_build_off16, dim3(grid_for(A->nrb, 1, 16384)), dim3(BLK), 0, A->rb, A->nrb, A->rp,
-             A->ci, A->off16, A->rbmeta, ctx->d_flags);
-  LMH_TRY(check_launch("k_build_off16"));
+  LMH_LAUNCH(ctx, "build_windows", k_build_windows<true>, grid, dim3(BLK), 0, A->rb, nrb, A->rp, A->ci, wsz.p,
+             nullptr, nullptr, nullptr, ctx->d_flags);
+  LMH_TRY(check_launch("k_build_windows"));
+  std::vector<int32_t> w0(nrb + 1);
   int bad = 0;
+  LMH_HIP(hipMemcpyAsync(w0.data(), wsz.p, sizeof(int32_t) * nrb, hipMemcpyDeviceToHost, ctx->stream));
   LMH_HIP(hipMemcpyAsync(&bad, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   LMH_HIP(hipStreamSynchronize(ctx->stream));
-  A->idx16_state = bad ? -1 : 1;
   if (bad) {
-    LMH_HIP(hipFree(A->off16));
+    A->idx16_state = -1;
+    return LMH_OK;
+  }
+  int64_t total = 0;
+  for (int64_t b = 0; b < nrb; ++b) {
+    const int32_t s = w0[b];
+    w0[b] = (int32_t)total;
+    total += s;
+  }
+  w0[nrb] = (int32_t)total;  // at most nnz < 2^31
+  LMH_HIP(hipMemcpyAsync(wsz.p, w0.data(), sizeof(int32_t) * (nrb + 1), hipMemcpyHostToDevice, ctx->stream));
+  LMH_HIP(hipMalloc((void **)&A->widx16, sizeof(uint16_t) * (A->nnz + LMH_PAD)));
+  LMH_HIP(hipMalloc((void **)&A->woff16, sizeof(uint16_t) * (total + LMH_PAD)));
+  LMH_HIP(hipMalloc((void **)&A->rbmeta, sizeof(int4) * (nrb ? nrb : 1)));
+  LMH_LAUNCH(ctx, "build_windows", k_build_windows<false>, grid, dim3(BLK), 0, A->rb, nrb, A->rp, A->ci, wsz.p,
+             A->widx16, A->woff16, A->rbmeta, ctx->d_flags);
+  LMH_TRY(check_launch("k_build_windows"));
+  LMH_HIP(hipMemcpyAsync(&bad, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  LMH_HIP(hipStreamSynchronize(ctx->stream));
+  A->idx16_state = bad ? -1 : 1;  // the second pass sees the blocks the first one accepted
+  if (bad) {
+    LMH_HIP(hipFree(A->widx16));
+    LMH_HIP(hipFree(A->woff16));
     LMH_HIP(hipFree(A->rbmeta));
-    A->off16 = nullptr;
+    A->widx16 = A->woff16 = nullptr;
     A->rbmeta = nullptr;
   }
   return LMH_OK;
@@ -807,8 +899,8 @@ int lmh_spmv_fp32(lmh_matrix *A, const float *x_d, float *y_d) {
   const int g = spmv_grid(ctx);
   LMH_TRY(ensure_off16(A));
   if (use_off16(A)) {
-    LMH_LAUNCH(ctx, "spmv_fp32", k_spmv16, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->off16, A->rbmeta,
-               A->vals32, x_d, y_d);
+    LMH_LAUNCH(ctx, "spmv_fp32", k_spmv16, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->widx16, A->woff16,
+               A->rbmeta, A->vals32, x_d, y_d);
     return check_launch("k_spmv16");
   }
   LMH_LAUNCH(ctx, "spmv_fp32", (k_spmv<float, float, float, float>), dim3(g),
@@ -1229,8 +1321,8 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
     for (int64_t i = 0; i < nb; ++i, ++K) {
       const int bank = (int)(K & 1);
       if (c16)
-        LMH_LAUNCH(ctx, "mx_spmv_fp32", k_it_spmv16, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->off16,
-                   A->rbmeta, A->vals32, p32, q32, part1, S, bank);
+        LMH_LAUNCH(ctx, "mx_spmv_fp32", k_it_spmv16, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->widx16,
+                   A->woff16, A->rbmeta, A->vals32, p32, q32, part1, S, bank);
       else
         LMH_LAUNCH(ctx, "mx_spmv_fp32", k_it_spmv<float>, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci,
                    A->vals32, p32, q32, part1, S, bank);

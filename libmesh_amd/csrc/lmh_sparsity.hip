@@ -352,7 +352,8 @@ int lmh_matrix_destroy(lmh_matrix *A) {
   (void)hipFree(A->rp);
   (void)hipFree(A->ci);
   (void)hipFree(A->rb);
-  (void)hipFree(A->off16);
+  (void)hipFree(A->widx16);
+  (void)hipFree(A->woff16);
   (void)hipFree(A->rbmeta);
   (void)hipFree(A->vals);
   (void)hipFree(A->vals32);

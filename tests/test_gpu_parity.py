@@ -226,11 +226,18 @@ def test_adjoint_gradient(ctx):
     assert np.abs(gr[elems] - fd).max() / np.abs(fd).max() < 1e-6
 
 
-@pytest.mark.parametrize("t,n", [(L.HEX27, 6), (L.HEX8, 10), (L.TET10, 2)])
+@pytest.mark.parametrize("t,n", [(L.HEX27, 6), (L.HEX8, 10), (L.TET10, 2), (L.TET4, 12), (L.HEX27, 64)])
 def test_fp32_operator_column_compression(ctx, t, n):
-    """The compressed-column FP32 operator decodes every column exactly: its SpMV
-    equals the int32-column SpMV bit for bit (same values, same summation order)."""
-    p, m, s = make(n, t)
+    """The x-window FP32 operator decodes every column exactly: its SpMV equals
+    the int32-column SpMV bit for bit (same values, same summation order).  At
+    64^3 Q2 the row blocks span more than 65536 columns, so the windows use the
+    two-cluster offsets; the oracle's SpMV is compared below that size."""
+    if n == 64:
+        p = L.Problem(n, t)
+        p.perturb(SEED_PERTURB, 0.2)
+        s = None
+    else:
+        p, m, s = make(n, t)
     kappa = L.lognormal(p.n_elem, SEED_KAPPA)
     g = L.GpuProblem.from_problem(ctx, p)
     kd = L.DeviceArray(ctx, p.n_elem, data=kappa)
@@ -247,6 +254,8 @@ def test_fp32_operator_column_compression(ctx, t, n):
     L.check(L.hip().lmh_matrix_set_fp32_compression(g.matrix, 0))
     L.check(L.hip().lmh_spmv_fp32(g.matrix, x.ptr, y2.ptr))
     assert np.array_equal(y1.download(), y2.download())
+    if s is None:
+        return
     rp, ci = s.pattern()
     v32 = g.values_fp32().astype(np.float64)
     yo = O.spmv(rp, ci, v32, xh.astype(np.float64))
