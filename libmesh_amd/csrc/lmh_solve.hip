@@ -12,6 +12,7 @@
 #include <climits>
 #include <cmath>
 #include <functional>
+#include <type_traits>
 
 #include "lmh_internal.h"
 
@@ -206,19 +207,20 @@ __device__ inline void rowblock_spmv_c(const int32_t *__restrict__ rb, int64_t n
 // 8-entry index chunk and two 4-entry value chunks per thread.
 typedef uint32_t vu4 __attribute__((ext_vector_type(4)));
 typedef float vf4 __attribute__((ext_vector_type(4)));
-constexpr int UCAP = 1024;  // distinct columns of one row block (x window)
-template <typename Epi>
-__device__ inline void rowblock_spmv16(const int32_t *__restrict__ rb, int64_t nrb, const int32_t *__restrict__ rp,
-                                       const uint16_t *__restrict__ widx, const uint16_t *__restrict__ woff,
-                                       const int4 *__restrict__ meta, const float *__restrict__ v,
-                                       const float *__restrict__ x, Epi &&epi) {
-  __shared__ float prod[RB_CAP];
+constexpr int UCAP = 768;  // distinct columns of one row block (x window)
+template <bool DOT>
+__device__ inline double rowblock_spmv16(const int32_t *__restrict__ rb, int64_t nrb, const int32_t *__restrict__ rp,
+                                         const uint16_t *__restrict__ widx, const uint16_t *__restrict__ woff,
+                                         const int4 *__restrict__ meta, const float *__restrict__ v,
+                                         const float *__restrict__ x, float *__restrict__ y) {
+  __shared__ float prod[RB_CAP + 1];  // + one sink slot for lanes whose chunk slot is outside the block
   __shared__ float xw[UCAP];
   __shared__ int32_t rowbuf[2][BLK];  // double-buffered: written before the first barrier
+  __shared__ float xrow[2][BLK];      // x of the block's rows, for the (x, Ax) partials
   __shared__ vu4 colbuf[BLK];         // 8 window indices per thread
   const uint16_t *cb = (const uint16_t *)colbuf;
   constexpr int PER = RB_CAP / BLK, UPER = UCAP / BLK;
-  static_assert(PER == 8, "one 8-entry index chunk per thread");
+  static_assert(PER == 8 && UCAP % BLK == 0, "one 8-entry index chunk per thread");
   int64_t lo, hi;
   block_range(nrb, lo, hi);
   const int tid = threadIdx.x;
@@ -227,6 +229,7 @@ __device__ inline void rowblock_spmv16(const int32_t *__restrict__ rb, int64_t n
   vf4 vv[PER / 4];  // values of entries a0 + 4 tid + 4 BLK j + e, a0 = block start rounded down to 8
   uint32_t uo[UPER];
   vu4 ov;
+  double acc = 0;
   // branch-free loads: out-of-range slots re-read a valid address and are dropped
   auto fetch = [&](int64_t bk) {
     m = meta[bk];
@@ -248,51 +251,90 @@ __device__ inline void rowblock_spmv16(const int32_t *__restrict__ rb, int64_t n
       uo[j] = __builtin_nontemporal_load(&woff[m.w + (u < nu ? u : 0)]);
     }
   };
-  if (lo < hi) fetch(lo);
+  // One row block.  The loads of the next block are issued (unconditionally
+  // in the main loop, so that the compiler's wait counts leave them in
+  // flight) after this block's window gathers and row values, and are waited
+  // for only at the top of the next block.
   int buf = 0;
-  for (int64_t bk = lo; bk < hi; ++bk) {
+  auto body = [&](int64_t bk, auto more) {
     int32_t *rows = rowbuf[buf];
+    float *xr = xrow[buf];
     buf ^= 1;
     const int32_t R0 = r0, R1 = r1, K0 = k0, CNT = cnt, RPN = rpn, lead = K0 & 7;
     const int4 M = m;
     const int32_t NU = M.z >> 16, S0 = M.z & 0xffff;
     const bool staged = R1 - R0 < BLK;
     float xu[UPER];
-    vf4 cv[PER / 4];
 #pragma unroll
     for (int j = 0; j < UPER; ++j) {
       const int32_t u = tid + j * BLK, o = (int32_t)uo[j];
       xu[j] = x[u < NU ? o + (o <= S0 ? M.x : M.y) : 0];
     }
+    float xo = 0;
+    if (DOT) xo = x[R0 + (tid < R1 - R0 ? tid : 0)];
     colbuf[tid] = ov;
+    if (staged && tid <= R1 - R0) rows[tid] = RPN - K0;
+    vf4 cv[PER / 4];
 #pragma unroll
     for (int j = 0; j < PER / 4; ++j) cv[j] = vv[j];
-    if (bk + 1 < hi) fetch(bk + 1);  // the next block's loads fly during this one
-    if (staged && tid <= R1 - R0) rows[tid] = RPN - K0;
+    if constexpr (decltype(more)::value) fetch(bk + 1);
 #pragma unroll
     for (int j = 0; j < UPER; ++j)
       if (tid + j * BLK < NU) xw[tid + j * BLK] = xu[j];
-    __syncthreads();  // window, indices and row pointers visible; the previous block's rows are reduced
+    if (DOT) xr[tid] = xo;
+    __syncthreads();  // window, indices and row data visible; the previous block's rows are reduced
+    float xs[PER];
 #pragma unroll
     for (int j = 0; j < PER / 4; ++j)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int32_t pos = 4 * tid + 4 * BLK * j + e, t = pos - lead;
-        if (t >= 0 && t < CNT) prod[t] = cv[j][e] * xw[cb[pos]];
+        const int32_t wi = cb[pos];
+        xs[4 * j + e] = xw[t >= 0 && t < CNT ? wi : 0];
+      }
+#pragma unroll
+    for (int j = 0; j < PER / 4; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int32_t pos = 4 * tid + 4 * BLK * j + e, t = pos - lead;
+        prod[t >= 0 && t < CNT ? t : RB_CAP] = cv[j][e] * xs[4 * j + e];
       }
     __syncthreads();
     const int g = tid >> 3, sub = tid & 7;
-    for (int32_t r = R0 + g; r < R1; r += BLK / 8) {
-      const int32_t a = staged ? rows[r - R0] : rp[r] - K0;
-      const int32_t e = staged ? rows[r - R0 + 1] : rp[r + 1] - K0;
-      float s = 0;
-      for (int32_t t = a + sub; t < e; t += 8) s += prod[t];
-      s += __shfl_xor(s, 4, 8);
-      s += __shfl_xor(s, 2, 8);
-      s += __shfl_xor(s, 1, 8);
-      if (sub == 0) epi(r, s);
+    if (staged) {
+      for (int32_t r = R0 + g; r < R1; r += BLK / 8) {
+        const int32_t a = rows[r - R0], e = rows[r - R0 + 1];
+        float s = 0;
+        for (int32_t t = a + sub; t < e; t += 8) s += prod[t];
+        s += __shfl_xor(s, 4, 8);
+        s += __shfl_xor(s, 2, 8);
+        s += __shfl_xor(s, 1, 8);
+        if (sub == 0) {
+          y[r] = s;
+          if (DOT) acc += (double)xr[r - R0] * (double)s;
+        }
+      }
+    } else {
+      for (int32_t r = R0 + g; r < R1; r += BLK / 8) {
+        const int32_t a = rp[r] - K0, e = rp[r + 1] - K0;
+        float s = 0;
+        for (int32_t t = a + sub; t < e; t += 8) s += prod[t];
+        s += __shfl_xor(s, 4, 8);
+        s += __shfl_xor(s, 2, 8);
+        s += __shfl_xor(s, 1, 8);
+        if (sub == 0) {
+          y[r] = s;
+          if (DOT) acc += (double)x[r] * (double)s;
+        }
+      }
     }
+  };
+  if (lo < hi) {
+    fetch(lo);
+    for (int64_t bk = lo; bk + 1 < hi; ++bk) body(bk, std::true_type{});
+    body(hi - 1, std::false_type{});
   }
+  return acc;
 }
 
 template <typename VT, typename XT, typename AT, typename Epi>
@@ -437,11 +479,7 @@ __global__ __launch_bounds__(BLK, 8) void k_it_spmv16(const int32_t *__restrict_
                                                    float *__restrict__ q, double *__restrict__ part,
                                                    const double *__restrict__ S, int bank) {
   if (S[bank * B_SIZE + B_DONE] != 0.0) return;
-  double acc[1] = {0};
-  rowblock_spmv16(rb, nrb, rp, widx, woff, meta, v, p, [&](int32_t r, float s) {
-    q[r] = s;
-    acc[0] += (double)p[r] * (double)s;
-  });
+  double acc[1] = {rowblock_spmv16<true>(rb, nrb, rp, widx, woff, meta, v, p, q)};
   publish<1>(acc, part);
 }
 
@@ -450,7 +488,7 @@ __global__ __launch_bounds__(BLK, 8) void k_spmv16(const int32_t *__restrict__ r
                                                 const uint16_t *__restrict__ woff, const int4 *__restrict__ meta,
                                                 const float *__restrict__ v, const float *__restrict__ x,
                                                 float *__restrict__ y) {
-  rowblock_spmv16(rb, nrb, rp, widx, woff, meta, v, x, [&](int32_t r, float s) { y[r] = s; });
+  rowblock_spmv16<false>(rb, nrb, rp, widx, woff, meta, v, x, y);
 }
 
 // Build the x windows, one row block per workgroup: sort the block's
