@@ -46,6 +46,13 @@ int lmesh_matrix_read(const char *filename, int64_t *m, int64_t *n, int64_t *nnz
  * (sparse_matrix.C:398-491), 17 significant digits, exact zeros skipped. */
 int lmesh_matrix_write_matlab(const char *filename, int64_t m, int64_t n, const int32_t *row_ptr,
                               const int32_t *col_idx, const double *vals);
+/* EquationSystems files (EquationSystems::read / write, equation_systems_io.C:
+ * 59-560): read in_name (XDR when the name holds ".xdr", ASCII otherwise) with
+ * its data and write it to out_name (encoding chosen the same way), values and
+ * header unchanged.  The reader and writer that EquationSystems::read / write
+ * use, without a mesh or systems.  Returns 0, or -1 with the message in
+ * lmesh_last_error. */
+int lmesh_es_file_convert(const char *in_name, const char *out_name);
 
 int64_t lmesh_n_nodes(const lmesh_problem *p);
 int64_t lmesh_n_elem(const lmesh_problem *p);

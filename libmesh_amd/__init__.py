@@ -119,6 +119,7 @@ HOST_API = {
     "lmesh_batch_recipe": (None, [C.c_uint64, _I, _I, _I, _P, _P]),
     "lmesh_matrix_read": (_I, [C.c_char_p, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), _P, _P, _P]),
     "lmesh_matrix_write_matlab": (_I, [C.c_char_p, _I64, _I64, _P, _P, _P]),
+    "lmesh_es_file_convert": (_I, [C.c_char_p, C.c_char_p]),
     "lmesh_n_nodes": (_I64, [_P]),
     "lmesh_n_elem": (_I64, [_P]),
     "lmesh_n_dofs": (_I64, [_P]),
@@ -240,6 +241,13 @@ def write_matrix_matlab(filename, m, n, row_ptr, col_idx, vals):
     if L.lmesh_matrix_write_matlab(str(filename).encode(), m, n, _ptr(rp), _ptr(ci), _ptr(v)) != 0:
         raise LmhError(L.lmesh_last_error().decode())
 
+
+
+def convert_equation_systems_file(in_name, out_name):
+    """Read an EquationSystems file and write it again (ASCII or XDR chosen by
+    each name: ".xdr" selects XDR), header and values unchanged."""
+    if host().lmesh_es_file_convert(str(in_name).encode(), str(out_name).encode()) != 0:
+        raise LmhError(host().lmesh_last_error().decode())
 
 def batch_recipe(seed, count, nmin=8, nmax=48):
     """(edge, element type) per problem of a seeded batch of cube problems."""

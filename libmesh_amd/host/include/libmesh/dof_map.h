@@ -42,6 +42,7 @@ class DofMap {
 
   unsigned add_variable(const std::string &name, const FEType &type);
   unsigned n_variables() const { return (unsigned)_names.size(); }
+  const std::string &variable_name(unsigned v) const { return _names.at(v); }
   const FEType &variable_type(unsigned v) const { return _types.at(v); }
   void add_dirichlet_boundary(const DirichletBoundary &d) { _dirichlet.push_back(d); }
 

@@ -14,6 +14,7 @@
 #include "libmesh/dof_map.h"
 #include "libmesh/linear_solver.h"
 #include "libmesh/replicated_mesh.h"
+#include "libmesh/xdr_cxx.h"
 
 namespace libMesh {
 
@@ -32,6 +33,13 @@ class System {
   virtual ~System() = default;
 
   const std::string &name() const { return _name; }
+  unsigned number() const { return _number; }
+  // System::system_type (include/systems/system.h:510 and overrides)
+  virtual std::string system_type() const { return "Basic"; }
+  // DoFs in the order of EquationSystems files: nodes by id, each node's
+  // variables in order (System::write_serialized_blocked_dof_objects,
+  // src/systems/system_io.C:1626-1910)
+  std::vector<dof_id_type> node_major_dofs() const;
   EquationSystems &get_equation_systems() { return _es; }
   DofMap &get_dof_map() { return _dof_map; }
   const DofMap &get_dof_map() const { return _dof_map; }
@@ -82,6 +90,7 @@ class System {
 class ImplicitSystem : public System {
  public:
   using System::System;
+  std::string system_type() const override { return "Implicit"; }
   SparseMatrix<Number> &get_system_matrix() { return get_matrix("System Matrix"); }
   // ImplicitSystem::assemble zeroes matrix and rhs first (implicit_system.C:94-108)
   void assemble() override;
@@ -98,6 +107,7 @@ class ImplicitSystem : public System {
 class LinearImplicitSystem : public ImplicitSystem {
  public:
   using ImplicitSystem::ImplicitSystem;
+  std::string system_type() const override { return "LinearImplicit"; }
   // LinearImplicitSystem::solve (src/systems/linear_implicit_system.C:122-161)
   void solve() override;
   unsigned n_linear_iterations() const { return _n_linear_iterations; }
@@ -129,6 +139,27 @@ class EquationSystems {
   void init() {
     for (auto &kv : _systems) kv.second->init();
   }
+  // include/systems/equation_systems.h:78-91
+  enum ReadFlags {
+    READ_HEADER = 1,
+    READ_DATA = 2,
+    READ_ADDITIONAL_DATA = 4,
+    READ_LEGACY_FORMAT = 8,
+    TRY_READ_IFEMS = 16,
+    READ_BASIC_ONLY = 32
+  };
+  enum WriteFlags { WRITE_DATA = 1, WRITE_ADDITIONAL_DATA = 2, WRITE_PARALLEL_FILES = 4, WRITE_SERIAL_FILES = 8 };
+  // EquationSystems::write (src/systems/equation_systems_io.C:335-560): the
+  // serial file, ASCII or XDR by name (".xdr") or mode, nodes in id order (a
+  // libMesh build without libHilbert writes the same order; with libHilbert
+  // and MPI, partition_agnostic renumbers along a Hilbert curve first)
+  void write(const std::string &name, unsigned int write_flags = WRITE_DATA) const;
+  void write(const std::string &name, XdrMODE mode, unsigned int write_flags = WRITE_DATA) const;
+  // EquationSystems::read (equation_systems_io.C:59-330) into systems that
+  // are already added and initialised: the file's header must describe them
+  // (names, variables, orders, families); READ_DATA fills their solutions
+  void read(const std::string &name, unsigned int read_flags = READ_HEADER | READ_DATA);
+  void read(const std::string &name, XdrMODE mode, unsigned int read_flags = READ_HEADER | READ_DATA);
   // EquationSystems parameters (equation_systems.C:59-60)
   Real linear_solver_tolerance = 1e-12;
   unsigned linear_solver_maximum_iterations = 5000;

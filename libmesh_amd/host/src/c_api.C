@@ -7,6 +7,7 @@
 #include <fstream>
 
 #include "libmesh/dof_map.h"
+#include "libmesh/equation_systems_io.h"
 #include "libmesh/matrix_io.h"
 #include "libmesh/replicated_mesh.h"
 
@@ -96,6 +97,24 @@ int lmesh_matrix_write_matlab(const char *filename, int64_t m, int64_t n, const 
     std::ofstream os(filename);
     if (!os.good()) libmesh_error_msg("cannot write " << filename);
     MatrixIO::write_matlab(os, (std::size_t)m, (std::size_t)n, rp, ci, v);
+    return 0;
+  } catch (const std::exception &e) {
+    g_lmesh_err = e.what();
+    return -1;
+  }
+}
+
+int lmesh_es_file_convert(const char *in_name, const char *out_name) {
+  try {
+    XdaFile f;
+    {
+      Xdr in(in_name, xdr_mode_for(in_name, false));
+      read_equation_systems_file(in, f, true);
+      in.close();
+    }
+    Xdr out(out_name, xdr_mode_for(out_name, true));
+    write_equation_systems_file(out, f, true);
+    out.close();
     return 0;
   } catch (const std::exception &e) {
     g_lmesh_err = e.what();

@@ -132,3 +132,35 @@ def test_fem_system_newton_dropin(tmp_path):
     zt, _, _ = O.cg_eigen(rp, ci, v, dq, 1e-14, 20000)
     exact = s.adjoint_exact(3, kappa, x, zt)
     assert np.abs(grad - exact).max() / np.abs(exact).max() < 1e-6, out
+
+
+def test_equation_systems_write_read_dropin(tmp_path):
+    """EquationSystems::write / read from C++ (es_io_dropin): two systems on a
+    perturbed 3^3 HEX27 cube.  The ASCII file's values, read here independently,
+    are the coordinate polynomials the example set, in node-major order (nodes
+    by id, each node's variables); the XDR file decodes to the same bits; both
+    read back into fresh systems exactly; a header that does not describe the
+    systems is rejected."""
+    import libmesh_amd as L
+    from es_files import parse_xda, parse_xdr
+
+    n = 3
+    out = run(["es_io_dropin", n, tmp_path / "es"], tmp_path)
+    assert "xdr_maxdiff 0.000e+00 xda_maxdiff 0.000e+00 mismatch_rejected 1" in out, out
+    p = L.Problem(n, L.HEX27)
+    p.perturb(0x5EED0001, 0.2)
+    x, y, z = np.asarray(p.coords, dtype=np.float64).reshape(-1, 3).T
+    a = parse_xda(tmp_path / "es.xda")
+    assert a["version"] == "libMesh-1.7.0"
+    assert [(s["name"], s["number"], s["type"]) for s in a["systems"]] == [("Poisson", 0, "LinearImplicit"),
+                                                                           ("aux", 1, "Basic")]
+    assert a["systems"][0]["vars"] == [("u", [], 2, 0)] and a["systems"][0]["vectors"] == []
+    assert a["systems"][1]["vars"] == [("a", [], 2, 0), ("b", [], 2, 0)]
+    u = a["systems"][0]["data"][0]
+    ab = a["systems"][1]["data"][0]
+    assert np.abs(u - (1 + x + 2 * y + 3 * z)).max() <= 1e-15 * 7
+    assert np.abs(ab[0::2] - (x * y - z)).max() <= 1e-15 * 2
+    assert np.abs(ab[1::2] - (x * x - y * z + 0.125)).max() <= 1e-15 * 2
+    b = parse_xdr(tmp_path / "es.xdr")
+    for sa, sb in zip(a["systems"], b["systems"]):
+        assert sa["vars"] == sb["vars"] and all(np.array_equal(d, e) for d, e in zip(sa["data"], sb["data"]))
