@@ -81,35 +81,32 @@ __device__ inline void reduce_partials(const double *__restrict__ part, int G, d
 // writes the products to LDS, then 8 lanes per row sum each row's products in a
 // fixed order.  A single row longer than RB_CAP is one block, reduced directly.
 // epi(row, sum) consumes each row result in the calling thread.
-// Column decoders: plain int32 CSR columns, or the compressed FP32-operator
-// columns (16-bit offsets against two per-block bases).
-struct PlainCols {
-  const int32_t *ci;
-  struct Meta {};
-  __device__ inline Meta meta_of(int64_t) const { return Meta{}; }
-  __device__ inline int32_t raw(int32_t k) const { return __builtin_nontemporal_load(&ci[k]); }
-  __device__ inline int32_t dec(int32_t c, const Meta &) const { return c; }
-};
-struct Off16Cols {
-  const uint16_t *off;
-  const int4 *meta;
-  using Meta = int4;  // {b0, b1, s0, -}
-  __device__ inline Meta meta_of(int64_t bk) const { return meta[bk]; }
-  __device__ inline int32_t raw(int32_t k) const { return (int32_t)__builtin_nontemporal_load(&off[k]); }
-  __device__ inline int32_t dec(int32_t o, const Meta &m) const { return o + (o <= m.z ? m.x : m.y); }
+// pre(row, out) loads the NPRE per-row values an epilogue needs (b[row],
+// p[row], ...); epi(row, sum, pre_values) consumes each row result in the
+// calling thread.
+struct NoPre {
+  __device__ inline void operator()(int32_t, double *) const {}
 };
 
 // Software-pipelined over the workgroup's row blocks: the value/column loads
-// and the row pointers of block bk+1 are issued right after the x gathers of
-// block bk, so they are in flight while block bk's products are formed and its
-// rows reduced.  Row pointers of a block with fewer than BLK rows are staged in
-// LDS; FP32 products are double-buffered so one barrier per block suffices.
-template <typename VT, typename XT, typename AT, typename Epi, typename Cols = PlainCols>
-__device__ inline void rowblock_spmv_c(const int32_t *__restrict__ rb, int64_t nrb, const int32_t *__restrict__ rp,
-                                       Cols cols, const VT *__restrict__ v, const XT *__restrict__ x, Epi &&epi) {
+// and the row pointers of block bk+1 are issued right after the x gathers and
+// the per-row epilogue values of block bk, and nothing of block bk's own work
+// loads from global memory after them, so they stay in flight while block
+// bk's products are formed and its rows reduced.  The main loop issues that
+// fetch unconditionally (the last block is peeled): behind a condition, the
+// compiler's wait counts merge both paths and every later wait drains the
+// fetch.  Row pointers and epilogue values of a block with fewer than BLK rows
+// are staged in LDS; FP32 products are double-buffered so one barrier per
+// block suffices.
+template <typename VT, typename XT, typename AT, int NPRE, typename Pre, typename Epi>
+__device__ inline void rowblock_spmv(const int32_t *__restrict__ rb, int64_t nrb, const int32_t *__restrict__ rp,
+                                     const int32_t *__restrict__ ci, const VT *__restrict__ v,
+                                     const XT *__restrict__ x, Pre &&pre, Epi &&epi) {
   constexpr int NBUF = sizeof(AT) == 4 ? 2 : 1;
+  constexpr int NP = NPRE > 0 ? NPRE : 1;
   __shared__ AT prodbuf[NBUF][RB_CAP];
   __shared__ int32_t rowbuf[NBUF][BLK];
+  __shared__ double prebuf[NBUF][NP][NPRE > 0 ? BLK : 1];
   __shared__ double red[BLK / 64];
   constexpr int PER = RB_CAP / BLK;
   int64_t lo, hi;
@@ -117,7 +114,6 @@ __device__ inline void rowblock_spmv_c(const int32_t *__restrict__ rb, int64_t n
   const int tid = threadIdx.x;
   // state of the block whose loads are in flight
   int32_t r0 = 0, r1 = 0, k0 = 0, cnt = 0, rpn = 0;
-  typename Cols::Meta meta{};
   VT vv[PER];
   int32_t cr[PER];
   // branch-free loads (out-of-range slots re-read entry k0 and are dropped)
@@ -126,27 +122,22 @@ __device__ inline void rowblock_spmv_c(const int32_t *__restrict__ rb, int64_t n
     r1 = rb[bk + 1];
     k0 = rp[r0];
     cnt = rp[r1] - k0;
-    meta = cols.meta_of(bk);
     rpn = rp[r0 + (tid <= r1 - r0 ? tid : 0)];
-    if (cnt <= RB_CAP) {
 #pragma unroll
-      for (int j = 0; j < PER; ++j) {
-        const int32_t t = tid + j * BLK;
-        const int32_t k = k0 + (t < cnt ? t : 0);
-        vv[j] = __builtin_nontemporal_load(&v[k]);
-        cr[j] = cols.raw(k);
-      }
+    for (int j = 0; j < PER; ++j) {
+      const int32_t t = tid + j * BLK;
+      const int32_t k = k0 + (t < cnt ? t : 0);
+      vv[j] = __builtin_nontemporal_load(&v[k]);
+      cr[j] = __builtin_nontemporal_load(&ci[k]);
     }
   };
-  if (lo < hi) fetch(lo);
   int buf = 0;
-  for (int64_t bk = lo; bk < hi; ++bk) {
+  auto body = [&](int64_t bk, auto more) {
     const int32_t R0 = r0, R1 = r1, K0 = k0, CNT = cnt, RPN = rpn;
-    const typename Cols::Meta M = meta;
     if (CNT > RB_CAP) {  // one long row: reduce it directly
       AT s = 0;
       for (int32_t t = tid; t < CNT; t += BLK)
-        s += (AT)__builtin_nontemporal_load(&v[K0 + t]) * (AT)x[cols.dec(cols.raw(K0 + t), M)];
+        s += (AT)__builtin_nontemporal_load(&v[K0 + t]) * (AT)x[__builtin_nontemporal_load(&ci[K0 + t])];
       double w = wave_sum((double)s);
       __syncthreads();
       if ((tid & 63) == 0) red[tid >> 6] = w;
@@ -154,24 +145,32 @@ __device__ inline void rowblock_spmv_c(const int32_t *__restrict__ rb, int64_t n
       if (tid == 0) {
         double t2 = 0;
         for (int k = 0; k < BLK / 64; ++k) t2 += red[k];
-        epi(R0, (AT)t2);
+        double pv[NP];
+        pre(R0, pv);
+        epi(R0, (AT)t2, pv);
       }
-      if (bk + 1 < hi) fetch(bk + 1);
-      continue;
+      if constexpr (decltype(more)::value) fetch(bk + 1);
+      return;
     }
     AT *prod = prodbuf[buf];
     int32_t *rows = rowbuf[buf];
+    double(*pb)[NPRE > 0 ? BLK : 1] = prebuf[buf];
     const bool staged = R1 - R0 < BLK;
     AT xx[PER];
     VT cv[PER];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-      xx[j] = (AT)x[cols.dec(cr[j], M)];
+      xx[j] = (AT)x[cr[j]];
       cv[j] = vv[j];
     }
-    if (bk + 1 < hi) fetch(bk + 1);
+    double pv[NP];
+    if (NPRE > 0 && staged && R1 > R0) pre(R0 + (tid < R1 - R0 ? tid : 0), pv);
+    if constexpr (decltype(more)::value) fetch(bk + 1);
     if (NBUF == 1) __syncthreads();  // the previous block's rows are reduced
     if (staged && tid <= R1 - R0) rows[tid] = RPN - K0;
+    if (NPRE > 0 && staged && tid < R1 - R0)
+#pragma unroll
+      for (int k = 0; k < NP; ++k) pb[k][tid] = pv[k];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
       const int32_t t = tid + j * BLK;
@@ -179,17 +178,42 @@ __device__ inline void rowblock_spmv_c(const int32_t *__restrict__ rb, int64_t n
     }
     __syncthreads();
     const int g = tid >> 3, sub = tid & 7;
-    for (int32_t r = R0 + g; r < R1; r += BLK / 8) {
-      const int32_t a = staged ? rows[r - R0] : rp[r] - K0;
-      const int32_t e = staged ? rows[r - R0 + 1] : rp[r + 1] - K0;
-      AT s = 0;
-      for (int32_t t = a + sub; t < e; t += 8) s += prod[t];
-      s += __shfl_xor(s, 4, 8);
-      s += __shfl_xor(s, 2, 8);
-      s += __shfl_xor(s, 1, 8);
-      if (sub == 0) epi(r, s);
+    if (staged) {
+      for (int32_t r = R0 + g; r < R1; r += BLK / 8) {
+        const int32_t a = rows[r - R0], e = rows[r - R0 + 1];
+        AT s = 0;
+        for (int32_t t = a + sub; t < e; t += 8) s += prod[t];
+        s += __shfl_xor(s, 4, 8);
+        s += __shfl_xor(s, 2, 8);
+        s += __shfl_xor(s, 1, 8);
+        if (sub == 0) {
+          double p2[NP];
+#pragma unroll
+          for (int k = 0; k < NP; ++k) p2[k] = NPRE > 0 ? pb[k][r - R0] : 0.0;
+          epi(r, s, p2);
+        }
+      }
+    } else {
+      for (int32_t r = R0 + g; r < R1; r += BLK / 8) {
+        const int32_t a = rp[r] - K0, e = rp[r + 1] - K0;
+        AT s = 0;
+        for (int32_t t = a + sub; t < e; t += 8) s += prod[t];
+        s += __shfl_xor(s, 4, 8);
+        s += __shfl_xor(s, 2, 8);
+        s += __shfl_xor(s, 1, 8);
+        if (sub == 0) {
+          double p2[NP];
+          pre(r, p2);
+          epi(r, s, p2);
+        }
+      }
     }
     buf ^= NBUF - 1;
+  };
+  if (lo < hi) {
+    fetch(lo);
+    for (int64_t bk = lo; bk + 1 < hi; ++bk) body(bk, std::true_type{});
+    body(hi - 1, std::false_type{});
   }
 }
 
@@ -202,7 +226,7 @@ __device__ inline void rowblock_spmv_c(const int32_t *__restrict__ rb, int64_t n
 // chunks and the window indices as 8-entry chunks, both from the block's start
 // rounded down to 8 entries; the indices pass through LDS, so every product
 // lands at its entry's slot and the rows are reduced exactly as in
-// rowblock_spmv_c (same products, same summation order, bit-identical sums).
+// rowblock_spmv (same products, same summation order, bit-identical sums).
 // Row blocks hold at most RB_CAP - 8 entries, so a block's chunks fit one
 // 8-entry index chunk and two 4-entry value chunks per thread.
 typedef uint32_t vu4 __attribute__((ext_vector_type(4)));
@@ -337,12 +361,6 @@ __device__ inline double rowblock_spmv16(const int32_t *__restrict__ rb, int64_t
   return acc;
 }
 
-template <typename VT, typename XT, typename AT, typename Epi>
-__device__ inline void rowblock_spmv(const int32_t *__restrict__ rb, int64_t nrb, const int32_t *__restrict__ rp,
-                                     const int32_t *__restrict__ ci, const VT *__restrict__ v,
-                                     const XT *__restrict__ x, Epi &&epi) {
-  rowblock_spmv_c<VT, XT, AT>(rb, nrb, rp, PlainCols{ci}, v, x, epi);
-}
 
 // ---------------------------------------------------------------------------
 // plain SpMV y = A x
@@ -352,7 +370,7 @@ __global__ __launch_bounds__(BLK) void k_spmv(const int32_t *__restrict__ rb, in
                                               const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
                                               const VT *__restrict__ v, const XT *__restrict__ x,
                                               YT *__restrict__ y) {
-  rowblock_spmv<VT, XT, AT>(rb, nrb, rp, ci, v, x, [&](int32_t r, AT s) { y[r] = (YT)s; });
+  rowblock_spmv<VT, XT, AT, 0>(rb, nrb, rp, ci, v, x, NoPre{}, [&](int32_t r, AT s, const double *) { y[r] = (YT)s; });
 }
 
 
@@ -363,12 +381,14 @@ __global__ __launch_bounds__(BLK) void k_residual(const int32_t *__restrict__ rb
                                                   const double *__restrict__ x, double *__restrict__ r,
                                                   double *__restrict__ part) {
   double acc[2] = {0, 0};
-  rowblock_spmv<double, double, double>(rb, nrb, rp, ci, v, x, [&](int32_t row, double s) {
-    const double bv = b[row], rv = bv - s;
-    r[row] = rv;
-    acc[0] += rv * rv;
-    acc[1] += bv * bv;
-  });
+  rowblock_spmv<double, double, double, 1>(
+      rb, nrb, rp, ci, v, x, [&](int32_t row, double *o) { o[0] = b[row]; },
+      [&](int32_t row, double s, const double *pr) {
+        const double bv = pr[0], rv = bv - s;
+        r[row] = rv;
+        acc[0] += rv * rv;
+        acc[1] += bv * bv;
+      });
   publish<2>(acc, part);
 }
 
@@ -463,10 +483,12 @@ __global__ __launch_bounds__(BLK) void k_it_spmv(const int32_t *__restrict__ rb,
                                                  const double *__restrict__ S, int bank) {
   if (S[bank * B_SIZE + B_DONE] != 0.0) return;
   double acc[1] = {0};
-  rowblock_spmv<VT, VT, VT>(rb, nrb, rp, ci, v, p, [&](int32_t r, VT s) {
-    q[r] = s;
-    acc[0] += (double)p[r] * (double)s;
-  });
+  rowblock_spmv<VT, VT, VT, 1>(
+      rb, nrb, rp, ci, v, p, [&](int32_t r, double *o) { o[0] = (double)p[r]; },
+      [&](int32_t r, VT s, const double *pr) {
+        q[r] = s;
+        acc[0] += pr[0] * (double)s;
+      });
   publish<1>(acc, part);
 }
 
@@ -739,16 +761,22 @@ __global__ __launch_bounds__(BLK) void k_refresh(int init, const int32_t *__rest
                                                  double *__restrict__ part, const double *__restrict__ S, int bank) {
   if (!init && S[bank * B_SIZE + B_DONE] != 0.0) return;
   double acc[3] = {0, 0, 0};
-  rowblock_spmv<double, double, double>(rb, nrb, rp, ci, v, x, [&](int32_t row, double s) {
-    const double bv = b[row], rv = bv - s;
-    const float rf = (float)rv;
-    r32[row] = rf;
-    const float z = dinv32[row] * rf;
-    if (init) p32[row] = z;
-    acc[0] += rv * rv;
-    acc[1] += bv * bv;
-    acc[2] += (double)rf * (double)z;
-  });
+  rowblock_spmv<double, double, double, 2>(
+      rb, nrb, rp, ci, v, x,
+      [&](int32_t row, double *o) {
+        o[0] = b[row];
+        o[1] = (double)dinv32[row];
+      },
+      [&](int32_t row, double s, const double *pr) {
+        const double bv = pr[0], rv = bv - s;
+        const float rf = (float)rv;
+        r32[row] = rf;
+        const float z = (float)pr[1] * rf;
+        if (init) p32[row] = z;
+        acc[0] += rv * rv;
+        acc[1] += bv * bv;
+        acc[2] += (double)rf * (double)z;
+      });
   publish<3>(acc, part);
 }
 
@@ -1093,10 +1121,12 @@ __global__ __launch_bounds__(BLK) void k_bcg_spmv_v(const int32_t *__restrict__ 
                                                     int nbank) {
   if (S[nbank * B_SIZE + B_DONE] != 0.0) return;
   double acc[1] = {0};
-  rowblock_spmv<double, double, double>(rb, nrb, rp, ci, vals, y, [&](int32_t row, double sv) {
-    v[row] = sv;
-    acc[0] += r0[row] * sv;
-  });
+  rowblock_spmv<double, double, double, 1>(
+      rb, nrb, rp, ci, vals, y, [&](int32_t row, double *o) { o[0] = r0[row]; },
+      [&](int32_t row, double sv, const double *pr) {
+        v[row] = sv;
+        acc[0] += pr[0] * sv;
+      });
   publish<1>(acc, part);
 }
 
@@ -1129,11 +1159,13 @@ __global__ __launch_bounds__(BLK) void k_bcg_spmv_t(const int32_t *__restrict__ 
                                                     int nbank) {
   if (S[nbank * B_SIZE + B_DONE] != 0.0) return;
   double acc[2] = {0, 0};
-  rowblock_spmv<double, double, double>(rb, nrb, rp, ci, vals, z, [&](int32_t row, double tv) {
-    t[row] = tv;
-    acc[0] += tv * tv;
-    acc[1] += tv * s[row];
-  });
+  rowblock_spmv<double, double, double, 1>(
+      rb, nrb, rp, ci, vals, z, [&](int32_t row, double *o) { o[0] = s[row]; },
+      [&](int32_t row, double tv, const double *pr) {
+        t[row] = tv;
+        acc[0] += tv * tv;
+        acc[1] += tv * pr[0];
+      });
   publish<2>(acc, part);
 }
 
