@@ -241,13 +241,24 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
                     int64_t maxits, int64_t *iters, double *rel_res, int *reason, double *stats_h);
 /* Build the FP32 operator by rounding the FP64 values. */
 int lmh_matrix_make_fp32(lmh_matrix *A);
-/* Column storage of the FP32 operator: on (default) stores 16-bit column
- * offsets against two bases per SpMV row block (6 bytes per entry with the
- * FP32 value instead of 8) whenever every block's columns fit; off keeps the
- * int32 CSR columns.  lmh_matrix_fp32_compression reports 1 when the
- * compressed columns are in use, 0 otherwise (builds them if needed). */
+/* Column storage of the FP32 operator: on (default) stores, per SpMV row
+ * block, the block's distinct columns once (16-bit offsets against two bases)
+ * and a 16-bit index into that list per entry (6 bytes per entry with the
+ * FP32 value instead of 8; x is gathered once per distinct column) whenever
+ * every block fits; off keeps the int32 CSR columns.
+ * lmh_matrix_fp32_compression reports 1 when the compressed columns are in
+ * use, 0 otherwise (builds them if needed). */
 int lmh_matrix_set_fp32_compression(lmh_matrix *A, int on);
 int lmh_matrix_fp32_compression(lmh_matrix *A);
+/* Preconditioner of lmh_cg_solve, lmh_bicgstab_solve and lmh_mixed_solve on
+ * this matrix, with libMesh's PreconditionerType values: LMH_PRECOND_JACOBI
+ * (default; Eigen DiagonalPreconditioner, 1/a_ii or 1 where the diagonal is
+ * absent or zero) or LMH_PRECOND_IDENTITY (Eigen IdentityPreconditioner),
+ * the two EigenSparseLinearSolver::solve runs without a factorisation
+ * (src/solvers/eigen_sparse_linear_solver.C:136-170, 177-205).  Other values:
+ * LMH_ERR_UNSUPPORTED. */
+enum { LMH_PRECOND_IDENTITY = 0, LMH_PRECOND_JACOBI = 1 };
+int lmh_matrix_set_preconditioner(lmh_matrix *A, int type);
 
 /* ---- adjoint sensitivity (ImplicitSystem::adjoint_qoi_parameter_sensitivity,
  * implicit_system.C:517-623, for per-element kappa parameters) ---- */

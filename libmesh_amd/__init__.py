@@ -27,6 +27,7 @@ SCATTER_COLORED, SCATTER_GATHER = 0, 1
 CONSTRAINT_FORBID_OVERWRITE, CONSTRAINT_ERROR_ON_LOOP, CONSTRAINT_SYMMETRIC_ROWS = 1, 2, 4
 ALL_FACES = 0x3F
 CONVERGED_RTOL, DIVERGED_ITS = 2, -3
+PRECOND_IDENTITY, PRECOND_JACOBI = 0, 1  # libMesh PreconditionerType values
 
 _P = C.c_void_p
 _I64 = C.c_int64
@@ -94,6 +95,7 @@ HIP_API = {
     "lmh_matrix_make_fp32": (_I, [_P]),
     "lmh_matrix_set_fp32_compression": (_I, [_P, _I]),
     "lmh_matrix_fp32_compression": (_I, [_P]),
+    "lmh_matrix_set_preconditioner": (_I, [_P, _I]),
     "lmh_qoi_box_derivative": (_I, [_P, _P, _I, _D, _D, _P]),
     "lmh_adjoint_kappa_gradient": (_I, [_P, _P, _I, _P, _P, _P, _P]),
     "lmh_batch_create": (_I, [_P, _I, _P, C.POINTER(_P)]),
@@ -474,6 +476,10 @@ class GpuProblem:
 
     def make_fp32(self):
         check(hip().lmh_matrix_make_fp32(self.matrix))
+
+    def set_preconditioner(self, precond):
+        """PRECOND_JACOBI (default) or PRECOND_IDENTITY for cg / mixed / bicgstab."""
+        check(hip().lmh_matrix_set_preconditioner(self.matrix, precond))
 
     def mixed(self, b_d, x_d, rtol, inner_rtol=0.1, mode=0, maxits=20000):
         its = _I64()

@@ -161,6 +161,7 @@ struct lmh_matrix {
   double *vals = nullptr;
   float *vals32 = nullptr;
   bool have32 = false;
+  int precond = LMH_PRECOND_JACOBI;  // preconditioner of the solvers on this matrix
   // FP32 operator column compression (x windows): each row block lists its
   // distinct columns once, sorted, as 16-bit offsets woff16[w0 .. w0+nu),
   // col = off + (off <= s0 ? b0 : base1); every entry stores the 16-bit index

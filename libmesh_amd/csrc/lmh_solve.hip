@@ -428,12 +428,12 @@ __global__ void k_diag_index(int64_t n, const int32_t *__restrict__ rp, const in
 
 // Jacobi inverse diagonal: 1/a_rr, or 1 when the diagonal is absent or zero
 template <typename VT, typename OT>
-__global__ void k_invdiag(int64_t n, const int32_t *__restrict__ diag, const VT *__restrict__ v,
+__global__ void k_invdiag(int64_t n, const int32_t *__restrict__ diag, const VT *__restrict__ v, int jacobi,
                           OT *__restrict__ dinv) {
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     const int32_t k = diag[r];
     OT d = (OT)1;
-    if (k >= 0 && v[k] != (VT)0) d = (OT)1 / (OT)v[k];
+    if (jacobi && k >= 0 && v[k] != (VT)0) d = (OT)1 / (OT)v[k];
     dinv[r] = d;
   }
 }
@@ -1220,7 +1220,7 @@ int lmh_cg_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int
   const int g = vec_grid(ctx), gs = spmv_grid(ctx);
   LMH_TRY(ensure_diag(A));
   LMH_LAUNCH(ctx, "invdiag", (k_invdiag<double, double>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, n, A->diag,
-             A->vals, dinv);
+             A->vals, A->precond == LMH_PRECOND_JACOBI, dinv);
   LMH_LAUNCH(ctx, "residual_fp64", k_residual, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals, b_d,
              x_d, r, part2);
   LMH_LAUNCH(ctx, "cg_init", k_cg_init, dim3(g), dim3(BLK), 0, n, dinv, r, p, part1);
@@ -1286,7 +1286,7 @@ int lmh_bicgstab_solve(lmh_matrix *A, const double *b_d, double *x_d, double rto
   const int g = vec_grid(ctx), gs = spmv_grid(ctx);
   LMH_TRY(ensure_diag(A));
   LMH_LAUNCH(ctx, "invdiag", (k_invdiag<double, double>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, n, A->diag,
-             A->vals, dinv);
+             A->vals, A->precond == LMH_PRECOND_JACOBI, dinv);
   LMH_LAUNCH(ctx, "residual_fp64", k_residual, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals, b_d,
              x_d, r, part2);
   LMH_LAUNCH(ctx, "bcg_init", k_bcg_init, dim3(g), dim3(BLK), 0, n, (const double *)r, r0, p, v, part1);
@@ -1367,7 +1367,7 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
   LMH_HIP(hipMemcpyAsync(S, init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
   LMH_TRY(ensure_diag(A));
   LMH_LAUNCH(ctx, "invdiag", (k_invdiag<float, float>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, n, A->diag,
-             A->vals32, d32);
+             A->vals32, A->precond == LMH_PRECOND_JACOBI, d32);
   LMH_HIP(hipMemsetAsync(xc, 0, sizeof(float) * n, ctx->stream));
   LMH_LAUNCH(ctx, "mx_refresh_fp64", k_refresh, dim3(gs), dim3(BLK), 0, 1, A->rb, A->nrb, A->rp, A->ci, A->vals,
              b_d, x_d, d32, r32, p32, part2, S, 0);
@@ -1452,6 +1452,14 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
 int lmh_matrix_set_fp32_compression(lmh_matrix *A, int on) {
   LMH_CHECK(A, LMH_ERR_INVALID, "null matrix");
   A->fp32_index = on ? 1 : 0;
+  return LMH_OK;
+}
+
+int lmh_matrix_set_preconditioner(lmh_matrix *A, int type) {
+  LMH_CHECK(A, LMH_ERR_INVALID, "null matrix");
+  LMH_CHECK(type == LMH_PRECOND_JACOBI || type == LMH_PRECOND_IDENTITY, LMH_ERR_UNSUPPORTED,
+            "preconditioner %d: only JACOBI_PRECOND and IDENTITY_PRECOND run on the device", type);
+  A->precond = type;
   return LMH_OK;
 }
 

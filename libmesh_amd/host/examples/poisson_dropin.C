@@ -4,14 +4,15 @@
 //   sys.linear_solver = make_unique<HipLinearSolver>             (ImplicitSystem::linear_solver)
 //   sys.attach_assemble_object(HipAssembler(...))                (System::attach_assemble_object)
 // Usage: poisson_dropin n elem_type(10|12|8|9) out_prefix [perturb] [kappa] [sine] [mixed] [adjoint] [distort]
-//                      [periodic] [bicgstab]
+//                      [periodic] [bicgstab] [identity]
 // Writes <prefix>_u.bin (solution) and, with `adjoint`, <prefix>_grad.bin.
 // `periodic`: Dirichlet on the y and z faces only, u(x=1) = u(x=0) through
 // constraint rows from a System::Constraint object (plus one heterogeneous
 // row u = 0.3 on an x=0 face DoF), symmetric constraint rows, and
 // enforce_constraints_exactly after the solve.  `periodic bicgstab`: the
 // default asymmetric constraint rows instead, solved by Jacobi-BiCGSTAB
-// (LinearSolver::set_solver_type(BICGSTAB)).
+// (LinearSolver::set_solver_type(BICGSTAB)).  `identity`: IDENTITY_PRECOND
+// instead of JACOBI_PRECOND (LinearSolver::set_preconditioner_type).
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -85,7 +86,7 @@ int main(int argc, char **argv) {
   const ElemType type = (ElemType)std::atoi(argv[2]);
   const std::string prefix = argv[3];
   bool perturb = false, kappa = false, sine = false, mixed = false, adjoint = false, distort = false,
-       periodic = false, bicgstab = false;
+       periodic = false, bicgstab = false, identity = false;
   for (int a = 4; a < argc; ++a) {
     perturb |= !std::strcmp(argv[a], "perturb");
     kappa |= !std::strcmp(argv[a], "kappa");
@@ -95,6 +96,7 @@ int main(int argc, char **argv) {
     distort |= !std::strcmp(argv[a], "distort");
     periodic |= !std::strcmp(argv[a], "periodic");
     bicgstab |= !std::strcmp(argv[a], "bicgstab");
+    identity |= !std::strcmp(argv[a], "identity");
   }
   try {
     Mesh mesh;
@@ -125,6 +127,7 @@ int main(int argc, char **argv) {
     auto solver = std::make_unique<HipLinearSolver<Number>>();
     solver->set_mixed_precision(mixed);
     if (bicgstab) solver->set_solver_type(BICGSTAB);
+    if (identity) solver->set_preconditioner_type(IDENTITY_PRECOND);
     sys.linear_solver = std::move(solver);
     es.linear_solver_tolerance = 1e-10;
     es.linear_solver_maximum_iterations = 20000;

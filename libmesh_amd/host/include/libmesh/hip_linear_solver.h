@@ -1,7 +1,8 @@
-// hip_linear_solver.h -- HipLinearSolver<T>: Jacobi-preconditioned CG on the
-// GPU for a HipCsrMatrix (EigenSparseLinearSolver CG + JACOBI_PRECOND,
-// src/solvers/eigen_sparse_linear_solver.C:82-170), optionally as FP32 inner CG
-// with FP64 reliable-update refinement to the same FP64 tolerance.
+// hip_linear_solver.h -- HipLinearSolver<T>: CG or BiCGSTAB on the GPU for a
+// HipCsrMatrix with JACOBI_PRECOND (default) or IDENTITY_PRECOND
+// (EigenSparseLinearSolver, src/solvers/eigen_sparse_linear_solver.C:82-205),
+// CG optionally as FP32 inner CG with FP64 reliable-update refinement to the
+// same FP64 tolerance.
 #pragma once
 
 #include "libmesh/hip_csr_matrix.h"

@@ -1,5 +1,7 @@
-// hip_linear_solver.C -- LinearSolver backend: Jacobi-CG (FP64 or mixed) and
-// Jacobi-BiCGSTAB (FP64) on the GPU.
+// hip_linear_solver.C -- LinearSolver backend: CG (FP64 or mixed) and
+// BiCGSTAB (FP64) on the GPU, with JACOBI_PRECOND or IDENTITY_PRECOND (the two
+// preconditioners EigenSparseLinearSolver::solve runs without a factorisation,
+// eigen_sparse_linear_solver.C:136-170, 177-205).
 #include "libmesh/hip_linear_solver.h"
 
 namespace libMesh {
@@ -7,7 +9,8 @@ namespace libMesh {
 template <typename T>
 void HipLinearSolver<T>::init(const char *) {
   if (this->_solver_type != CG && this->_solver_type != BICGSTAB) libmesh_not_implemented();
-  if (this->_preconditioner_type != JACOBI_PRECOND) libmesh_not_implemented();
+  if (this->_preconditioner_type != JACOBI_PRECOND && this->_preconditioner_type != IDENTITY_PRECOND)
+    libmesh_not_implemented();
   this->_is_initialized = true;
 }
 
@@ -36,13 +39,17 @@ std::pair<unsigned int, Real> HipLinearSolver<T>::solve(SparseMatrix<T> &matrix_
   A->close();
   x->close();
   b->close();
+  hip_check(lmh_matrix_set_preconditioner(A->device_matrix(), this->_preconditioner_type == IDENTITY_PRECOND
+                                                                  ? LMH_PRECOND_IDENTITY
+                                                                  : LMH_PRECOND_JACOBI),
+            "HipLinearSolver (preconditioner)");
   const double rtol = tol.value_or(this->default_tolerance());
   const unsigned maxits = m_its.value_or(this->default_max_iterations());
   int64_t its = 0;
   double res = 0;
   int reason = 0;
   if (this->_solver_type == BICGSTAB) {
-    // EigenSparseLinearSolver::solve, case BICGSTAB + JACOBI_PRECOND (eigen_sparse_linear_solver.C:175-205)
+    // EigenSparseLinearSolver::solve, case BICGSTAB (eigen_sparse_linear_solver.C:175-205)
     if (_mixed) libmesh_not_implemented();
     hip_check(lmh_bicgstab_solve(A->device_matrix(), (const double *)b->device_data(), (double *)x->device_data(),
                                  rtol, maxits, &its, &res, &reason),

@@ -56,3 +56,39 @@ extern "C" int ref_eigen_bicgstab(int64_t n, const int32_t *rp, const int32_t *c
   *err = solver.error();
   return (int)solver.info();
 }
+
+// CG and BICGSTAB with IDENTITY_PRECOND (eigen_sparse_linear_solver.C:136-143,
+// 177-183): ConjugateGradient / BiCGSTAB with IdentityPreconditioner.
+extern "C" int ref_eigen_cg_identity(int64_t n, const int32_t *rp, const int32_t *ci, const double *v,
+                                     const double *b, double *x, double tol, int64_t maxits, int64_t *iters,
+                                     double *err) {
+  Eigen::Map<const EigenSM> view((Eigen::Index)n, (Eigen::Index)n, (Eigen::Index)rp[n], rp, ci, v);
+  EigenSM A(view);
+  Eigen::Map<const Eigen::VectorXd> bb(b, n);
+  Eigen::VectorXd xx = Eigen::Map<const Eigen::VectorXd>(x, n);
+  Eigen::ConjugateGradient<EigenSM, Eigen::Lower | Eigen::Upper, Eigen::IdentityPreconditioner> solver(A);
+  solver.setMaxIterations((Eigen::Index)maxits);
+  solver.setTolerance(tol);
+  xx = solver.solveWithGuess(bb, xx);
+  std::memcpy(x, xx.data(), sizeof(double) * n);
+  *iters = solver.iterations();
+  *err = solver.error();
+  return (int)solver.info();
+}
+
+extern "C" int ref_eigen_bicgstab_identity(int64_t n, const int32_t *rp, const int32_t *ci, const double *v,
+                                           const double *b, double *x, double tol, int64_t maxits, int64_t *iters,
+                                           double *err) {
+  Eigen::Map<const EigenSM> view((Eigen::Index)n, (Eigen::Index)n, (Eigen::Index)rp[n], rp, ci, v);
+  EigenSM A(view);
+  Eigen::Map<const Eigen::VectorXd> bb(b, n);
+  Eigen::VectorXd xx = Eigen::Map<const Eigen::VectorXd>(x, n);
+  Eigen::BiCGSTAB<EigenSM, Eigen::IdentityPreconditioner> solver(A);
+  solver.setMaxIterations((Eigen::Index)maxits);
+  solver.setTolerance(tol);
+  xx = solver.solveWithGuess(bb, xx);
+  std::memcpy(x, xx.data(), sizeof(double) * n);
+  *iters = solver.iterations();
+  *err = solver.error();
+  return (int)solver.info();
+}

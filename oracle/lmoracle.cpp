@@ -1310,12 +1310,23 @@ static double dotv(const std::vector<double> &a, const std::vector<double> &b) {
 // DiagonalPreconditioner (BasicPreconditioners.h:66-76: 1/a_jj, or 1 when the
 // diagonal is absent or zero).  Stops when |r|^2 < max(tol^2 |b|^2, DBL_MIN);
 // reports iterations and sqrt(|r|^2/|b|^2).
-void orc_cg_eigen(int64_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *b, double *x,
-                  double tol, int64_t maxits, int64_t *iters, double *rel_err) {
+// precond: 1 JACOBI_PRECOND (DiagonalPreconditioner), 0 IDENTITY_PRECOND
+// (IdentityPreconditioner, BasicPreconditioners.h:200-226: z = r), the two
+// EigenSparseLinearSolver::solve offers without a factorisation
+// (eigen_sparse_linear_solver.C:136-170, 177-205).
+static std::vector<double> precond_diag(int64_t n, const int32_t *rp, const int32_t *ci, const double *v,
+                                        int precond) {
   std::vector<double> invd(n, 1.0);
-  for (i64 r = 0; r < n; ++r)
-    for (i64 k = rp[r]; k < rp[r + 1]; ++k)
-      if (ci[k] == r && v[k] != 0.0) invd[r] = 1.0 / v[k];
+  if (precond == 1)
+    for (i64 r = 0; r < n; ++r)
+      for (i64 k = rp[r]; k < rp[r + 1]; ++k)
+        if (ci[k] == r && v[k] != 0.0) invd[r] = 1.0 / v[k];
+  return invd;
+}
+
+void orc_cg_eigen_pc(int64_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *b, double *x,
+                     double tol, int64_t maxits, int precond, int64_t *iters, double *rel_err) {
+  const std::vector<double> invd = precond_diag(n, rp, ci, v, precond);
   std::vector<double> bb(b, b + n), xx(x, x + n), res(n), tmp(n), p(n), z(n);
   orc_spmv(n, rp, ci, v, x, tmp.data());
   for (i64 i = 0; i < n; ++i) res[i] = bb[i] - tmp[i];
@@ -1361,12 +1372,9 @@ void orc_cg_eigen(int64_t n, const int32_t *rp, const int32_t *ci, const double 
 // |r0.r| < eps^2 |r0|^2 (the first restart also resets the iteration count);
 // stops when |r|^2 <= tol^2 |b|^2.  Returns the iteration count; rel_err =
 // |r|/|b|.
-void orc_bicgstab_eigen(int64_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *b, double *x,
-                        double tol, int64_t maxits, int64_t *iters, double *rel_err) {
-  std::vector<double> invd(n, 1.0);
-  for (i64 r = 0; r < n; ++r)
-    for (i64 k = rp[r]; k < rp[r + 1]; ++k)
-      if (ci[k] == r && v[k] != 0.0) invd[r] = 1.0 / v[k];
+void orc_bicgstab_eigen_pc(int64_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *b,
+                           double *x, double tol, int64_t maxits, int precond, int64_t *iters, double *rel_err) {
+  const std::vector<double> invd = precond_diag(n, rp, ci, v, precond);
   std::vector<double> bb(b, b + n), xx(x, x + n), r(n), r0(n), vv(n, 0.), p(n, 0.), y(n), z(n), s(n), t(n), tmp(n);
   orc_spmv(n, rp, ci, v, x, tmp.data());
   for (i64 i = 0; i < n; ++i) r[i] = bb[i] - tmp[i];
@@ -1410,6 +1418,16 @@ void orc_bicgstab_eigen(int64_t n, const int32_t *rp, const int32_t *ci, const d
   std::memcpy(x, xx.data(), n * 8);
   *iters = i;
   *rel_err = std::sqrt(dotv(r, r) / rhs_sqnorm);
+}
+
+void orc_cg_eigen(int64_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *b, double *x,
+                  double tol, int64_t maxits, int64_t *iters, double *rel_err) {
+  orc_cg_eigen_pc(n, rp, ci, v, b, x, tol, maxits, 1, iters, rel_err);
+}
+
+void orc_bicgstab_eigen(int64_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *b, double *x,
+                        double tol, int64_t maxits, int64_t *iters, double *rel_err) {
+  orc_bicgstab_eigen_pc(n, rp, ci, v, b, x, tol, maxits, 1, iters, rel_err);
 }
 
 // Laspack CGIter with JacobiPrecond, Omega = 1 (itersolv.C:377-476, precond.C:26-46):

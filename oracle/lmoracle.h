@@ -109,11 +109,19 @@ int orc_element_elasticity(const orc_system *s, int npts, double E, double nu, d
 void orc_cg_eigen(int64_t n, const int32_t *row_ptr, const int32_t *col_idx, const double *vals,
                   const double *b, double *x, double tol, int64_t maxits, int64_t *iters,
                   double *rel_err);
-/* Laspack CGIter + JacobiPrecond restated (contrib/laspack/itersolv.C:377-476,
- * precond.C:26-46, rtc.C:48-72); x zeroed first as laspack_linear_solver.C:130. */
 /* Eigen BiCGSTAB + DiagonalPreconditioner (BiCGSTAB.h:28-112), restated. */
 void orc_bicgstab_eigen(int64_t n, const int32_t *row_ptr, const int32_t *col_idx, const double *vals,
                         const double *b, double *x, double tol, int64_t maxits, int64_t *iters, double *rel_err);
+/* The same two with the preconditioner chosen: precond 1 JACOBI_PRECOND,
+ * 0 IDENTITY_PRECOND (Eigen IdentityPreconditioner: z = r). */
+void orc_cg_eigen_pc(int64_t n, const int32_t *row_ptr, const int32_t *col_idx, const double *vals,
+                     const double *b, double *x, double tol, int64_t maxits, int precond, int64_t *iters,
+                     double *rel_err);
+void orc_bicgstab_eigen_pc(int64_t n, const int32_t *row_ptr, const int32_t *col_idx, const double *vals,
+                           const double *b, double *x, double tol, int64_t maxits, int precond, int64_t *iters,
+                           double *rel_err);
+/* Laspack CGIter + JacobiPrecond restated (contrib/laspack/itersolv.C:377-476,
+ * precond.C:26-46, rtc.C:48-72); x zeroed first as laspack_linear_solver.C:130. */
 void orc_cg_laspack(int64_t n, const int32_t *row_ptr, const int32_t *col_idx, const double *vals,
                     const double *b, double *x, double tol, int64_t maxits, int64_t *iters,
                     double *rel_res);

@@ -20,6 +20,7 @@ _ref = None
 _P = C.c_void_p
 _I64 = C.c_int64
 _D = C.c_double
+_I = C.c_int
 
 
 def _ptr(a):
@@ -71,6 +72,8 @@ def lib():
             "orc_cg_eigen": (None, [_I64, _P, _P, _P, _P, _P, _D, _I64, _P, _P]),
             "orc_cg_laspack": (None, [_I64, _P, _P, _P, _P, _P, _D, _I64, _P, _P]),
             "orc_bicgstab_eigen": (None, [_I64, _P, _P, _P, _P, _P, _D, _I64, _P, _P]),
+            "orc_cg_eigen_pc": (None, [_I64, _P, _P, _P, _P, _P, _D, _I64, _I, _P, _P]),
+            "orc_bicgstab_eigen_pc": (None, [_I64, _P, _P, _P, _P, _P, _D, _I64, _I, _P, _P]),
             "orc_spmv": (None, [_I64, _P, _P, _P, _P, _P]),
             "orc_qoi_box_derivative": (C.c_int, [_P, C.c_int, _D, _D, _P]),
             "orc_qoi_box": (_D, [_P, C.c_int, _D, _D, _P]),
@@ -101,6 +104,9 @@ def ref():
         R.ref_eigen_cg.argtypes = [_I64, _P, _P, _P, _P, _P, _D, _I64, _P, _P]
         R.ref_eigen_bicgstab.restype = C.c_int
         R.ref_eigen_bicgstab.argtypes = [_I64, _P, _P, _P, _P, _P, _D, _I64, _P, _P]
+        for name in ("ref_eigen_cg_identity", "ref_eigen_bicgstab_identity"):
+            getattr(R, name).restype = C.c_int
+            getattr(R, name).argtypes = [_I64, _P, _P, _P, _P, _P, _D, _I64, _P, _P]
         R.ref_eigen_spmv.restype = None
         R.ref_eigen_spmv.argtypes = [_I64, _P, _P, _P, _P, _P]
         _ref = R
@@ -320,30 +326,39 @@ def _cg(fn, rp, ci, vals, b, tol, maxits, x0=None):
     return x, its.value, err.value
 
 
-def cg_eigen(rp, ci, vals, b, tol=1e-12, maxits=5000, x0=None):
-    return _cg(lib().orc_cg_eigen, rp, ci, vals, b, tol, maxits, x0)
+PRECOND = {"jacobi": 1, "identity": 0}  # libMesh JACOBI_PRECOND / IDENTITY_PRECOND
+
+
+def _pc(fn, precond):
+    return lambda n, *a: fn(n, *a[:7], PRECOND[precond], *a[7:])
+
+
+def cg_eigen(rp, ci, vals, b, tol=1e-12, maxits=5000, x0=None, precond="jacobi"):
+    return _cg(_pc(lib().orc_cg_eigen_pc, precond), rp, ci, vals, b, tol, maxits, x0)
 
 
 def cg_laspack(rp, ci, vals, b, tol=1e-12, maxits=5000):
     return _cg(lib().orc_cg_laspack, rp, ci, vals, b, tol, maxits)
 
 
-def bicgstab_eigen(rp, ci, vals, b, tol=1e-12, maxits=5000, x0=None):
-    return _cg(lib().orc_bicgstab_eigen, rp, ci, vals, b, tol, maxits, x0)
+def bicgstab_eigen(rp, ci, vals, b, tol=1e-12, maxits=5000, x0=None, precond="jacobi"):
+    return _cg(_pc(lib().orc_bicgstab_eigen_pc, precond), rp, ci, vals, b, tol, maxits, x0)
 
 
-def ref_bicgstab_eigen(rp, ci, vals, b, tol=1e-12, maxits=5000, x0=None):
+def ref_bicgstab_eigen(rp, ci, vals, b, tol=1e-12, maxits=5000, x0=None, precond="jacobi"):
     R = ref()
     if R is None:
         return None
-    return _cg(R.ref_eigen_bicgstab, rp, ci, vals, b, tol, maxits, x0)
+    fn = R.ref_eigen_bicgstab if precond == "jacobi" else R.ref_eigen_bicgstab_identity
+    return _cg(fn, rp, ci, vals, b, tol, maxits, x0)
 
 
-def ref_cg_eigen(rp, ci, vals, b, tol=1e-12, maxits=5000, x0=None):
+def ref_cg_eigen(rp, ci, vals, b, tol=1e-12, maxits=5000, x0=None, precond="jacobi"):
     R = ref()
     if R is None:
         return None
-    return _cg(R.ref_eigen_cg, rp, ci, vals, b, tol, maxits, x0)
+    fn = R.ref_eigen_cg if precond == "jacobi" else R.ref_eigen_cg_identity
+    return _cg(fn, rp, ci, vals, b, tol, maxits, x0)
 
 
 def spmv(rp, ci, vals, x):

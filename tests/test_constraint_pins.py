@@ -257,21 +257,23 @@ def test_fem_residual(t, n):
     assert np.abs(R0 - (A @ u - b)).max() <= 1e-13 * scale
 
 
+@pytest.mark.parametrize("precond", ["jacobi", "identity"])
 @pytest.mark.parametrize("name,t,n,mask", CASES)
-def test_bicgstab_restatement(name, t, n, mask):
+def test_bicgstab_restatement(name, t, n, mask, precond):
     """The BiCGSTAB restatement against the reference's own vendored Eigen
-    3.3.9 BiCGSTAB (oracle/_ref) and a direct solve, on the asymmetric
+    3.3.9 BiCGSTAB (oracle/_ref; DiagonalPreconditioner or
+    IdentityPreconditioner) and a direct solve, on the asymmetric
     constraint-row systems that CG cannot solve."""
     m, s = _build(name, t, n, mask, asym=True)
     npts = 3 if t == O.HEX27 else 2
     kappa = O.kappa_lognormal(m.n_elem, 3)
     vals, rhs = s.assemble_poisson(npts, kappa, 1)
     rp, ci = s.pattern()
-    x, its, err = O.bicgstab_eigen(rp, ci, vals, rhs, 1e-12, 5000)
+    x, its, err = O.bicgstab_eigen(rp, ci, vals, rhs, 1e-12, 5000, precond=precond)
     assert err <= 1e-12
     xd = spla.spsolve(sp.csr_matrix((vals, ci, rp), shape=(s.n_dofs,) * 2).tocsc(), rhs)
     assert np.abs(x - xd).max() <= 1e-9 * np.abs(xd).max()
-    ref = O.ref_bicgstab_eigen(rp, ci, vals, rhs, 1e-12, 5000)
+    ref = O.ref_bicgstab_eigen(rp, ci, vals, rhs, 1e-12, 5000, precond=precond)
     if ref is None:
         pytest.skip("oracle/_ref not built (needs the reference tree)")
     xr, itr, errr = ref

@@ -164,3 +164,19 @@ def test_equation_systems_write_read_dropin(tmp_path):
     b = parse_xdr(tmp_path / "es.xdr")
     for sa, sb in zip(a["systems"], b["systems"]):
         assert sa["vars"] == sb["vars"] and all(np.array_equal(d, e) for d, e in zip(sa["data"], sb["data"]))
+
+
+def test_poisson_identity_preconditioner_dropin(tmp_path):
+    """LinearSolver::set_preconditioner_type(IDENTITY_PRECOND) from C++: the
+    solution of the oracle's identity-preconditioned Eigen CG restatement."""
+    out = run(["poisson_dropin", 12, 10, tmp_path / "id", "perturb", "kappa", "sine", "identity"], tmp_path)
+    u = np.fromfile(tmp_path / "id_u.bin")
+    m = O.Mesh(12, O.HEX8)
+    m.perturb(0x5EED0001, 0.2)
+    kappa = O.kappa_lognormal(m.n_elem, 0x5EED0002)
+    s = O.System(m)
+    v, b = s.assemble_poisson(2, kappa, 1)
+    rp, ci = s.pattern()
+    x, its, _ = O.cg_eigen(rp, ci, v, b, 1e-10, 5000, precond="identity")
+    assert its > 20 and abs(int(out.split("iterations ")[1].split()[0]) - its) <= 2, (its, out)
+    assert np.abs(u - x).max() <= 1e-9 * np.abs(x).max(), out

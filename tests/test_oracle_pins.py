@@ -291,13 +291,15 @@ def _c1_system(n):
 
 
 @pytest.mark.skipif(O.ref() is None, reason="oracle/_ref not built (needs the reference tree)")
-def test_cg_matches_reference_eigen():
+@pytest.mark.parametrize("precond", ["jacobi", "identity"])
+def test_cg_matches_reference_eigen(precond):
     """The restated CG against the reference's own vendored Eigen 3.3.9
-    ConjugateGradient + DiagonalPreconditioner on the same CSR."""
+    ConjugateGradient with DiagonalPreconditioner (JACOBI_PRECOND) or
+    IdentityPreconditioner (IDENTITY_PRECOND) on the same CSR."""
     m, s, kappa, v, b, rp, ci = _c1_system(6)
     for tol in (1e-6, 1e-10, 1e-12):
-        x, its, err = O.cg_eigen(rp, ci, v, b, tol, 5000)
-        xr, itr, errr = O.ref_cg_eigen(rp, ci, v, b, tol, 5000)
+        x, its, err = O.cg_eigen(rp, ci, v, b, tol, 5000, precond=precond)
+        xr, itr, errr = O.ref_cg_eigen(rp, ci, v, b, tol, 5000, precond=precond)
         assert abs(its - itr) <= 1
         assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 50 * tol
         assert err < tol and errr < tol
