@@ -227,6 +227,15 @@ int lmh_cg_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int
  * vanishes relative to |r0|^2 is reported as LMH_DIVERGED_BREAKDOWN. */
 int lmh_bicgstab_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int64_t maxits, int64_t *iters,
                        double *rel_res, int *reason);
+/* The same BiCGSTAB on the transpose, A^T x = b: what LinearSolver::
+ * adjoint_solve does for BICGSTAB (src/solvers/linear_solver.C:147-171, the
+ * explicit transpose of eigen_sparse_linear_solver.C:331-350).  The pattern
+ * must be structurally symmetric (every (i, j) stored with its (j, i), as the
+ * constraint-coupled finite-element patterns are; otherwise
+ * LMH_ERR_UNSUPPORTED); A^T then has A's pattern with mirrored values, which
+ * are regathered on the device at each call. */
+int lmh_bicgstab_solve_transpose(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int64_t maxits,
+                                 int64_t *iters, double *rel_res, int *reason);
 
 /* Mixed precision: FP32 Jacobi-CG on the FP32 operator inside FP64 refinement
  * of the FP64 residual r = b - A x.  mode 0: reliable updates (the FP32

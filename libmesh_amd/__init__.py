@@ -96,6 +96,7 @@ HIP_API = {
     "lmh_matrix_set_fp32_compression": (_I, [_P, _I]),
     "lmh_matrix_fp32_compression": (_I, [_P]),
     "lmh_matrix_set_preconditioner": (_I, [_P, _I]),
+    "lmh_bicgstab_solve_transpose": (_I, [_P, _P, _P, _D, _I64, C.POINTER(_I64), C.POINTER(_D), C.POINTER(_I)]),
     "lmh_qoi_box_derivative": (_I, [_P, _P, _I, _D, _D, _P]),
     "lmh_adjoint_kappa_gradient": (_I, [_P, _P, _I, _P, _P, _P, _P]),
     "lmh_batch_create": (_I, [_P, _I, _P, C.POINTER(_P)]),
@@ -466,12 +467,12 @@ class GpuProblem:
                                  C.byref(reason)))
         return its.value, res.value, reason.value
 
-    def bicgstab(self, b_d, x_d, rtol, maxits=5000):
+    def bicgstab(self, b_d, x_d, rtol, maxits=5000, transpose=False):
         its = _I64()
         res = _D()
         reason = _I()
-        check(hip().lmh_bicgstab_solve(self.matrix, b_d.ptr, x_d.ptr, rtol, maxits, C.byref(its), C.byref(res),
-                                       C.byref(reason)))
+        f = hip().lmh_bicgstab_solve_transpose if transpose else hip().lmh_bicgstab_solve
+        check(f(self.matrix, b_d.ptr, x_d.ptr, rtol, maxits, C.byref(its), C.byref(res), C.byref(reason)))
         return its.value, res.value, reason.value
 
     def make_fp32(self):

@@ -162,6 +162,12 @@ struct lmh_matrix {
   float *vals32 = nullptr;
   bool have32 = false;
   int precond = LMH_PRECOND_JACOBI;  // preconditioner of the solvers on this matrix
+  // transpose solves of a structurally symmetric pattern: mirror position of
+  // each entry (0 not built, 1 built, -1 pattern not symmetric) and the
+  // mirrored values
+  int tpos_state = 0;
+  int32_t *tpos = nullptr;
+  double *vals_t = nullptr;
   // FP32 operator column compression (x windows): each row block lists its
   // distinct columns once, sorted, as 16-bit offsets woff16[w0 .. w0+nu),
   // col = off + (off <= s0 ? b0 : base1); every entry stores the 16-bit index

@@ -1270,12 +1270,14 @@ int lmh_cg_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int
   return LMH_OK;
 }
 
-int lmh_bicgstab_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int64_t maxits, int64_t *iters,
-                       double *rel_res, int *reason) {
-  LMH_CHECK(A && b_d && x_d, LMH_ERR_INVALID, "null argument");
-  LMH_CHECK(rtol >= 0 && maxits >= 0, LMH_ERR_INVALID, "bad tolerance or iteration limit");
-  LMH_CHECK(A->rows == A->cols, LMH_ERR_INVALID, "BiCGSTAB needs a square matrix");
-  LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
+}  // extern "C"
+
+namespace lmh {
+
+// Jacobi/identity BiCGSTAB on A's pattern with the values `vals` (A's own, or
+// the mirrored values of a structurally symmetric A, i.e. A transposed).
+static int bicgstab_impl(lmh_matrix *A, const double *vals, const double *b_d, double *x_d, double rtol,
+                         int64_t maxits, int64_t *iters, double *rel_res, int *reason) {
   lmh_context *ctx = A->ctx;
   const int64_t n = A->rows;
   if (!A->wbcg) LMH_HIP(hipMalloc((void **)&A->wbcg, sizeof(double) * 9 * std::max<int64_t>(1, n)));
@@ -1286,8 +1288,8 @@ int lmh_bicgstab_solve(lmh_matrix *A, const double *b_d, double *x_d, double rto
   const int g = vec_grid(ctx), gs = spmv_grid(ctx);
   LMH_TRY(ensure_diag(A));
   LMH_LAUNCH(ctx, "invdiag", (k_invdiag<double, double>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, n, A->diag,
-             A->vals, A->precond == LMH_PRECOND_JACOBI, dinv);
-  LMH_LAUNCH(ctx, "residual_fp64", k_residual, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals, b_d,
+             vals, A->precond == LMH_PRECOND_JACOBI, dinv);
+  LMH_LAUNCH(ctx, "residual_fp64", k_residual, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, vals, b_d,
              x_d, r, part2);
   LMH_LAUNCH(ctx, "bcg_init", k_bcg_init, dim3(g), dim3(BLK), 0, n, (const double *)r, r0, p, v, part1);
   LMH_LAUNCH(ctx, "bcg_init", k_bcg_init_fin, dim3(g), dim3(BLK), 0, part2, gs, rtol, S, x_d, n);
@@ -1301,10 +1303,10 @@ int lmh_bicgstab_solve(lmh_matrix *A, const double *b_d, double *x_d, double rto
   auto iteration = [&](int bank) {
     const int nb = bank ^ 1;
     LMH_LAUNCH(ctx, "bcg_p", k_bcg_p, dim3(g), dim3(BLK), 0, n, part1, g, dinv, r, v, p, y, S, bank);
-    LMH_LAUNCH(ctx, "bcg_spmv_fp64", k_bcg_spmv_v, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals, y,
+    LMH_LAUNCH(ctx, "bcg_spmv_fp64", k_bcg_spmv_v, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, vals, y,
                r0, v, part1, S, nb);
     LMH_LAUNCH(ctx, "bcg_s", k_bcg_s, dim3(g), dim3(BLK), 0, n, part1, gs, dinv, r, v, sv, z, S, nb);
-    LMH_LAUNCH(ctx, "bcg_spmv_fp64", k_bcg_spmv_t, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals, z,
+    LMH_LAUNCH(ctx, "bcg_spmv_fp64", k_bcg_spmv_t, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, vals, z,
                sv, tv, part2, S, nb);
     LMH_LAUNCH(ctx, "bcg_update", k_bcg_update, dim3(g), dim3(BLK), 0, n, part2, gs, y, z, sv, tv, r0, x_d, r, part1,
                S, nb);
@@ -1339,6 +1341,75 @@ int lmh_bicgstab_solve(lmh_matrix *A, const double *b_d, double *x_d, double rto
   if (rel_res) *rel_res = gbb > 0 ? std::sqrt(Bk[B_RR] / gbb) : 0.0;
   if (reason) *reason = conv ? (int)Bk[B_REASON] : LMH_DIVERGED_ITS;
   return LMH_OK;
+}
+
+// Position of each entry's mirror: tpos[k] = position of (c, r) for entry
+// k = (r, c), found by binary search in row c; *bad is raised when a mirror is
+// missing (the pattern is not structurally symmetric).
+__global__ __launch_bounds__(256) void k_transpose_pos(int64_t n, const int32_t *__restrict__ rp,
+                                                       const int32_t *__restrict__ ci, int32_t *__restrict__ tpos,
+                                                       int *__restrict__ bad) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+    for (int32_t k = rp[r]; k < rp[r + 1]; ++k) {
+      const int32_t c = ci[k];
+      int32_t lo = rp[c], hi = rp[c + 1];
+      while (lo < hi) {
+        const int32_t mid = lo + (hi - lo) / 2;
+        if (ci[mid] < r)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      const bool found = lo < rp[c + 1] && ci[lo] == r;
+      tpos[k] = found ? lo : k;
+      if (!found) atomicOr(bad, 1);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gather_mirror(int64_t nnz, const int32_t *__restrict__ tpos,
+                                                       const double *__restrict__ v, double *__restrict__ vt) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x)
+    vt[k] = v[tpos[k]];
+}
+
+}  // namespace lmh
+
+extern "C" {
+
+int lmh_bicgstab_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int64_t maxits, int64_t *iters,
+                       double *rel_res, int *reason) {
+  LMH_CHECK(A && b_d && x_d, LMH_ERR_INVALID, "null argument");
+  LMH_CHECK(rtol >= 0 && maxits >= 0, LMH_ERR_INVALID, "bad tolerance or iteration limit");
+  LMH_CHECK(A->rows == A->cols, LMH_ERR_INVALID, "BiCGSTAB needs a square matrix");
+  LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
+  return bicgstab_impl(A, A->vals, b_d, x_d, rtol, maxits, iters, rel_res, reason);
+}
+
+int lmh_bicgstab_solve_transpose(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int64_t maxits,
+                                 int64_t *iters, double *rel_res, int *reason) {
+  LMH_CHECK(A && b_d && x_d, LMH_ERR_INVALID, "null argument");
+  LMH_CHECK(rtol >= 0 && maxits >= 0, LMH_ERR_INVALID, "bad tolerance or iteration limit");
+  LMH_CHECK(A->rows == A->cols, LMH_ERR_INVALID, "BiCGSTAB needs a square matrix");
+  LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
+  lmh_context *ctx = A->ctx;
+  if (A->tpos_state == 0) {
+    LMH_HIP(hipMalloc((void **)&A->tpos, sizeof(int32_t) * std::max<int64_t>(1, A->nnz)));
+    LMH_HIP(hipMemsetAsync(ctx->d_flags, 0, sizeof(int), ctx->stream));
+    LMH_LAUNCH(ctx, "transpose_pos", k_transpose_pos, dim3(grid_for(A->rows, 256, 8192)), dim3(256), 0, A->rows,
+               A->rp, A->ci, A->tpos, ctx->d_flags);
+    LMH_TRY(check_launch("k_transpose_pos"));
+    int bad = 0;
+    LMH_HIP(hipMemcpyAsync(&bad, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    LMH_HIP(hipStreamSynchronize(ctx->stream));
+    A->tpos_state = bad ? -1 : 1;
+  }
+  LMH_CHECK(A->tpos_state == 1, LMH_ERR_UNSUPPORTED,
+            "transpose solve needs a structurally symmetric pattern (every (i, j) with its (j, i))");
+  if (!A->vals_t) LMH_HIP(hipMalloc((void **)&A->vals_t, sizeof(double) * (A->nnz + LMH_PAD)));
+  LMH_LAUNCH(ctx, "gather_mirror", k_gather_mirror, dim3(grid_for(A->nnz, 256, 16384)), dim3(256), 0, A->nnz,
+             A->tpos, A->vals, A->vals_t);
+  LMH_TRY(check_launch("k_gather_mirror"));
+  return bicgstab_impl(A, A->vals_t, b_d, x_d, rtol, maxits, iters, rel_res, reason);
 }
 
 int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, double inner_rtol, int mode,

@@ -357,6 +357,8 @@ int lmh_matrix_destroy(lmh_matrix *A) {
   (void)hipFree(A->rbmeta);
   (void)hipFree(A->vals);
   (void)hipFree(A->vals32);
+  (void)hipFree(A->tpos);
+  (void)hipFree(A->vals_t);
   (void)hipFree(A->off);
   (void)hipFree(A->diag);
   (void)hipFree(A->color_elems);

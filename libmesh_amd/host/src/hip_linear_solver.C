@@ -75,10 +75,35 @@ template <typename T>
 std::pair<unsigned int, Real> HipLinearSolver<T>::adjoint_solve(SparseMatrix<T> &matrix, NumericVector<T> &solution,
                                                                 NumericVector<T> &rhs, std::optional<double> tol,
                                                                 std::optional<unsigned int> m_its) {
-  // the system matrices of this path are symmetric for CG; BiCGSTAB serves the
-  // nonsymmetric ones, whose transpose solve is not provided
-  if (this->_solver_type == BICGSTAB) libmesh_not_implemented();
-  return solve(matrix, solution, rhs, tol, m_its);
+  // CG: the system matrices are symmetric, so A^T = A.  BiCGSTAB serves the
+  // nonsymmetric ones (asymmetric constraint rows): A^T x = b on A's
+  // structurally symmetric pattern with mirrored values, what
+  // LinearSolver::adjoint_solve's transpose-solve-transpose does
+  // (src/solvers/linear_solver.C:147-171).
+  if (this->_solver_type != BICGSTAB) return solve(matrix, solution, rhs, tol, m_its);
+  this->init();
+  auto *A = dynamic_cast<HipCsrMatrix<T> *>(&matrix);
+  auto *x = dynamic_cast<HipVector<T> *>(&solution);
+  auto *b = dynamic_cast<HipVector<T> *>(&rhs);
+  if (!A || !x || !b) libmesh_error_msg("HipLinearSolver needs a HipCsrMatrix and HipVectors");
+  if (_mixed) libmesh_not_implemented();
+  A->close();
+  x->close();
+  b->close();
+  hip_check(lmh_matrix_set_preconditioner(A->device_matrix(), this->_preconditioner_type == IDENTITY_PRECOND
+                                                                  ? LMH_PRECOND_IDENTITY
+                                                                  : LMH_PRECOND_JACOBI),
+            "HipLinearSolver (preconditioner)");
+  int64_t its = 0;
+  double res = 0;
+  int reason = 0;
+  hip_check(lmh_bicgstab_solve_transpose(A->device_matrix(), (const double *)b->device_data(),
+                                         (double *)x->device_data(), tol.value_or(this->default_tolerance()),
+                                         m_its.value_or(this->default_max_iterations()), &its, &res, &reason),
+            "HipLinearSolver::adjoint_solve (BiCGSTAB)");
+  x->mark_device_modified();
+  _reason = to_reason(reason);
+  return std::make_pair((unsigned int)its, (Real)res);
 }
 
 template class HipLinearSolver<double>;

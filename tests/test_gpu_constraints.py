@@ -173,3 +173,41 @@ def test_bicgstab_limits(ctx):
     x.upload(np.ones(p.n_dofs))
     its, res, reason = g.bicgstab(z, x, 1e-10)
     assert its == 0 and reason == L.CONVERGED_RTOL and not x.download().any()
+
+
+@pytest.mark.parametrize("name,t,n,mask", CASES)
+def test_bicgstab_transpose_asymmetric_rows(ctx, name, t, n, mask):
+    """A^T x = b (LinearSolver::adjoint_solve with BICGSTAB) on the asymmetric
+    constraint-row system, whose pattern is structurally symmetric: against
+    the oracle's BiCGSTAB on the explicitly transposed CSR (iterations +-2,
+    solution 1e-8) and a direct solve of A^T."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    p, m, s, g = build(ctx, name, t, n, mask, symmetric=False)
+    kappa = L.lognormal(p.n_elem, SEED_KAPPA)
+    ov, orhs = s.assemble_poisson(NPTS[t], kappa, 1)
+    rp, ci = s.pattern()
+    A = sp.csr_matrix((ov, ci, rp), shape=(p.n_dofs,) * 2)
+    AT = A.T.tocsr()
+    AT.sort_indices()
+    assert np.array_equal(AT.indptr, rp) and np.array_equal(AT.indices, ci)
+    assert np.abs(AT.data - ov).max() > 0  # the rows really are asymmetric
+    rhs = np.random.default_rng(17).standard_normal(p.n_dofs)
+    ox, oits, _ = O.bicgstab_eigen(AT.indptr, AT.indices, AT.data, rhs, 1e-10, 5000)
+    kd = L.DeviceArray(ctx, p.n_elem, data=kappa)
+    b = L.DeviceArray(ctx, p.n_dofs)
+    g.assemble_poisson(NPTS[t], kd, 1, b, L.FP64, L.SCATTER_GATHER)
+    bt = L.DeviceArray(ctx, p.n_dofs, data=rhs)
+    x = L.DeviceArray(ctx, p.n_dofs)
+    x.zero()
+    its, res, reason = g.bicgstab(bt, x, 1e-10, transpose=True)
+    assert reason == L.CONVERGED_RTOL and res <= 1e-10 and abs(its - oits) <= 2, (its, oits)
+    xs = x.download()
+    assert rel(xs, ox) < 1e-8
+    assert rel(xs, spla.spsolve(AT.tocsc(), rhs)) < 1e-8
+    # the forward solve afterwards still sees A, not A^T
+    x.zero()
+    its, res, reason = g.bicgstab(bt, x, 1e-10)
+    fx, fits, _ = O.bicgstab_eigen(rp, ci, ov, rhs, 1e-10, 5000)
+    assert reason == L.CONVERGED_RTOL and abs(its - fits) <= 2 and rel(x.download(), fx) < 1e-8

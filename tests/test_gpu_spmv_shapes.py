@@ -120,3 +120,25 @@ def test_cg_and_mixed_on_short_rows(ctx):
         assert rel(x2.download(), ox) < 1e-7
     finally:
         L.hip().lmh_matrix_destroy(h)
+
+
+def test_transpose_solve_needs_symmetric_pattern(ctx):
+    """lmh_bicgstab_solve_transpose refuses a pattern that lacks a mirror entry."""
+    n = 64
+    A = sp.diags([np.ones(n - 1), 4.0 * np.ones(n)], [1, 0], format="csr")  # upper bidiagonal
+    h, A, rp, ci, v = make(ctx, A)
+    try:
+        b = L.DeviceArray(ctx, n, data=np.ones(n))
+        x = L.DeviceArray(ctx, n)
+        x.zero()
+        its, res, reason = C.c_int64(), C.c_double(), C.c_int()
+        with pytest.raises(L.LmhError, match="structurally symmetric"):
+            L.check(L.hip().lmh_bicgstab_solve_transpose(h, b.ptr, x.ptr, 1e-10, 100, C.byref(its), C.byref(res),
+                                                         C.byref(reason)))
+        # the forward solve of the same matrix is unaffected
+        L.check(L.hip().lmh_bicgstab_solve(h, b.ptr, x.ptr, 1e-12, 100, C.byref(its), C.byref(res),
+                                           C.byref(reason)))
+        assert reason.value == L.CONVERGED_RTOL
+        assert rel(x.download(), np.linalg.solve(A.toarray(), np.ones(n))) < 1e-10
+    finally:
+        L.hip().lmh_matrix_destroy(h)
