@@ -492,7 +492,7 @@ __global__ __launch_bounds__(BLK) void k_it_spmv(const int32_t *__restrict__ rb,
   publish<1>(acc, part);
 }
 
-// q = A32 p on the compressed-column FP32 operator; partials (p, q).  Held to
+// q = A32 p on the x-window FP32 operator; partials (p, q).  Held to
 // 64 VGPRs (8 waves per SIMD): the pipelined loop needs every resident wave.
 __global__ __launch_bounds__(BLK, 8) void k_it_spmv16(const int32_t *__restrict__ rb, int64_t nrb,
                                                    const int32_t *__restrict__ rp, const uint16_t *__restrict__ widx,
@@ -876,7 +876,7 @@ static int spmv_grid(lmh_context *ctx) {
 
 // x windows of the FP32 operator, built once per pattern: window sizes, their
 // prefix sums on the host, then offsets and indices
-static int ensure_off16(lmh_matrix *A) {
+static int ensure_windows(lmh_matrix *A) {
   if (A->idx16_state != 0 || !A->fp32_index) return LMH_OK;
   lmh_context *ctx = A->ctx;
   const int64_t nrb = A->nrb;
@@ -934,7 +934,7 @@ static int ensure_diag(lmh_matrix *A) {
   return check_launch("k_diag_index");
 }
 
-static bool use_off16(const lmh_matrix *A) { return A->fp32_index && A->idx16_state == 1; }
+static bool use_windows(const lmh_matrix *A) { return A->fp32_index && A->idx16_state == 1; }
 
 static int ensure_work(lmh_matrix *A, bool mixed) {
   if (!A->w64) LMH_HIP(hipMalloc((void **)&A->w64, sizeof(double) * 4 * A->rows));
@@ -963,8 +963,8 @@ int lmh_spmv_fp32(lmh_matrix *A, const float *x_d, float *y_d) {
   LMH_CHECK(A->have32 && A->vals32, LMH_ERR_INVALID, "matrix has no FP32 operator");
   lmh_context *ctx = A->ctx;
   const int g = spmv_grid(ctx);
-  LMH_TRY(ensure_off16(A));
-  if (use_off16(A)) {
+  LMH_TRY(ensure_windows(A));
+  if (use_windows(A)) {
     LMH_LAUNCH(ctx, "spmv_fp32", k_spmv16, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->widx16, A->woff16,
                A->rbmeta, A->vals32, x_d, y_d);
     return check_launch("k_spmv16");
@@ -1422,8 +1422,8 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
   LMH_CHECK(rtol > 0 && inner_rtol > 0 && inner_rtol < 1, LMH_ERR_INVALID, "bad tolerances");
   lmh_context *ctx = A->ctx;
   LMH_TRY(ensure_work(A, true));
-  LMH_TRY(ensure_off16(A));
-  const bool c16 = use_off16(A);
+  LMH_TRY(ensure_windows(A));
+  const bool c16 = use_windows(A);
   const int64_t n = A->rows;
   // xc: FP32 solution correction accumulated since the last refresh, folded
   // into the FP64 x before each refresh
@@ -1536,8 +1536,8 @@ int lmh_matrix_set_preconditioner(lmh_matrix *A, int type) {
 
 int lmh_matrix_fp32_compression(lmh_matrix *A) {
   if (!A) return -1;
-  if (ensure_off16(A) != LMH_OK) return -1;
-  return use_off16(A) ? 1 : 0;
+  if (ensure_windows(A) != LMH_OK) return -1;
+  return use_windows(A) ? 1 : 0;
 }
 
 }  // extern "C"
