@@ -1,8 +1,10 @@
 // es_io_dropin.C -- EquationSystems::write / read as a libMesh application
 // uses them for restart files: two systems on a perturbed HEX27 cube (a
 // LinearImplicitSystem "Poisson" with u, a basic System "aux" with a and b),
-// solutions set from the node coordinates, written as <prefix>.xda and
-// <prefix>.xdr, read back into fresh EquationSystems and compared.
+// solutions set from the node coordinates, and an additional vector "u_old" =
+// 2u + 1 of the Poisson system, written as <prefix>.xda and <prefix>.xdr with
+// WRITE_ADDITIONAL_DATA, read back with READ_ADDITIONAL_DATA into fresh
+// EquationSystems and compared.
 // Usage: es_io_dropin n out_prefix
 // Prints: nodes, DoFs, the largest difference after each round trip, and
 // whether a header that does not describe the systems is rejected.
@@ -25,7 +27,9 @@ Real f_a(const Point &p) { return p.x * p.y - p.z; }
 Real f_b(const Point &p) { return p.x * p.x - p.y * p.z + 0.125; }
 
 void declare(EquationSystems &es, const char *aux_var_b) {
-  es.add_system<LinearImplicitSystem>("Poisson").add_variable("u", SECOND, LAGRANGE);
+  LinearImplicitSystem &p = es.add_system<LinearImplicitSystem>("Poisson");
+  p.add_variable("u", SECOND, LAGRANGE);
+  p.add_vector("u_old", false);
   System &aux = es.add_system<System>("aux");
   aux.add_variable("a", SECOND, LAGRANGE);
   aux.add_variable(aux_var_b, SECOND, LAGRANGE);
@@ -40,10 +44,12 @@ void fill(EquationSystems &es, const Mesh &mesh) {
     for (unsigned k = 0; k < npe; ++k) {
       const Point x = mesh.point(mesh.node_id(e, k));
       p.solution->set(dp.elem_dofs()[(size_t)e * dp.dofs_per_elem() + k], f_u(x));
+      p.get_vector("u_old").set(dp.elem_dofs()[(size_t)e * dp.dofs_per_elem() + k], 2. * f_u(x) + 1.);
       aux.solution->set(da.elem_dofs()[(size_t)e * da.dofs_per_elem() + k], f_a(x));
       aux.solution->set(da.elem_dofs()[(size_t)e * da.dofs_per_elem() + npe + k], f_b(x));
     }
   p.solution->close();
+  p.get_vector("u_old").close();
   aux.solution->close();
 }
 
@@ -56,6 +62,11 @@ double max_diff(EquationSystems &a, EquationSystems &b) {
     if (x.size() != y.size()) return INFINITY;
     for (size_t i = 0; i < x.size(); ++i) d = std::fmax(d, std::fabs(x[i] - y[i]));
   }
+  std::vector<Number> x, y;
+  a.get_system<System>("Poisson").get_vector("u_old").localize(x);
+  b.get_system<System>("Poisson").get_vector("u_old").localize(y);
+  if (x.size() != y.size()) return INFINITY;
+  for (size_t i = 0; i < x.size(); ++i) d = std::fmax(d, std::fabs(x[i] - y[i]));
   return d;
 }
 
@@ -75,13 +86,16 @@ int main(int argc, char **argv) {
     EquationSystems es(mesh);
     declare(es, "b");
     fill(es, mesh);
-    es.write(prefix + ".xda", EquationSystems::WRITE_DATA);
-    es.write(prefix + ".xdr", EquationSystems::WRITE_DATA);
+    const unsigned wflags = EquationSystems::WRITE_DATA | EquationSystems::WRITE_ADDITIONAL_DATA;
+    es.write(prefix + ".xda", wflags);
+    es.write(prefix + ".xdr", wflags);
     EquationSystems from_xdr(mesh), from_xda(mesh);
     declare(from_xdr, "b");
     declare(from_xda, "b");
-    from_xdr.read(prefix + ".xdr", EquationSystems::READ_HEADER | EquationSystems::READ_DATA);
-    from_xda.read(prefix + ".xda");
+    const unsigned rflags =
+        EquationSystems::READ_HEADER | EquationSystems::READ_DATA | EquationSystems::READ_ADDITIONAL_DATA;
+    from_xdr.read(prefix + ".xdr", rflags);
+    from_xda.read(prefix + ".xda", rflags);
     int rejected = 0;
     try {
       EquationSystems other(mesh);

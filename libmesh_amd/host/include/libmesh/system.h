@@ -74,6 +74,24 @@ class System {
   std::unique_ptr<NumericVector<Number>> solution;
   NumericVector<Number> *rhs = nullptr;
 
+  // System::add_vector / get_vector / have_vector / n_vectors
+  // (src/systems/system.C:751-800, 893-960): named vectors sized like the
+  // solution (created now when the system is initialised, else by init());
+  // listed, with their projection flag and parallel type, in EquationSystems
+  // files written with WRITE_ADDITIONAL_DATA.
+  NumericVector<Number> &add_vector(const std::string &vec_name, bool projections = true,
+                                    ParallelType type = PARALLEL);
+  NumericVector<Number> &get_vector(const std::string &vec_name);
+  const NumericVector<Number> &get_vector(const std::string &vec_name) const;
+  bool have_vector(const std::string &vec_name) const { return _vectors.count(vec_name) != 0; }
+  unsigned int n_vectors() const { return (unsigned int)_vectors.size(); }
+  struct NamedVector {
+    std::unique_ptr<NumericVector<Number>> vec;
+    bool projections = true;
+    ParallelType type = PARALLEL;
+  };
+  const std::map<std::string, NamedVector> &vectors() const { return _vectors; }
+
  protected:
   virtual void init_matrices();
   EquationSystems &_es;
@@ -81,6 +99,7 @@ class System {
   unsigned _number;
   DofMap _dof_map;
   std::map<std::string, std::unique_ptr<SparseMatrix<Number>>> _matrices;
+  std::map<std::string, NamedVector> _vectors;  // name order, as libMesh's std::map
   std::unique_ptr<NumericVector<Number>> _rhs;
   void (*_assemble_fptr)(EquationSystems &, const std::string &) = nullptr;
   Assembly *_assemble_obj = nullptr;

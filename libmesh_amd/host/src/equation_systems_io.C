@@ -180,14 +180,26 @@ void EquationSystems::write(const std::string &name, XdrMODE mode, unsigned int 
       var.family = (int)dm.variable_type(v).family;
       s.vars.push_back(var);
     }
-    // the systems here hold no additional vectors, so WRITE_ADDITIONAL_DATA lists none
+    // additional vectors, name order, listed only with WRITE_ADDITIONAL_DATA (system_io.C:1277-1300)
+    if (write_flags & WRITE_ADDITIONAL_DATA)
+      for (const auto &v : sys.vectors()) {
+        XdaVector xv;
+        xv.name = v.first;
+        xv.projection = v.second.projections ? 1 : 0;
+        xv.parallel_type = (int)v.second.type;
+        s.vectors.push_back(std::move(xv));
+      }
     if (write_data) {
       if (!sys.solution) libmesh_error_msg("EquationSystems::write: system " << s.name << " is not initialised");
-      std::vector<Number> x;
-      sys.solution->localize(x);
       const std::vector<dof_id_type> order = sys.node_major_dofs();
-      s.solution.resize(order.size());
-      for (size_t i = 0; i < order.size(); ++i) s.solution[i] = x.at(order[i]);
+      auto node_major = [&](const NumericVector<Number> &vec, std::vector<Number> &out) {
+        std::vector<Number> x;
+        vec.localize(x);
+        out.resize(order.size());
+        for (size_t i = 0; i < order.size(); ++i) out[i] = x.at(order[i]);
+      };
+      node_major(*sys.solution, s.solution);
+      for (XdaVector &xv : s.vectors) node_major(sys.get_vector(xv.name), xv.values);
     }
     f.systems.push_back(std::move(s));
   }
@@ -211,8 +223,11 @@ void EquationSystems::read(const std::string &name, XdrMODE mode, unsigned int r
     io.close();
   }
   // systems are declared (variables added, init() called) before reading:
-  // the header must describe them; additional vectors in the file are read
-  // and dropped, as the systems here hold none
+  // the header must describe them.  Additional vectors follow
+  // System::read_serialized_data (system_io.C:533-600): with
+  // READ_ADDITIONAL_DATA a system that holds vectors must hold as many as the
+  // file and receives them in order (both in name order); a system without
+  // vectors, or a read without the flag, drops them.
   for (const XdaSystem &s : f.systems) {
     auto it = _systems.find(s.name);
     if (it == _systems.end())
@@ -232,11 +247,25 @@ void EquationSystems::read(const std::string &name, XdrMODE mode, unsigned int r
     if (!read_data) continue;
     if (!sys.solution) libmesh_error_msg("EquationSystems::read: system " << s.name << " is not initialised");
     const std::vector<dof_id_type> order = sys.node_major_dofs();
-    if (s.solution.size() != order.size())
-      libmesh_error_msg("EquationSystems::read: system '" << s.name << "' solution has " << s.solution.size()
-                                                          << " values in the file, " << order.size() << " DoFs here");
-    for (size_t i = 0; i < order.size(); ++i) sys.solution->set(order[i], s.solution[i]);
-    sys.solution->close();
+    auto fill = [&](NumericVector<Number> &vec, const std::vector<Number> &vals, const std::string &what) {
+      if (vals.size() != order.size())
+        libmesh_error_msg("EquationSystems::read: system '" << s.name << "' " << what << " has " << vals.size()
+                                                            << " values in the file, " << order.size()
+                                                            << " DoFs here");
+      for (size_t i = 0; i < order.size(); ++i) vec.set(order[i], vals[i]);
+      vec.close();
+    };
+    fill(*sys.solution, s.solution, "solution");
+    if ((read_flags & READ_ADDITIONAL_DATA) && sys.n_vectors()) {
+      if (sys.n_vectors() != s.vectors.size())
+        libmesh_error_msg("EquationSystems::read: additional vectors in file do not match system '" << s.name
+                                                                                                      << "'");
+      auto it = sys.vectors().begin();
+      for (const XdaVector &xv : s.vectors) {
+        fill(*it->second.vec, xv.values, "vector " + xv.name);
+        ++it;
+      }
+    }
   }
 }
 

@@ -42,7 +42,32 @@ void System::init() {
   solution = std::move(sol);
   _rhs = std::make_unique<HipVector<Number>>(_dof_map.n_dofs());
   rhs = _rhs.get();
+  for (auto &kv : _vectors)
+    if (kv.second.vec->size() != _dof_map.n_dofs()) kv.second.vec->init(_dof_map.n_dofs());
   init_matrices();
+}
+
+NumericVector<Number> &System::add_vector(const std::string &vec_name, bool projections, ParallelType type) {
+  auto it = _vectors.find(vec_name);
+  if (it != _vectors.end()) return *it->second.vec;  // libMesh returns the existing vector
+  NamedVector &nv = _vectors[vec_name];
+  nv.projections = projections;
+  nv.type = type;
+  // sized now when the DoFs are distributed, else by init()
+  nv.vec = std::make_unique<HipVector<Number>>(solution ? _dof_map.n_dofs() : 0);
+  return *nv.vec;
+}
+
+NumericVector<Number> &System::get_vector(const std::string &vec_name) {
+  auto it = _vectors.find(vec_name);
+  if (it == _vectors.end()) libmesh_error_msg("System::get_vector: no vector named " << vec_name);
+  return *it->second.vec;
+}
+
+const NumericVector<Number> &System::get_vector(const std::string &vec_name) const {
+  auto it = _vectors.find(vec_name);
+  if (it == _vectors.end()) libmesh_error_msg("System::get_vector: no vector named " << vec_name);
+  return *it->second.vec;
 }
 
 void System::init_matrices() {

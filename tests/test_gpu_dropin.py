@@ -136,7 +136,7 @@ def test_fem_system_newton_dropin(tmp_path):
 
 def test_equation_systems_write_read_dropin(tmp_path):
     """EquationSystems::write / read from C++ (es_io_dropin): two systems on a
-    perturbed 3^3 HEX27 cube.  The ASCII file's values, read here independently,
+    perturbed 3^3 HEX27 cube, one with an additional vector (WRITE_ / READ_ADDITIONAL_DATA).  The ASCII file's values, read here independently,
     are the coordinate polynomials the example set, in node-major order (nodes
     by id, each node's variables); the XDR file decodes to the same bits; both
     read back into fresh systems exactly; a header that does not describe the
@@ -154,9 +154,10 @@ def test_equation_systems_write_read_dropin(tmp_path):
     assert a["version"] == "libMesh-1.7.0"
     assert [(s["name"], s["number"], s["type"]) for s in a["systems"]] == [("Poisson", 0, "LinearImplicit"),
                                                                            ("aux", 1, "Basic")]
-    assert a["systems"][0]["vars"] == [("u", [], 2, 0)] and a["systems"][0]["vectors"] == []
-    assert a["systems"][1]["vars"] == [("a", [], 2, 0), ("b", [], 2, 0)]
+    assert a["systems"][0]["vars"] == [("u", [], 2, 0)] and a["systems"][0]["vectors"] == [("u_old", 0, 2)]
+    assert a["systems"][1]["vars"] == [("a", [], 2, 0), ("b", [], 2, 0)] and a["systems"][1]["vectors"] == []
     u = a["systems"][0]["data"][0]
+    assert np.abs(a["systems"][0]["data"][1] - (2 * u + 1)).max() <= 1e-15 * 16
     ab = a["systems"][1]["data"][0]
     assert np.abs(u - (1 + x + 2 * y + 3 * z)).max() <= 1e-15 * 7
     assert np.abs(ab[0::2] - (x * y - z)).max() <= 1e-15 * 2
