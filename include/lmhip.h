@@ -271,10 +271,19 @@ int lmh_matrix_set_preconditioner(lmh_matrix *A, int type);
 
 /* ---- adjoint sensitivity (ImplicitSystem::adjoint_qoi_parameter_sensitivity,
  * implicit_system.C:517-623, for per-element kappa parameters) ---- */
-/* dJ/du for J = integral of u over [lo,hi]^3 (per-qp inclusion), constrained
- * entries zero (System::assemble_qoi_derivative + enforce_adjoint_constraints_exactly). */
+/* dJ/du for J = integral of u over [lo,hi]^3 (per-qp inclusion), with the
+ * constraints applied as System::assemble_qoi_derivative(apply_constraints =
+ * true) does: C^T dq, constrained entries zero (constrain_element_vector,
+ * dof_map_constraints.C:2981-3020); for Dirichlet-only maps that is the
+ * constrained entries zeroed. */
 int lmh_qoi_box_derivative(const lmh_mesh *m, const lmh_dofmap *dm, int npts, double lo, double hi, double *dq_d);
-/* dJ/dkappa_e = -(1/kappa_e) lambda_e^T Kc_e u_e for every element, FP64. */
+/* dJ/dkappa_e = -(1/kappa_e) lambda_e^T Kc_e u_e for every element, FP64.
+ * With general constraint rows, u is the solution with its constraints
+ * enforced (lmh_enforce_constraints, heterogeneous) and lambda the adjoint
+ * with them enforced homogeneously (enforce_adjoint_constraints_exactly,
+ * implicit_system.C:230-240); the adjoint itself solves A^T lambda = dq
+ * (lmh_cg_solve for symmetric rows, lmh_bicgstab_solve_transpose for the
+ * default asymmetric rows). */
 int lmh_adjoint_kappa_gradient(const lmh_mesh *m, const lmh_dofmap *dm, int npts, const double *kappa_d,
                                const double *u_d, const double *lambda_d, double *grad_d);
 

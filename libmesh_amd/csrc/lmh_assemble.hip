@@ -830,7 +830,6 @@ int lmh_assemble_residual_poisson(lmh_matrix *A, const lmh_mesh *m, const lmh_do
 int lmh_qoi_box_derivative(const lmh_mesh *m, const lmh_dofmap *dm, int npts, double lo, double hi, double *dq_d) {
   LMH_CHECK(m && dm && dq_d, LMH_ERR_INVALID, "null argument");
   LMH_CHECK(dm->nv == 1, LMH_ERR_UNSUPPORTED, "QoI derivative needs one variable");
-  LMH_CHECK(dm->cr.n_general == 0, LMH_ERR_UNSUPPORTED, "QoI derivative with general constraint rows");
   lmh_context *ctx = m->ctx;
   const FETable *t = nullptr;
   LMH_TRY(fe_table(ctx, m->type, npts, &t));
@@ -848,7 +847,27 @@ int lmh_qoi_box_derivative(const lmh_mesh *m, const lmh_dofmap *dm, int npts, do
              dm->cons, dm->n_dofs);
   LMH_TRY(check_launch("qoi"));
   LMH_HIP(hipFreeAsync(Q, ctx->stream));
-  return read_flag(ctx, "lmh_qoi_box_derivative");
+  LMH_TRY(read_flag(ctx, "lmh_qoi_box_derivative"));
+  // general constraint rows: dq <- C^T dq, what assemble_qoi_derivative with
+  // apply_constraints = true gives (constrain_element_vector per element,
+  // R^T F with constrained entries zeroed, dof_map_constraints.C:2981-3020,
+  // summed): each constrained entry is
+  // distributed to the DoFs of its row, then zeroed.  Processed rows hold no
+  // chains; rows are applied in ascending DoF order on the host.
+  const lmh_constraints &c = dm->cr;
+  if (c.n_general > 0) {
+    std::vector<double> h((size_t)dm->n_dofs);
+    LMH_HIP(hipMemcpyAsync(h.data(), dq_d, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    LMH_HIP(hipStreamSynchronize(ctx->stream));
+    for (size_t r = 0; r < c.dof.size(); ++r) {
+      const double q = h[c.dof[r]];
+      for (int32_t k = c.ptr[r]; k < c.ptr[r + 1]; ++k) h[c.col[k]] += c.coef[k] * q;
+      h[c.dof[r]] = 0.0;
+    }
+    LMH_HIP(hipMemcpyAsync(dq_d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+    LMH_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return LMH_OK;
 }
 
 int lmh_adjoint_kappa_gradient(const lmh_mesh *m, const lmh_dofmap *dm, int npts, const double *kappa_d,
@@ -856,7 +875,10 @@ int lmh_adjoint_kappa_gradient(const lmh_mesh *m, const lmh_dofmap *dm, int npts
   (void)kappa_d;  // dJ/dkappa_e does not depend on kappa_e for a per-element constant kappa
   LMH_CHECK(m && dm && u_d && lambda_d && grad_d, LMH_ERR_INVALID, "null argument");
   LMH_CHECK(dm->nv == 1, LMH_ERR_UNSUPPORTED, "adjoint gradient needs one variable");
-  LMH_CHECK(dm->cr.n_general == 0, LMH_ERR_UNSUPPORTED, "adjoint gradient with general constraint rows");
+  // With general constraint rows, u and lambda are the constraint-enforced
+  // vectors (heterogeneous for u, homogeneous for lambda): -(C lambda)^T
+  // (K_e / kappa_e)(C u + H) is then the element term over the element's own
+  // DoFs, which the kernel forms; only empty-row DoFs are masked.
   lmh_context *ctx = m->ctx;
   const FETable *t = nullptr;
   LMH_TRY(fe_table(ctx, m->type, npts, &t));

@@ -114,7 +114,8 @@ class ImplicitSystem : public System {
   // ImplicitSystem::assemble zeroes matrix and rhs first (implicit_system.C:94-108)
   void assemble() override;
   // ImplicitSystem::adjoint_solve with a caller-assembled QoI derivative:
-  // K^T z = dq (implicit_system.C:196-240)
+  // K^T z = dq, then the constraints enforced homogeneously on z
+  // (implicit_system.C:196-240)
   std::pair<unsigned int, Real> adjoint_solve(NumericVector<Number> &z, NumericVector<Number> &dq);
   // ImplicitSystem::linear_solver (public, include/systems/implicit_system.h:328)
   mutable std::unique_ptr<LinearSolver<Number>> linear_solver;

@@ -102,8 +102,13 @@ void ImplicitSystem::assemble() {
 }
 
 std::pair<unsigned int, Real> ImplicitSystem::adjoint_solve(NumericVector<Number> &z, NumericVector<Number> &dq) {
-  return linear_solver->adjoint_solve(get_system_matrix(), z, dq, _es.linear_solver_tolerance,
-                                      _es.linear_solver_maximum_iterations);
+  const auto r = linear_solver->adjoint_solve(get_system_matrix(), z, dq, _es.linear_solver_tolerance,
+                                              _es.linear_solver_maximum_iterations);
+  // the linear solver may not have fit the constraints exactly:
+  // enforce_adjoint_constraints_exactly (implicit_system.C:230-240), the
+  // homogeneous form of the constraint rows
+  if (!_dof_map.added_constraint_rows().empty()) _dof_map.enforce_constraints_exactly(*this, &z, true);
+  return r;
 }
 
 void LinearImplicitSystem::solve() {

@@ -211,3 +211,71 @@ def test_bicgstab_transpose_asymmetric_rows(ctx, name, t, n, mask):
     its, res, reason = g.bicgstab(bt, x, 1e-10)
     fx, fits, _ = O.bicgstab_eigen(rp, ci, ov, rhs, 1e-10, 5000)
     assert reason == L.CONVERGED_RTOL and abs(its - fits) <= 2 and rel(x.download(), fx) < 1e-8
+
+
+@pytest.mark.parametrize("symmetric", [True, False])
+@pytest.mark.parametrize("name,t,n,mask", [("periodic_x", L.HEX27, 3, Z0 | Y0 | Y1 | Z1), ("random", L.TET10, 2, L.ALL_FACES)])
+def test_adjoint_gradient_with_constraint_rows(ctx, name, t, n, mask, symmetric):
+    """adjoint_qoi_parameter_sensitivity for per-element kappa with general
+    constraint rows: QoI derivative with C^T applied (constrain_element_vector),
+    adjoint A^T z = dq (CG for symmetric rows, transpose BiCGSTAB for the
+    default asymmetric ones), z enforced homogeneously, u heterogeneously.
+    Against the oracle's element terms on scipy solves (1e-8) and central
+    finite differences of J(kappa) = q . u(kappa) through the oracle's
+    constrained solves (1e-6 of the largest).  The box [0.2371, 0.7129]^3
+    keeps every quadrature point off its faces (on the unperturbed tet mesh,
+    points at exactly 0.25 or 0.75 would be in or out by rounding)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    p, m, s, g = build(ctx, name, t, n, mask, symmetric=symmetric)
+    s0 = O.System(m, 1, mask)  # the same mesh without rows: q and the element terms
+    npts = NPTS[t]
+    kappa = L.lognormal(p.n_elem, SEED_KAPPA)
+    kd = L.DeviceArray(ctx, p.n_elem, data=kappa)
+    b = L.DeviceArray(ctx, p.n_dofs)
+    g.assemble_poisson(npts, kd, 1, b, L.FP64, L.SCATTER_GATHER)
+    u = L.DeviceArray(ctx, p.n_dofs)
+    z = L.DeviceArray(ctx, p.n_dofs)
+    dq = L.DeviceArray(ctx, p.n_dofs)
+    grad = L.DeviceArray(ctx, p.n_elem)
+    u.zero()
+    z.zero()
+    its, res, reason = g.cg(b, u, 1e-13) if symmetric else g.bicgstab(b, u, 1e-13)
+    assert reason == L.CONVERGED_RTOL
+    g.enforce_constraints(u)
+    lo, hi = 0.2371, 0.7129
+    g.qoi_box_derivative(npts, lo, hi, dq)
+    its, res, reason = g.cg(dq, z, 1e-13) if symmetric else g.bicgstab(dq, z, 1e-13, transpose=True)
+    assert reason == L.CONVERGED_RTOL
+    g.enforce_constraints(z, homogeneous=True)
+    g.adjoint_kappa_gradient(npts, kd, u, z, grad)
+    gr = grad.download()
+
+    def solve(kap):
+        v, rhs = s.assemble_poisson(npts, kap, 1)
+        rp, ci = s.pattern()
+        A = sp.csr_matrix((v, ci, rp), shape=(p.n_dofs,) * 2)
+        return A, s.enforce_constraints(spla.spsolve(A.tocsc(), rhs))
+
+    q = s0.qoi_box_derivative(npts, lo, hi)
+    A, uo = solve(kappa)
+    dqo = q.copy()
+    for d, (row, val) in sorted(s.constraint_rows().items()):
+        for j, c in row.items():
+            dqo[j] += c * dqo[d]
+        dqo[d] = 0.0
+    assert rel(dq.download(), dqo) < 1e-13
+    zo = s.enforce_constraints(spla.spsolve(A.T.tocsc(), dqo), homogeneous=True)
+    go = s0.adjoint_exact(npts, kappa, uo, zo)
+    assert rel(gr, go) < 1e-8
+    fds = []
+    for e in (0, n - 1, p.n_elem // 2, p.n_elem - 1):
+        h = 1e-6 * kappa[e]
+        kp, km = kappa.copy(), kappa.copy()
+        kp[e] += h
+        km[e] -= h
+        fds.append((e, (q @ solve(kp)[1] - q @ solve(km)[1]) / (2 * h)))
+    scale = max(abs(f) for _, f in fds)
+    for e, f in fds:
+        assert abs(gr[e] - f) <= 1e-6 * scale, (e, gr[e], f)
