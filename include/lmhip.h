@@ -191,8 +191,10 @@ int lmh_assemble_poisson(lmh_matrix *A, const lmh_mesh *m, const lmh_dofmap *dm,
  * unconstrained element matrices, C^T applied for the constraint rows
  * (constrain_element_vector / constrain_element_matrix_and_vector,
  * dof_map_constraints.C:2439-2516: constrained entries 0), added in element
- * order into R_d (overwritten).  Uses the matrix's element buffers; general
- * constraint rows are not supported here. */
+ * order into R_d (overwritten).  Uses the matrix's element buffers.  With
+ * general constraint rows, u_d is the vector with its constraints enforced
+ * (C u + H, lmh_enforce_constraints) and R_d = C^T (K u - F), constrained
+ * entries 0. */
 int lmh_assemble_residual_poisson(lmh_matrix *A, const lmh_mesh *m, const lmh_dofmap *dm, int npts,
                                   const double *kappa_d, int f_kind, const double *u_d, double *R_d);
 /* Element matrices/vectors only (no scatter), constrained if constrain != 0:

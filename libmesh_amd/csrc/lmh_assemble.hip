@@ -638,6 +638,28 @@ static int read_flag(lmh_context *ctx, const char *what) {
   return LMH_OK;
 }
 
+// v <- C^T v for general constraint rows: constrain_element_vector per element
+// (R^T F with constrained entries zeroed, dof_map_constraints.C:2981-3020)
+// summed over elements.  Each constrained entry is distributed to the DoFs of
+// its row, then zeroed; processed rows hold no chains.  Applied on the host in
+// ascending DoF order (deterministic); a no-op without general rows, where the
+// element kernels' mask already zeroes the constrained entries.
+static int apply_constraint_transpose(lmh_context *ctx, const lmh_dofmap *dm, double *v_d) {
+  const lmh_constraints &c = dm->cr;
+  if (c.n_general == 0) return LMH_OK;
+  std::vector<double> h((size_t)dm->n_dofs);
+  LMH_HIP(hipMemcpyAsync(h.data(), v_d, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+  LMH_HIP(hipStreamSynchronize(ctx->stream));
+  for (size_t r = 0; r < c.dof.size(); ++r) {
+    const double q = h[c.dof[r]];
+    for (int32_t k = c.ptr[r]; k < c.ptr[r + 1]; ++k) h[c.col[k]] += c.coef[k] * q;
+    h[c.dof[r]] = 0.0;
+  }
+  LMH_HIP(hipMemcpyAsync(v_d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+  LMH_HIP(hipStreamSynchronize(ctx->stream));
+  return LMH_OK;
+}
+
 // Element matrices for a scalar Poisson problem into Ke (CT) and Fe (FP64).
 template <typename CT>
 static int element_poisson_impl(lmh_context *ctx, const lmh_mesh *m, const lmh_dofmap *dm, int npts,
@@ -809,7 +831,6 @@ int lmh_assemble_residual_poisson(lmh_matrix *A, const lmh_mesh *m, const lmh_do
                                   const double *kappa_d, int f_kind, const double *u_d, double *R_d) {
   LMH_CHECK(A && m && dm && u_d && R_d, LMH_ERR_INVALID, "null argument");
   LMH_CHECK(A->dm == dm && dm->mesh == m, LMH_ERR_INVALID, "matrix, dofmap and mesh do not belong together");
-  LMH_CHECK(dm->cr.n_general == 0, LMH_ERR_UNSUPPORTED, "residual assembly with general constraint rows");
   LMH_CHECK(dm->dpe <= 32, LMH_ERR_UNSUPPORTED, "residual assembly needs at most 32 DoFs per element");
   lmh_context *ctx = A->ctx;
   const int64_t ne = m->n_elem, dd = (int64_t)dm->dpe * dm->dpe;
@@ -824,7 +845,10 @@ int lmh_assemble_residual_poisson(lmh_matrix *A, const lmh_mesh *m, const lmh_do
   LMH_LAUNCH(ctx, "add_element_vectors", k_add_vec_gather, dim3(grid_for(dm->n_dofs, 256, 8192)), dim3(256), 0,
              dm->inc_ptr, dm->inc, dm->n_dofs, (const double *)A->Fe, R_d);
   LMH_TRY(check_launch("residual"));
-  return read_flag(ctx, "lmh_assemble_residual_poisson");
+  LMH_TRY(read_flag(ctx, "lmh_assemble_residual_poisson"));
+  // general rows: u holds the enforced values (C u + H), so the summed element
+  // residuals get C^T, as constrain_element_residual does per element
+  return apply_constraint_transpose(ctx, dm, R_d);
 }
 
 int lmh_qoi_box_derivative(const lmh_mesh *m, const lmh_dofmap *dm, int npts, double lo, double hi, double *dq_d) {
@@ -848,26 +872,8 @@ int lmh_qoi_box_derivative(const lmh_mesh *m, const lmh_dofmap *dm, int npts, do
   LMH_TRY(check_launch("qoi"));
   LMH_HIP(hipFreeAsync(Q, ctx->stream));
   LMH_TRY(read_flag(ctx, "lmh_qoi_box_derivative"));
-  // general constraint rows: dq <- C^T dq, what assemble_qoi_derivative with
-  // apply_constraints = true gives (constrain_element_vector per element,
-  // R^T F with constrained entries zeroed, dof_map_constraints.C:2981-3020,
-  // summed): each constrained entry is
-  // distributed to the DoFs of its row, then zeroed.  Processed rows hold no
-  // chains; rows are applied in ascending DoF order on the host.
-  const lmh_constraints &c = dm->cr;
-  if (c.n_general > 0) {
-    std::vector<double> h((size_t)dm->n_dofs);
-    LMH_HIP(hipMemcpyAsync(h.data(), dq_d, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
-    LMH_HIP(hipStreamSynchronize(ctx->stream));
-    for (size_t r = 0; r < c.dof.size(); ++r) {
-      const double q = h[c.dof[r]];
-      for (int32_t k = c.ptr[r]; k < c.ptr[r + 1]; ++k) h[c.col[k]] += c.coef[k] * q;
-      h[c.dof[r]] = 0.0;
-    }
-    LMH_HIP(hipMemcpyAsync(dq_d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, ctx->stream));
-    LMH_HIP(hipStreamSynchronize(ctx->stream));
-  }
-  return LMH_OK;
+  // what assemble_qoi_derivative with apply_constraints = true gives
+  return apply_constraint_transpose(ctx, dm, dq_d);
 }
 
 int lmh_adjoint_kappa_gradient(const lmh_mesh *m, const lmh_dofmap *dm, int npts, const double *kappa_d,

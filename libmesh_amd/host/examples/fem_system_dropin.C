@@ -3,10 +3,14 @@
 // Jacobian through FEMSystem::assembly), the way FEMSystem applications
 // (adjoints_ex1/ex2, fem_system_ex*) drive assembly and solve.
 // Usage: fem_system_dropin n elem_type(10|12|8|9) out_prefix [perturb] [kappa] [sine] [rtol=R] [adjoint]
+//                          [periodic] [bicgstab]
 // Writes <prefix>_u.bin; prints Newton and linear iteration counts.  `adjoint`:
 // adjoint_qoi_parameter_sensitivity of J = integral of u over [0.25,0.75]^3 for
 // every per-element kappa through the FEMSystem's Jacobian, written to
-// <prefix>_grad.bin.
+// <prefix>_grad.bin.  `periodic`: Dirichlet on the y and z faces only and the
+// rows u(x=1) = u(x=0) plus u = 0.3 at one x=0 face DoF from a
+// System::Constraint object, symmetric rows with CG; with `bicgstab` the
+// default asymmetric rows and BiCGSTAB.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -19,6 +23,7 @@
 #include "libmesh/hip_assembler.h"
 #include "libmesh/hip_csr_matrix.h"
 #include "libmesh/hip_linear_solver.h"
+#include "periodic_x.h"
 
 using namespace libMesh;
 
@@ -30,13 +35,15 @@ int main(int argc, char **argv) {
   const unsigned n = std::atoi(argv[1]);
   const ElemType type = (ElemType)std::atoi(argv[2]);
   const std::string prefix = argv[3];
-  bool perturb = false, kappa = false, sine = false, adjoint = false;
+  bool perturb = false, kappa = false, sine = false, adjoint = false, periodic = false, bicgstab = false;
   Real rtol = 1e-9;
   for (int a = 4; a < argc; ++a) {
     perturb |= !std::strcmp(argv[a], "perturb");
     kappa |= !std::strcmp(argv[a], "kappa");
     sine |= !std::strcmp(argv[a], "sine");
     adjoint |= !std::strcmp(argv[a], "adjoint");
+    periodic |= !std::strcmp(argv[a], "periodic");
+    bicgstab |= !std::strcmp(argv[a], "bicgstab");
     if (!std::strncmp(argv[a], "rtol=", 5)) rtol = std::atof(argv[a] + 5);
   }
   try {
@@ -47,9 +54,18 @@ int main(int argc, char **argv) {
     FEMSystem &sys = es.add_system<FEMSystem>("Laplace");
     const Order order = (type == HEX8 || type == TET4) ? FIRST : SECOND;
     sys.add_variable("u", order, LAGRANGE);
-    sys.get_dof_map().add_dirichlet_boundary(DirichletBoundary({0, 1, 2, 3, 4, 5}, {0}));
+    PeriodicX periodic_rows(sys, mesh, n);
+    if (periodic) {
+      sys.get_dof_map().add_dirichlet_boundary(DirichletBoundary({0, 1, 3, 5}, {0}));
+      sys.get_dof_map().set_error_on_constraint_loop(true);
+      sys.get_dof_map().set_asymmetric_constraint_rows(bicgstab);
+      sys.attach_constraint_object(periodic_rows);
+    } else {
+      sys.get_dof_map().add_dirichlet_boundary(DirichletBoundary({0, 1, 2, 3, 4, 5}, {0}));
+    }
     sys.add_matrix("System Matrix", std::make_unique<HipCsrMatrix<Number>>());
     sys.linear_solver = std::make_unique<HipLinearSolver<Number>>();
+    if (bicgstab) sys.linear_solver->set_solver_type(BICGSTAB);
     const FEType fe_type(order, LAGRANGE);
     const QGauss qrule(3, fe_type.default_quadrature_order());
     HipAssembler physics(sys, mesh, sys.get_dof_map(), fe_type, qrule, sine ? HipAssembler::SINE : HipAssembler::UNIT);

@@ -82,9 +82,10 @@ class FEMSystem : public ImplicitSystem {
   void init() override;
   // FEMSystem::assembly (fem_system.C:875-1010): residual at the current
   // solution into rhs, Jacobian into the system matrix, constrained as with
-  // constrain_in_solver (element constraint with symmetric rows; the Newton
-  // solver enforces constraints exactly).  General constraint rows are not
-  // supported on this path.
+  // constrain_in_solver (element constraint; the Newton solver enforces the
+  // constraints exactly).  With general constraint rows the residual is
+  // C^T (K u - F) at the enforced solution and the Jacobian C^T K C in the
+  // DofMap's row form (CG for symmetric rows, BICGSTAB for asymmetric).
   void assembly(bool get_residual, bool get_jacobian, bool apply_heterogeneous_constraints = false,
                 bool apply_no_constraints = false);
   // DifferentiableSystem::solve -> time_solver->solve()

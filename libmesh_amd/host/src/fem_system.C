@@ -25,8 +25,9 @@ void FEMSystem::assembly(bool get_residual, bool get_jacobian, bool apply_hetero
   // offered by the device kernels
   (void)apply_heterogeneous_constraints;
   if (apply_no_constraints) libmesh_not_implemented();
-  const auto &rows = get_dof_map().added_constraint_rows();
-  if (!rows.empty()) libmesh_error_msg("FEMSystem::assembly: general constraint rows are not supported on this path");
+  // general constraint rows: the Jacobian is C^T K C with the DofMap's row
+  // form, the residual C^T (K u - F) at the solution, whose constraints the
+  // Newton solver keeps enforced
   if (get_jacobian) _physics->assemble_jacobian();
   if (get_residual) {
     solution->close();

@@ -181,3 +181,33 @@ def test_poisson_identity_preconditioner_dropin(tmp_path):
     x, its, _ = O.cg_eigen(rp, ci, v, b, 1e-10, 5000, precond="identity")
     assert its > 20 and abs(int(out.split("iterations ")[1].split()[0]) - its) <= 2, (its, out)
     assert np.abs(u - x).max() <= 1e-9 * np.abs(x).max(), out
+
+
+@pytest.mark.parametrize("bicgstab", [False, True])
+def test_fem_system_periodic_constraints_dropin(tmp_path, bicgstab):
+    """FEMSystem + NewtonSolver with System::Constraint rows (u(x=1) = u(x=0)
+    plus a heterogeneous row): symmetric rows with CG, or the default
+    asymmetric rows with BiCGSTAB.  The oracle's constrained solution."""
+    from constraint_cases import Y0, Y1, Z0, Z1, periodic_rows, value_row_on
+
+    n = 4
+    flags = ["perturb", "kappa", "sine", "periodic", "rtol=1e-11"] + (["bicgstab"] if bicgstab else [])
+    out = run(["fem_system_dropin", n, 12, tmp_path / "fp"] + flags, tmp_path)
+    u = np.fromfile(tmp_path / "fp_u.bin")
+    m = O.Mesh(n, O.HEX27)
+    m.perturb(0x5EED0001, 0.2)
+    kappa = O.kappa_lognormal(m.n_elem, 0x5EED0002)
+    s = O.System(m, 1, Z0 | Y0 | Y1 | Z1)
+    taken = set()
+    args = (m.coords(), m.conn(), s.elem_dofs(), s.constrained())
+    rows = value_row_on(*args, taken, 0, 0.3) + periodic_rows(*args, n, (0,), taken)
+    for d, row, v in rows:
+        s.add_constraint_row(d, row, v)
+    s.set_asymmetric_constraint_rows(False)
+    s.process_constraints(error_on_loop=True)
+    v, b = s.assemble_poisson(3, kappa, 1)
+    rp, ci = s.pattern()
+    x, its, _ = O.cg_eigen(rp, ci, v, b, 1e-12, 20000)
+    x = s.enforce_constraints(x)
+    assert np.abs(u - x).max() <= 1e-8 * np.abs(x).max(), out
+    assert abs(u[rows[0][0]] - 0.3) <= 1e-12, out
