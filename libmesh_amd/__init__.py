@@ -27,6 +27,7 @@ SCATTER_COLORED, SCATTER_GATHER = 0, 1
 CONSTRAINT_FORBID_OVERWRITE, CONSTRAINT_ERROR_ON_LOOP, CONSTRAINT_SYMMETRIC_ROWS = 1, 2, 4
 ALL_FACES = 0x3F
 CONVERGED_RTOL, DIVERGED_ITS = 2, -3
+DIVERGED_BREAKDOWN, DIVERGED_NANORINF = -5, -9  # LMH_DIVERGED_* (include/lmhip.h)
 PRECOND_IDENTITY, PRECOND_JACOBI = 0, 1  # libMesh PreconditionerType values
 
 _P = C.c_void_p

@@ -743,6 +743,9 @@ __global__ __launch_bounds__(BLK) void k_cg_init_fin(const double *__restrict__ 
     B0[B_RR] = 0;
     B0[B_DONE] = 1;
     B0[B_REASON] = LMH_CONVERGED_RTOL;
+  } else if (!isfinite(rr) || !isfinite(bb)) {  // NaN/Inf in b or x: stop before iterating
+    B0[B_DONE] = 1;
+    B0[B_REASON] = LMH_DIVERGED_NANORINF;
   } else if (rr < thr) {
     B0[B_DONE] = 1;
     B0[B_REASON] = LMH_CONVERGED_RTOL;
@@ -803,6 +806,9 @@ __global__ __launch_bounds__(BLK) void k_refresh_fin(const double *__restrict__ 
     if (t[1] == 0.0 || rr <= S[G_THR]) {
       Sb[B_DONE] = 1;
       Sb[B_REASON] = LMH_CONVERGED_RTOL;
+    } else if (!isfinite(rr) || !isfinite(t[1])) {
+      Sb[B_DONE] = 1;
+      Sb[B_REASON] = LMH_DIVERGED_NANORINF;
     }
     return;
   }
@@ -1064,6 +1070,9 @@ __global__ __launch_bounds__(BLK) void k_bcg_init_fin(const double *__restrict__
     B0[B_RR] = 0.0;
     B0[B_DONE] = 1;
     B0[B_REASON] = LMH_CONVERGED_RTOL;
+  } else if (!isfinite(rr) || !isfinite(bb)) {
+    B0[B_DONE] = 1;
+    B0[B_REASON] = LMH_DIVERGED_NANORINF;
   }
 }
 

@@ -83,3 +83,34 @@ def test_zero_rhs(ctx):
     its, res, reason = g.cg(b, x, 1e-10, 100)
     assert its == 0 and reason == L.CONVERGED_RTOL
     assert not x.download().any()
+
+
+def test_solver_failure_reasons(ctx):
+    """Reasons the solvers report instead of a result: an iteration limit
+    (DIVERGED_ITS with the residual reached), a NaN in the right-hand side
+    (DIVERGED_NANORINF before the first iteration, as PETSc reports it), and a
+    zero operator (DIVERGED_BREAKDOWN: (p, Ap) = 0).  Every solve returns."""
+    p, s, g = pair(ctx, L.HEX27, 4)
+    kd = L.DeviceArray(ctx, p.n_elem, data=np.ones(p.n_elem))
+    b = L.DeviceArray(ctx, p.n_dofs)
+    g.assemble_poisson(3, kd, 1, b)
+    g.make_fp32()
+    x = L.DeviceArray(ctx, p.n_dofs)
+    x.zero()
+    its, res, reason = g.cg(b, x, 1e-14, 3)
+    assert its == 3 and reason == L.DIVERGED_ITS and res > 1e-14
+    x.zero()
+    its, res, reason, st = g.mixed(b, x, 1e-14, 0.1, 0, 3)
+    assert reason == L.DIVERGED_ITS and res > 1e-14
+    bh = b.download()
+    bh[p.n_dofs // 2] = np.nan
+    bn = L.DeviceArray(ctx, p.n_dofs, data=bh)
+    for solve in (lambda: g.cg(bn, x, 1e-10, 100), lambda: g.bicgstab(bn, x, 1e-10, 100),
+                  lambda: g.mixed(bn, x, 1e-10, 0.1, 0, 100)[:3]):
+        x.zero()
+        its, res, reason = solve()
+        assert reason == L.DIVERGED_NANORINF and its == 0, (its, reason)
+    g.set_values(np.zeros(g.nnz))
+    x.zero()
+    its, res, reason = g.cg(b, x, 1e-10, 100)
+    assert reason == L.DIVERGED_BREAKDOWN
