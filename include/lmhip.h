@@ -53,6 +53,7 @@ typedef struct lmh_mesh lmh_mesh;
 typedef struct lmh_dofmap lmh_dofmap;
 typedef struct lmh_matrix lmh_matrix;
 typedef struct lmh_batch lmh_batch;
+typedef struct lmh_condensation lmh_condensation;
 
 /* Last error message of the calling thread ("" if none). */
 const char *lmh_last_error(void);
@@ -325,6 +326,42 @@ int lmh_profile_enable(lmh_context *ctx, int on);
 /* Average duration in ms and launch count of a named kernel since enable. */
 int lmh_profile_query(lmh_context *ctx, const char *kernel, double *avg_ms, int64_t *count);
 int lmh_profile_reset(lmh_context *ctx);
+
+/* ---- static condensation (StaticCondensation, include/numerics/static_condensation.h:64;
+ * src/numerics/static_condensation.C) ----
+ * The DoFs of element-internal nodes (static_condensation_dof_map.C:187-199,
+ * elem.is_internal) are eliminated: the reduced matrix holds A's pattern
+ * without the condensed rows and columns (reduced DoFs keep the full order)
+ * and the values S = A_uu - A_uc A_cc^-1 A_cu, which for DoFs that each live
+ * on one element equals the sum of the element Schur complements
+ * (static_condensation.C:205-211).  The condensed block must be diagonal (one
+ * condensed DoF per coupled block, e.g. a scalar variable on HEX27): otherwise
+ * LMH_ERR_UNSUPPORTED; a zero pivot gives LMH_ERR_DEGENERATE.  A keeps its
+ * values and must outlive the condensation. */
+
+/* DoFs of the internal nodes of dm's elements (HEX27: local node 26, every
+ * variable), element by element.  *n receives the count; dofs_h (may be NULL
+ * to query the count) receives the DoFs. */
+int lmh_dofmap_condensed_dofs(const lmh_dofmap *dm, int64_t *n, int32_t *dofs_h);
+/* Builds the reduced pattern and values from A's current values (StaticCondensation::init + close). */
+int lmh_condensation_create(lmh_matrix *A, int64_t n_cond, const int32_t *cond_dofs_h, lmh_condensation **out);
+/* Recomputes the reduced values after A's values changed (StaticCondensation::setup/close). */
+int lmh_condensation_setup(lmh_condensation *sc);
+/* The reduced operator (get_condensed_mat); owned by sc; every solver and lmh_matrix_make_fp32 apply. */
+lmh_matrix *lmh_condensation_reduced_matrix(lmh_condensation *sc);
+int64_t lmh_condensation_n_condensed(const lmh_condensation *sc);
+/* full_index_h[r] = full DoF of reduced DoF r (rows of the reduced matrix). */
+int lmh_condensation_reduced_dofs(const lmh_condensation *sc, int32_t *full_index_h);
+/* forward_elimination (static_condensation.C:383-417): g = b_u - A_uc A_cc^-1 b_c. */
+int lmh_condensation_forward(lmh_condensation *sc, const double *full_rhs_d, double *reduced_rhs_d);
+/* backwards_substitution (:419-456): x_u = y, x_c = A_cc^-1 (b_c - A_cu x_u). */
+int lmh_condensation_backward(lmh_condensation *sc, const double *full_rhs_d, const double *reduced_sol_d,
+                              double *full_sol_d);
+/* apply (:458-473): forward elimination, Jacobi-CG on the reduced matrix from
+ * x's uncondensed values (rtol on the reduced residual), back substitution into x. */
+int lmh_condensation_solve(lmh_condensation *sc, const double *b_d, double *x_d, double rtol, int64_t maxits,
+                           int64_t *iters, double *rel_res, int *reason);
+int lmh_condensation_destroy(lmh_condensation *sc);
 
 #ifdef __cplusplus
 }

@@ -98,6 +98,16 @@ HIP_API = {
     "lmh_matrix_fp32_compression": (_I, [_P]),
     "lmh_matrix_set_preconditioner": (_I, [_P, _I]),
     "lmh_bicgstab_solve_transpose": (_I, [_P, _P, _P, _D, _I64, C.POINTER(_I64), C.POINTER(_D), C.POINTER(_I)]),
+    "lmh_dofmap_condensed_dofs": (_I, [_P, C.POINTER(_I64), _P]),
+    "lmh_condensation_create": (_I, [_P, _I64, _P, C.POINTER(_P)]),
+    "lmh_condensation_setup": (_I, [_P]),
+    "lmh_condensation_reduced_matrix": (_P, [_P]),
+    "lmh_condensation_n_condensed": (_I64, [_P]),
+    "lmh_condensation_reduced_dofs": (_I, [_P, _P]),
+    "lmh_condensation_forward": (_I, [_P, _P, _P]),
+    "lmh_condensation_backward": (_I, [_P, _P, _P, _P]),
+    "lmh_condensation_solve": (_I, [_P, _P, _P, _D, _I64, C.POINTER(_I64), C.POINTER(_D), C.POINTER(_I)]),
+    "lmh_condensation_destroy": (_I, [_P]),
     "lmh_qoi_box_derivative": (_I, [_P, _P, _I, _D, _D, _P]),
     "lmh_adjoint_kappa_gradient": (_I, [_P, _P, _I, _P, _P, _P, _P]),
     "lmh_batch_create": (_I, [_P, _I, _P, C.POINTER(_P)]),
@@ -468,6 +478,15 @@ class GpuProblem:
                                  C.byref(reason)))
         return its.value, res.value, reason.value
 
+    def condensed_dofs(self):
+        """DoFs of the elements' internal nodes (HEX27 node 26), element by element."""
+        n = _I64()
+        check(hip().lmh_dofmap_condensed_dofs(self.dofmap, C.byref(n), None))
+        out = np.empty(n.value, dtype=np.int32)
+        if n.value:
+            check(hip().lmh_dofmap_condensed_dofs(self.dofmap, C.byref(n), _ptr(out)))
+        return out
+
     def bicgstab(self, b_d, x_d, rtol, maxits=5000, transpose=False):
         its = _I64()
         res = _D()
@@ -597,3 +616,66 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class StaticCondensation:
+    """Static condensation of a GpuProblem's internal-node DoFs on the device
+    (StaticCondensation, include/numerics/static_condensation.h:64): the
+    reduced matrix (get_condensed_mat), forward elimination, back substitution
+    and apply() as one solve.  The reduced DoFs are the uncondensed ones in
+    ascending full order."""
+
+    def __init__(self, problem, cond_dofs=None):
+        self.problem = problem
+        cd = problem.condensed_dofs() if cond_dofs is None else np.ascontiguousarray(cond_dofs, dtype=np.int32)
+        self.h = _P()
+        check(hip().lmh_condensation_create(problem.matrix, cd.size, _ptr(cd) if cd.size else None, C.byref(self.h)))
+        self.matrix = _P(hip().lmh_condensation_reduced_matrix(self.h))
+        self.n_condensed = hip().lmh_condensation_n_condensed(self.h)
+        self.n_reduced = hip().lmh_matrix_rows(self.matrix)
+        self.nnz = hip().lmh_matrix_nnz(self.matrix)
+
+    def close(self):
+        """Frees the device data; a no-op once the context or the problem's matrix is gone."""
+        m = getattr(self.problem, "matrix", None)
+        if self.h.value and self.problem.ctx.h and m is not None and m.value:
+            check(hip().lmh_condensation_destroy(self.h))
+        self.h = _P()
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) is not None:
+                self.close()
+        except Exception:
+            pass
+
+    def setup(self):
+        check(hip().lmh_condensation_setup(self.h))
+
+    def reduced_dofs(self):
+        out = np.empty(self.n_reduced, dtype=np.int32)
+        check(hip().lmh_condensation_reduced_dofs(self.h, _ptr(out)))
+        return out
+
+    def pattern(self):
+        rp = np.empty(self.n_reduced + 1, dtype=np.int32)
+        ci = np.empty(self.nnz, dtype=np.int32)
+        check(hip().lmh_matrix_get_pattern(self.matrix, _ptr(rp), _ptr(ci)))
+        return rp, ci
+
+    def values(self):
+        v = np.empty(self.nnz)
+        check(hip().lmh_matrix_get_values(self.matrix, _ptr(v)))
+        return v
+
+    def forward(self, b_d, g_d):
+        check(hip().lmh_condensation_forward(self.h, b_d.ptr, g_d.ptr))
+
+    def backward(self, b_d, y_d, x_d):
+        check(hip().lmh_condensation_backward(self.h, b_d.ptr, y_d.ptr, x_d.ptr))
+
+    def solve(self, b_d, x_d, rtol, maxits=5000):
+        its, res, reason = _I64(), _D(), _I()
+        check(hip().lmh_condensation_solve(self.h, b_d.ptr, x_d.ptr, rtol, maxits, C.byref(its), C.byref(res),
+                                           C.byref(reason)))
+        return its.value, res.value, reason.value

@@ -1,0 +1,318 @@
+// Static condensation of element-interior DoFs on the device.
+//
+// libMesh's StaticCondensation (include/numerics/static_condensation.h:64,
+// src/numerics/static_condensation.C) condenses the DoFs on an element's
+// internal nodes (static_condensation_dof_map.C:187-199: elem.is_internal);
+// for each element it factors Acc, forms Auu - Auc Acc^-1 Acu
+// (static_condensation.C:205-211) and sums these into the reduced matrix,
+// condenses the right-hand side (forward_elimination, :383-417) and recovers
+// the condensed values afterwards (backwards_substitution, :419-456).
+//
+// Here the same operator is formed algebraically from the assembled CSR
+// matrix.  A condensed DoF c lives on one element only, so every assembled
+// entry K_ic, K_cj equals that element's entry and the assembled condensed
+// block is diagonal when each element condenses one DoF per coupled block
+// (a scalar variable on HEX27).  Summing the element Schur complements is then
+//     S_ij = K_ij - sum_{c in row i, condensed} K_ic K_cj / K_cc,
+// computed row by row (one thread owns one reduced row: no atomics, the
+// result does not depend on scheduling).  The reduced pattern is A's pattern
+// without the condensed rows and columns, numbered in the full order.
+#include <algorithm>
+#include <memory>
+#include <vector>
+
+#include "lmh_internal.h"
+
+struct lmh_condensation {
+  lmh_matrix *A = nullptr;  // full operator (not owned)
+  lmh_matrix *R = nullptr;  // reduced operator (owned)
+  int64_t n_full = 0, n_cond = 0, n_red = 0;
+  int32_t *ridx = nullptr;   // n_full: reduced index, or -1 - q for the q-th condensed DoF
+  int32_t *rfull = nullptr;  // n_red: full index of each reduced DoF
+  int32_t *cdof = nullptr;   // n_cond: condensed DoFs, ascending
+  int32_t *dpos = nullptr;   // n_cond: position of K_cc in A's row c
+  double *g = nullptr, *y = nullptr;  // reduced right-hand side and solution
+};
+
+namespace lmh {
+
+enum { SC_OFFDIAG = 1, SC_NODIAG = 2, SC_ZEROPIVOT = 4, SC_FILL = 8 };
+
+// Per condensed DoF: the diagonal position, and a check that row c couples to
+// no other condensed DoF (the condensed block must be diagonal).
+__global__ __launch_bounds__(256) void k_sc_check(int64_t n_cond, const int32_t *__restrict__ cdof,
+                                                  const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                  const double *__restrict__ v, const int32_t *__restrict__ ridx,
+                                                  int32_t *__restrict__ dpos, int *__restrict__ flags) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n_cond; q += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t c = cdof[q];
+    int32_t d = -1;
+    int f = 0;
+    for (int32_t k = rp[c]; k < rp[c + 1]; ++k) {
+      const int32_t j = ci[k];
+      if (j == c)
+        d = k;
+      else if (ridx[j] < 0 && v[k] != 0.0)
+        f |= SC_OFFDIAG;
+    }
+    if (d < 0)
+      f |= SC_NODIAG;
+    else if (v[d] == 0.0)
+      f |= SC_ZEROPIVOT;
+    dpos[q] = d < 0 ? rp[c] : d;
+    if (f) atomicOr(flags, f);
+  }
+}
+
+// Reduced values: copy row i of A without its condensed columns, then for
+// each condensed column c of row i subtract (K_ic / K_cc) * row c, merged
+// into the (sorted) reduced row.
+__global__ __launch_bounds__(256) void k_sc_values(int64_t n_red, const int32_t *__restrict__ rfull,
+                                                   const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                   const double *__restrict__ v, const int32_t *__restrict__ ridx,
+                                                   const int32_t *__restrict__ dpos, const int32_t *__restrict__ srp,
+                                                   const int32_t *__restrict__ sci, double *__restrict__ sv,
+                                                   int *__restrict__ flags) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_red; r += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t i = rfull[r];
+    const int32_t s0 = srp[r], s1 = srp[r + 1];
+    int32_t p = s0;
+    for (int32_t k = rp[i]; k < rp[i + 1]; ++k)
+      if (ridx[ci[k]] >= 0) sv[p++] = v[k];
+    for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+      const int32_t c = ci[k], qc = ridx[c];
+      if (qc >= 0 || v[k] == 0.0) continue;
+      const double f = v[k] / v[dpos[-1 - qc]];
+      p = s0;
+      for (int32_t m = rp[c]; m < rp[c + 1]; ++m) {
+        const int32_t rj = ridx[ci[m]];
+        if (rj < 0) continue;
+        while (p < s1 && sci[p] < rj) ++p;
+        if (p == s1 || sci[p] != rj) {
+          if (v[m] != 0.0) atomicOr(flags, SC_FILL);
+          continue;
+        }
+        sv[p] -= f * v[m];
+      }
+    }
+  }
+}
+
+// g_r = b_i - sum_{c in row i, condensed} K_ic b_c / K_cc
+__global__ __launch_bounds__(256) void k_sc_forward(int64_t n_red, const int32_t *__restrict__ rfull,
+                                                    const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                    const double *__restrict__ v, const int32_t *__restrict__ ridx,
+                                                    const int32_t *__restrict__ dpos, const double *__restrict__ b,
+                                                    double *__restrict__ g) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_red; r += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t i = rfull[r];
+    double s = b[i];
+    for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+      const int32_t c = ci[k], qc = ridx[c];
+      if (qc < 0) s -= v[k] * (b[c] / v[dpos[-1 - qc]]);
+    }
+    g[r] = s;
+  }
+}
+
+// y_r = x_{rfull[r]} (TO_RED) or x_{rfull[r]} = y_r
+template <bool TO_RED>
+__global__ __launch_bounds__(256) void k_sc_map(int64_t n_red, const int32_t *__restrict__ rfull, double *__restrict__ x,
+                                                double *__restrict__ y) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_red; r += (int64_t)gridDim.x * blockDim.x) {
+    if (TO_RED)
+      y[r] = x[rfull[r]];
+    else
+      x[rfull[r]] = y[r];
+  }
+}
+
+// x_c = (b_c - sum_{j != c} K_cj x_j) / K_cc, after the uncondensed x_j are in place
+__global__ __launch_bounds__(256) void k_sc_backward(int64_t n_cond, const int32_t *__restrict__ cdof,
+                                                     const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                     const double *__restrict__ v, const int32_t *__restrict__ dpos,
+                                                     const double *__restrict__ b, double *__restrict__ x) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n_cond; q += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t c = cdof[q], d = dpos[q];
+    double s = b[c];
+    for (int32_t k = rp[c]; k < rp[c + 1]; ++k)
+      if (k != d) s -= v[k] * x[ci[k]];
+    x[c] = s / v[d];
+  }
+}
+
+static int sc_setup(lmh_condensation *sc) {
+  lmh_matrix *A = sc->A, *R = sc->R;
+  lmh_context *ctx = A->ctx;
+  LMH_HIP(hipSetDevice(ctx->device));
+  LMH_HIP(hipMemsetAsync(ctx->d_flags, 0, sizeof(int), ctx->stream));
+  if (sc->n_cond) {
+    LMH_LAUNCH(ctx, "sc_check", k_sc_check, dim3(grid_for(sc->n_cond, 256, 8192)), dim3(256), 0, sc->n_cond, sc->cdof,
+               A->rp, A->ci, A->vals, sc->ridx, sc->dpos, ctx->d_flags);
+    LMH_TRY(check_launch("k_sc_check"));
+  }
+  LMH_LAUNCH(ctx, "sc_values", k_sc_values, dim3(grid_for(sc->n_red, 256, 16384)), dim3(256), 0, sc->n_red, sc->rfull,
+             A->rp, A->ci, A->vals, sc->ridx, sc->dpos, R->rp, R->ci, R->vals, ctx->d_flags);
+  LMH_TRY(check_launch("k_sc_values"));
+  int flags = 0;
+  LMH_HIP(hipMemcpyAsync(&flags, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  LMH_HIP(hipStreamSynchronize(ctx->stream));
+  R->have32 = false;
+  LMH_CHECK(!(flags & SC_OFFDIAG), LMH_ERR_UNSUPPORTED,
+            "condensed DoFs couple to each other: only a diagonal condensed block is supported");
+  LMH_CHECK(!(flags & SC_NODIAG), LMH_ERR_INVALID, "a condensed row has no diagonal entry");
+  LMH_CHECK(!(flags & SC_ZEROPIVOT), LMH_ERR_DEGENERATE, "zero diagonal on a condensed DoF");
+  LMH_CHECK(!(flags & SC_FILL), LMH_ERR_UNSUPPORTED,
+            "condensation needs an entry outside the matrix pattern (the condensed DoF is not element-interior)");
+  return LMH_OK;
+}
+
+}  // namespace lmh
+
+using namespace lmh;
+
+extern "C" {
+
+int lmh_condensation_create(lmh_matrix *A, int64_t n_cond, const int32_t *cond_dofs_h, lmh_condensation **out) {
+  LMH_CHECK(A && out && n_cond >= 0 && (n_cond == 0 || cond_dofs_h), LMH_ERR_INVALID, "bad argument");
+  LMH_CHECK(A->rows == A->cols, LMH_ERR_INVALID, "condensation needs a square matrix");
+  LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
+  const int64_t n = A->rows;
+  LMH_CHECK(n_cond < n, LMH_ERR_INVALID, "every DoF condensed");
+  std::vector<int32_t> cd(cond_dofs_h, cond_dofs_h + n_cond);
+  std::sort(cd.begin(), cd.end());
+  for (int64_t q = 0; q < n_cond; ++q) {
+    LMH_CHECK(cd[q] >= 0 && cd[q] < n, LMH_ERR_INVALID, "condensed DoF %d out of range", cd[q]);
+    LMH_CHECK(q == 0 || cd[q] != cd[q - 1], LMH_ERR_INVALID, "condensed DoF %d listed twice", cd[q]);
+  }
+  lmh_context *ctx = A->ctx;
+  LMH_HIP(hipSetDevice(ctx->device));
+  std::vector<int32_t> rp(n + 1), ci(A->nnz);
+  LMH_TRY(lmh_matrix_get_pattern(A, rp.data(), ci.data()));
+  std::vector<int32_t> ridx(n), rfull;
+  rfull.reserve(n - n_cond);
+  for (int64_t i = 0, q = 0; i < n; ++i) {
+    if (q < n_cond && cd[q] == i) {
+      ridx[i] = (int32_t)(-1 - q);
+      ++q;
+    } else {
+      ridx[i] = (int32_t)rfull.size();
+      rfull.push_back((int32_t)i);
+    }
+  }
+  const int64_t nr = (int64_t)rfull.size();
+  std::vector<int32_t> srp(nr + 1, 0), sci;
+  sci.reserve(A->nnz);
+  for (int64_t r = 0; r < nr; ++r) {
+    const int32_t i = rfull[r];
+    for (int32_t k = rp[i]; k < rp[i + 1]; ++k)
+      if (ridx[ci[k]] >= 0) sci.push_back(ridx[ci[k]]);
+    srp[r + 1] = (int32_t)sci.size();
+  }
+  std::unique_ptr<lmh_condensation> sc(new lmh_condensation);
+  sc->A = A;
+  sc->n_full = n;
+  sc->n_cond = n_cond;
+  sc->n_red = nr;
+  const int32_t dummy = 0;
+  LMH_TRY(lmh_matrix_create_csr2(ctx, nr, nr, srp.data(), sci.empty() ? &dummy : sci.data(), &sc->R));
+  sc->R->precond = A->precond;
+  auto fail = [&](int rc) {
+    lmh_condensation_destroy(sc.release());
+    return rc;
+  };
+  if (hipMalloc((void **)&sc->ridx, sizeof(int32_t) * n) != hipSuccess ||
+      hipMalloc((void **)&sc->rfull, sizeof(int32_t) * nr) != hipSuccess ||
+      hipMalloc((void **)&sc->cdof, sizeof(int32_t) * std::max<int64_t>(1, n_cond)) != hipSuccess ||
+      hipMalloc((void **)&sc->dpos, sizeof(int32_t) * std::max<int64_t>(1, n_cond)) != hipSuccess ||
+      hipMalloc((void **)&sc->g, sizeof(double) * nr) != hipSuccess ||
+      hipMalloc((void **)&sc->y, sizeof(double) * nr) != hipSuccess)
+    return fail(set_error(LMH_ERR_ALLOC, "hipMalloc failed for the condensation"));
+  LMH_HIP(hipMemcpy(sc->ridx, ridx.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+  LMH_HIP(hipMemcpy(sc->rfull, rfull.data(), sizeof(int32_t) * nr, hipMemcpyHostToDevice));
+  if (n_cond) LMH_HIP(hipMemcpy(sc->cdof, cd.data(), sizeof(int32_t) * n_cond, hipMemcpyHostToDevice));
+  const int rc = sc_setup(sc.get());
+  if (rc != LMH_OK) return fail(rc);
+  *out = sc.release();
+  return LMH_OK;
+}
+
+int lmh_condensation_setup(lmh_condensation *sc) {
+  LMH_CHECK(sc, LMH_ERR_INVALID, "null condensation");
+  return sc_setup(sc);
+}
+
+lmh_matrix *lmh_condensation_reduced_matrix(lmh_condensation *sc) { return sc ? sc->R : nullptr; }
+int64_t lmh_condensation_n_condensed(const lmh_condensation *sc) { return sc ? sc->n_cond : -1; }
+
+int lmh_condensation_reduced_dofs(const lmh_condensation *sc, int32_t *full_index_h) {
+  LMH_CHECK(sc && full_index_h, LMH_ERR_INVALID, "null argument");
+  LMH_HIP(hipMemcpy(full_index_h, sc->rfull, sizeof(int32_t) * sc->n_red, hipMemcpyDeviceToHost));
+  return LMH_OK;
+}
+
+int lmh_condensation_forward(lmh_condensation *sc, const double *full_rhs_d, double *reduced_rhs_d) {
+  LMH_CHECK(sc && full_rhs_d && reduced_rhs_d, LMH_ERR_INVALID, "null argument");
+  lmh_matrix *A = sc->A;
+  lmh_context *ctx = A->ctx;
+  LMH_LAUNCH(ctx, "sc_forward", k_sc_forward, dim3(grid_for(sc->n_red, 256, 16384)), dim3(256), 0, sc->n_red,
+             sc->rfull, A->rp, A->ci, A->vals, sc->ridx, sc->dpos, full_rhs_d, reduced_rhs_d);
+  return check_launch("k_sc_forward");
+}
+
+int lmh_condensation_backward(lmh_condensation *sc, const double *full_rhs_d, const double *reduced_sol_d,
+                              double *full_sol_d) {
+  LMH_CHECK(sc && full_rhs_d && reduced_sol_d && full_sol_d, LMH_ERR_INVALID, "null argument");
+  lmh_matrix *A = sc->A;
+  lmh_context *ctx = A->ctx;
+  LMH_LAUNCH(ctx, "sc_map", k_sc_map<false>, dim3(grid_for(sc->n_red, 256, 16384)), dim3(256), 0, sc->n_red,
+             sc->rfull, full_sol_d, const_cast<double *>(reduced_sol_d));
+  LMH_TRY(check_launch("k_sc_map"));
+  if (!sc->n_cond) return LMH_OK;
+  LMH_LAUNCH(ctx, "sc_backward", k_sc_backward, dim3(grid_for(sc->n_cond, 256, 8192)), dim3(256), 0, sc->n_cond,
+             sc->cdof, A->rp, A->ci, A->vals, sc->dpos, full_rhs_d, full_sol_d);
+  return check_launch("k_sc_backward");
+}
+
+int lmh_condensation_solve(lmh_condensation *sc, const double *b_d, double *x_d, double rtol, int64_t maxits,
+                           int64_t *iters, double *rel_res, int *reason) {
+  LMH_CHECK(sc && b_d && x_d, LMH_ERR_INVALID, "null argument");
+  lmh_context *ctx = sc->A->ctx;
+  LMH_TRY(lmh_condensation_forward(sc, b_d, sc->g));
+  LMH_LAUNCH(ctx, "sc_map", k_sc_map<true>, dim3(grid_for(sc->n_red, 256, 16384)), dim3(256), 0, sc->n_red, sc->rfull,
+             x_d, sc->y);
+  LMH_TRY(check_launch("k_sc_map"));
+  LMH_TRY(lmh_cg_solve(sc->R, sc->g, sc->y, rtol, maxits, iters, rel_res, reason));
+  return lmh_condensation_backward(sc, b_d, sc->y, x_d);
+}
+
+int lmh_condensation_destroy(lmh_condensation *sc) {
+  if (!sc) return LMH_OK;
+  (void)hipStreamSynchronize(sc->A->ctx->stream);
+  lmh_matrix_destroy(sc->R);
+  (void)hipFree(sc->ridx);
+  (void)hipFree(sc->rfull);
+  (void)hipFree(sc->cdof);
+  (void)hipFree(sc->dpos);
+  (void)hipFree(sc->g);
+  (void)hipFree(sc->y);
+  delete sc;
+  return LMH_OK;
+}
+
+int lmh_dofmap_condensed_dofs(const lmh_dofmap *dm, int64_t *n_h, int32_t *dofs_h) {
+  LMH_CHECK(dm && n_h, LMH_ERR_INVALID, "null argument");
+  const lmh_mesh *m = dm->mesh;
+  const int internal = m->type == LMH_HEX27 ? 26 : -1;  // the only internal node among the supported types
+  const int64_t n = internal < 0 ? 0 : m->n_elem * dm->nv;
+  if (dofs_h && n) {
+    std::vector<int32_t> ed((size_t)m->n_elem * dm->dpe);
+    LMH_HIP(hipMemcpy(ed.data(), dm->edofs, sizeof(int32_t) * ed.size(), hipMemcpyDeviceToHost));
+    for (int64_t e = 0, q = 0; e < m->n_elem; ++e)
+      for (int v = 0; v < dm->nv; ++v) dofs_h[q++] = ed[(size_t)e * dm->dpe + (size_t)v * m->npe + internal];
+  }
+  *n_h = n;
+  return LMH_OK;
+}
+
+}  // extern "C"

@@ -397,3 +397,57 @@ def shapes(elem_type, pts):
     dphi = np.empty((n, nq, 3))
     lib().orc_shapes(elem_type, 0, nq, _ptr(pts), _ptr(phi), _ptr(dphi))
     return phi, dphi
+
+
+class StaticCondensation:
+    """Element-by-element static condensation, restating libMesh's
+    StaticCondensation (src/numerics/static_condensation.C): per element the
+    condensed block Acc is factored and Auu - Auc Acc^-1 Acu is added into the
+    reduced matrix (:205-211); forward_elimination adds -Auc Acc^-1 b_c to the
+    uncondensed part of b (:383-417); backwards_substitution sets
+    x_c = Acc^-1 (b_c - Acu x_u) (:419-456).  numpy's LU solve stands in for
+    Eigen's partialPivLu.  cond_local: element-local indices condensed in
+    every element (the DoFs of internal nodes, static_condensation_dof_map.C:187-199).
+    Reduced DoFs are the uncondensed ones in ascending full order."""
+
+    def __init__(self, elem_dofs, Ke, n_dofs, cond_local):
+        import scipy.sparse as sp
+        ed = np.asarray(elem_dofs, dtype=np.int64)
+        dpe = ed.shape[1]
+        c = np.asarray(cond_local, dtype=np.int64)
+        u = np.setdiff1d(np.arange(dpe), c)
+        self.ed, self.c, self.u = ed, c, u
+        self.cdofs = np.unique(ed[:, c])
+        keep = np.ones(n_dofs, dtype=bool)
+        keep[self.cdofs] = False
+        self.rfull = np.nonzero(keep)[0]
+        self.ridx = np.full(n_dofs, -1, dtype=np.int64)
+        self.ridx[self.rfull] = np.arange(self.rfull.size)
+        Ke = np.asarray(Ke)
+        self.Acc = Ke[:, c][:, :, c]
+        self.Acu = Ke[:, c][:, :, u]
+        self.Auc = Ke[:, u][:, :, c]
+        Auu = Ke[:, u][:, :, u]
+        S = Auu - self.Auc @ np.linalg.solve(self.Acc, self.Acu)
+        ru = self.ridx[ed[:, u]]
+        rows = np.repeat(ru, u.size, axis=1).ravel()
+        cols = np.tile(ru, (1, u.size)).ravel()
+        m = self.rfull.size
+        self.S = sp.csr_matrix((S.ravel(), (rows, cols)), shape=(m, m))
+        self.S.sum_duplicates()
+        self.S.sort_indices()
+
+    def forward(self, b):
+        g = np.asarray(b, dtype=np.float64)[self.rfull].copy()
+        bc = b[self.ed[:, self.c]][:, :, None]
+        gu = -(self.Auc @ np.linalg.solve(self.Acc, bc))[:, :, 0]
+        np.add.at(g, self.ridx[self.ed[:, self.u]], gu)
+        return g
+
+    def backward(self, b, y):
+        x = np.zeros(self.ridx.size)
+        x[self.rfull] = y
+        xu = x[self.ed[:, self.u]][:, :, None]
+        bc = b[self.ed[:, self.c]][:, :, None]
+        x[self.ed[:, self.c]] = np.linalg.solve(self.Acc, bc - self.Acu @ xu)[:, :, 0]
+        return x
