@@ -169,6 +169,88 @@ def bench_constrained(args, L, ctx, kappa, log):
     return out
 
 
+def bench_condensed(args, L, ctx, kappa, log):
+    """Static condensation of the HEX27 centre-node DoFs on the same problem:
+    reduced operator set-up (Schur complement values), then Jacobi-CG and the
+    mixed solver on the reduced system with forward elimination and back
+    substitution, beside Jacobi-CG on the full system in the same run."""
+    p = L.Problem(args.n, L.HEX27, 1, L.ALL_FACES)
+    p.perturb(SEED_PERTURB, 0.2)
+    g = L.GpuProblem.from_problem(ctx, p)
+    kd = L.DeviceArray(ctx, p.n_elem, data=kappa)
+    b = L.DeviceArray(ctx, p.n_dofs)
+    x = L.DeviceArray(ctx, p.n_dofs)
+    g.assemble_poisson(3, kd, 0, b, L.FP64)
+    ctx.sync()
+    T0 = time.perf_counter()
+    sc = L.StaticCondensation(g)
+    t_create = time.perf_counter() - T0
+    reps = 3
+    T0 = time.perf_counter()
+    for _ in range(reps):
+        sc.setup()
+    t_setup = (time.perf_counter() - T0) / reps
+    ctx.profile_reset()
+    ctx.profile(True)
+    sc.setup()
+    ctx.sync()
+    ctx.profile(False)
+    ms, c = ctx.profile_query("sc_values")
+    k_values = round(ms * c, 4)
+    x.zero()
+    ctx.sync()
+    T0 = time.perf_counter()
+    f_its, _, f_reason = g.cg(b, x, args.rtol, 20000)
+    ctx.sync()
+    t_full = time.perf_counter() - T0
+    xf = x.download()
+    x.zero()
+    ctx.sync()
+    T0 = time.perf_counter()
+    its, res, reason = sc.solve(b, x, args.rtol, 20000)
+    ctx.sync()
+    t_cond = time.perf_counter() - T0
+    if reason != L.CONVERGED_RTOL or f_reason != L.CONVERGED_RTOL:
+        raise RuntimeError(f"condensed solve did not converge: its={its} reason={reason} full={f_reason}")
+    diff = float(np.abs(x.download() - xf).max() / np.abs(xf).max())
+    # mixed solver on the reduced operator
+    L.check(L.hip().lmh_matrix_make_fp32(sc.matrix))
+    gr = L.DeviceArray(ctx, sc.n_reduced)
+    y = L.DeviceArray(ctx, sc.n_reduced)
+    y.zero()
+    m_its, m_res, m_reason = L.C.c_int64(), L.C.c_double(), L.C.c_int()
+    stats = np.zeros(3)
+    ctx.sync()
+    T0 = time.perf_counter()
+    sc.forward(b, gr)
+    L.check(L.hip().lmh_mixed_solve(sc.matrix, gr.ptr, y.ptr, args.rtol, args.inner, 0, 20000, L.C.byref(m_its),
+                                    L.C.byref(m_res), L.C.byref(m_reason), L._ptr(stats)))
+    sc.backward(b, y, x)
+    ctx.sync()
+    t_mixed = time.perf_counter() - T0
+    if m_reason.value != L.CONVERGED_RTOL:
+        raise RuntimeError(f"condensed mixed solve did not converge: reason={m_reason.value}")
+    out = {
+        "condensed_dofs": int(sc.n_condensed),
+        "reduced_rows": int(sc.n_reduced),
+        "nnz_full": int(g.nnz),
+        "nnz_reduced": int(sc.nnz),
+        "create_host_s": round(t_create, 3),
+        "setup_ms": round(t_setup * 1e3, 3),
+        "sc_values_kernel_ms": k_values,
+        "full_cg_ms": round(t_full * 1e3, 3),
+        "full_cg_iterations": int(f_its),
+        "condensed_cg_ms": round(t_cond * 1e3, 3),
+        "condensed_cg_iterations": int(its),
+        "condensed_mixed_ms": round(t_mixed * 1e3, 3),
+        "condensed_mixed_iterations": int(m_its.value),
+        "max_rel_diff_vs_full_cg": diff,
+    }
+    sc.close()
+    g.close()
+    return out
+
+
 def cpu_baseline(n_sample):
     """The oracle (C++ restatement of libMesh's FP64 path) on host cores, on a
     smaller instance of the same workload."""
@@ -488,6 +570,8 @@ def main():
 
     constrained = bench_constrained(args, L, ctx, kappa, log)
     log(f"constrained: {constrained}")
+    condensed = bench_condensed(args, L, ctx, kappa, log)
+    log(f"condensed: {condensed}")
 
     # assembly kernel timings (FP64 path, one profiled assembly)
     ctx.profile_reset()
@@ -580,6 +664,7 @@ def main():
         "assembly_kernels_ms": {k: round(v, 4) for k, v in asm_k.items()},
         "adjoint": adjoint,
         "constrained": constrained,
+        "condensed": condensed,
         "fem_residual_ms": fem_residual,
         "contraction_gflop": flops_contract / 1e9,
         "setup_s": round(t_setup, 3),

@@ -32,6 +32,7 @@ struct lmh_condensation {
   int32_t *cdof = nullptr;   // n_cond: condensed DoFs, ascending
   int32_t *dpos = nullptr;   // n_cond: position of K_cc in A's row c
   double *g = nullptr, *y = nullptr;  // reduced right-hand side and solution
+  bool pattern_checked = false;       // k_sc_values found no fill-in (then k_sc_values_wave is used)
 };
 
 namespace lmh {
@@ -98,6 +99,57 @@ __global__ __launch_bounds__(256) void k_sc_values(int64_t n_red, const int32_t 
   }
 }
 
+// The same values with one wavefront per reduced row (used once the pattern
+// has passed k_sc_values' fill check): lane t owns entry (i, j) of the row
+// and subtracts f_c K_cj for the condensed columns c of row i in ascending
+// order (wave-uniform loop), K_cj found by binary search in row c.  The
+// kept entries are compacted to the reduced row with a ballot prefix count.
+__global__ __launch_bounds__(256) void k_sc_values_wave(int64_t n_red, const int32_t *__restrict__ rfull,
+                                                        const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                        const double *__restrict__ v, const int32_t *__restrict__ ridx,
+                                                        const int32_t *__restrict__ dpos,
+                                                        const int32_t *__restrict__ srp, double *__restrict__ sv) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = (1ull << lane) - 1;
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); r < n_red; r += nw) {
+    const int32_t i = rfull[r], k0 = rp[i], k1 = rp[i + 1];
+    int32_t out = srp[r];
+    for (int32_t kb = k0; kb < k1; kb += 64) {
+      const int32_t k = kb + lane;
+      const int32_t j = k < k1 ? ci[k] : 0;
+      const bool keep = k < k1 && ridx[j] >= 0;
+      double acc = keep ? v[k] : 0.0;
+      for (int32_t cb = k0; cb < k1; cb += 64) {
+        const int32_t kc = cb + lane;
+        const int32_t cc = kc < k1 ? ci[kc] : 0;
+        const int32_t qc = kc < k1 ? ridx[cc] : 0;
+        uint64_t m = __ballot(kc < k1 && qc < 0 && v[kc] != 0.0);
+        while (m) {
+          const int b = __builtin_ctzll(m);
+          m &= m - 1;
+          const int32_t c = __shfl(cc, b), q = -1 - __shfl(qc, b);
+          const double f = v[cb + b] / v[dpos[q]];
+          if (keep) {
+            int32_t lo = rp[c], hi = rp[c + 1];
+            while (lo < hi) {
+              const int32_t mid = (lo + hi) >> 1;
+              if (ci[mid] < j)
+                lo = mid + 1;
+              else
+                hi = mid;
+            }
+            if (lo < rp[c + 1] && ci[lo] == j) acc -= f * v[lo];
+          }
+        }
+      }
+      const uint64_t km = __ballot(keep);
+      if (keep) sv[out + __popcll(km & below)] = acc;
+      out += __popcll(km);
+    }
+  }
+}
+
 // g_r = b_i - sum_{c in row i, condensed} K_ic b_c / K_cc
 __global__ __launch_bounds__(256) void k_sc_forward(int64_t n_red, const int32_t *__restrict__ rfull,
                                                     const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
@@ -151,9 +203,15 @@ static int sc_setup(lmh_condensation *sc) {
                A->rp, A->ci, A->vals, sc->ridx, sc->dpos, ctx->d_flags);
     LMH_TRY(check_launch("k_sc_check"));
   }
-  LMH_LAUNCH(ctx, "sc_values", k_sc_values, dim3(grid_for(sc->n_red, 256, 16384)), dim3(256), 0, sc->n_red, sc->rfull,
-             A->rp, A->ci, A->vals, sc->ridx, sc->dpos, R->rp, R->ci, R->vals, ctx->d_flags);
-  LMH_TRY(check_launch("k_sc_values"));
+  if (!sc->pattern_checked) {
+    LMH_LAUNCH(ctx, "sc_values_check", k_sc_values, dim3(grid_for(sc->n_red, 256, 16384)), dim3(256), 0, sc->n_red,
+               sc->rfull, A->rp, A->ci, A->vals, sc->ridx, sc->dpos, R->rp, R->ci, R->vals, ctx->d_flags);
+    LMH_TRY(check_launch("k_sc_values"));
+  } else {
+    LMH_LAUNCH(ctx, "sc_values", k_sc_values_wave, dim3(grid_for(sc->n_red, 4, 65536)), dim3(256), 0, sc->n_red,
+               sc->rfull, A->rp, A->ci, A->vals, sc->ridx, sc->dpos, R->rp, R->vals);
+    LMH_TRY(check_launch("k_sc_values_wave"));
+  }
   int flags = 0;
   LMH_HIP(hipMemcpyAsync(&flags, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   LMH_HIP(hipStreamSynchronize(ctx->stream));
@@ -164,6 +222,7 @@ static int sc_setup(lmh_condensation *sc) {
   LMH_CHECK(!(flags & SC_ZEROPIVOT), LMH_ERR_DEGENERATE, "zero diagonal on a condensed DoF");
   LMH_CHECK(!(flags & SC_FILL), LMH_ERR_UNSUPPORTED,
             "condensation needs an entry outside the matrix pattern (the condensed DoF is not element-interior)");
+  sc->pattern_checked = true;
   return LMH_OK;
 }
 

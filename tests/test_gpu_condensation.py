@@ -54,6 +54,9 @@ def test_condensed_operator_and_solve(ctx, n):
     rp, ci = sc.pattern()
     assert np.array_equal(rp, osc.S.indptr) and np.array_equal(ci, osc.S.indices)
     assert rel(sc.values(), osc.S.data) < 1e-12
+    v_first = sc.values()
+    sc.setup()  # values again through the wavefront-per-row kernel
+    assert rel(sc.values(), osc.S.data) < 1e-12 and rel(sc.values(), v_first) < 1e-14
     gr = L.DeviceArray(ctx, sc.n_reduced)
     sc.forward(b, gr)
     og = osc.forward(orhs)
