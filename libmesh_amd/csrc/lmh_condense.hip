@@ -150,20 +150,25 @@ __global__ __launch_bounds__(256) void k_sc_values_wave(int64_t n_red, const int
   }
 }
 
-// g_r = b_i - sum_{c in row i, condensed} K_ic b_c / K_cc
+// g_r = b_i - sum_{c in row i, condensed} K_ic b_c / K_cc; one wavefront per
+// reduced row, lanes over the row's entries (coalesced), then a fixed-order
+// shuffle reduction.
 __global__ __launch_bounds__(256) void k_sc_forward(int64_t n_red, const int32_t *__restrict__ rfull,
                                                     const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
                                                     const double *__restrict__ v, const int32_t *__restrict__ ridx,
                                                     const int32_t *__restrict__ dpos, const double *__restrict__ b,
                                                     double *__restrict__ g) {
-  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_red; r += (int64_t)gridDim.x * blockDim.x) {
-    const int32_t i = rfull[r];
-    double s = b[i];
-    for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); r < n_red; r += nw) {
+    const int32_t i = rfull[r], k1 = rp[i + 1];
+    double s = 0.0;
+    for (int32_t k = rp[i] + lane; k < k1; k += 64) {
       const int32_t c = ci[k], qc = ridx[c];
-      if (qc < 0) s -= v[k] * (b[c] / v[dpos[-1 - qc]]);
+      if (qc < 0) s += v[k] * (b[c] / v[dpos[-1 - qc]]);
     }
-    g[r] = s;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+    if (lane == 0) g[r] = b[i] - s;
   }
 }
 
@@ -314,7 +319,7 @@ int lmh_condensation_forward(lmh_condensation *sc, const double *full_rhs_d, dou
   LMH_CHECK(sc && full_rhs_d && reduced_rhs_d, LMH_ERR_INVALID, "null argument");
   lmh_matrix *A = sc->A;
   lmh_context *ctx = A->ctx;
-  LMH_LAUNCH(ctx, "sc_forward", k_sc_forward, dim3(grid_for(sc->n_red, 256, 16384)), dim3(256), 0, sc->n_red,
+  LMH_LAUNCH(ctx, "sc_forward", k_sc_forward, dim3(grid_for(sc->n_red, 4, 65536)), dim3(256), 0, sc->n_red,
              sc->rfull, A->rp, A->ci, A->vals, sc->ridx, sc->dpos, full_rhs_d, reduced_rhs_d);
   return check_launch("k_sc_forward");
 }
