@@ -17,6 +17,9 @@ Each oracle function is pinned by something that a plausible mistake would fail:
                     restatement vs Eigen restatement, O(h^2) manufactured convergence
   adjoint         : sum_e kappa_e dJ/dkappa_e = -J (J is homogeneous of degree -1 in kappa),
                     restated FD sensitivity vs closed form
+  QoI             : J = int_box u of polynomial interpolants in closed form (1, x, xy, xyz),
+                    sum dJ/du = |box|, zero derivative off the box's support, a shifted box
+                    fails, an unaligned box against an independent affine Gauss sum
 No fixture of the reference pins assembled matrices, patterns or CG results
 (tests/numerics and tests/systems hold none); those are pinned by the checks above.
 """
@@ -401,3 +404,81 @@ def test_elasticity_constrained_rows():
         assert d == float((ed == r).any(axis=1).sum())
         assert np.count_nonzero(vals[row]) == 1
         assert rhs[r] == 0.0
+
+
+def _nodal(m, s, fn):
+    """DoF vector of the nodal interpolant of fn(x, y, z) for one scalar variable."""
+    X = m.coords()
+    u = np.zeros(s.n_dofs)
+    u[s.elem_dofs().ravel()] = fn(*X[m.conn().ravel()].T)
+    return u
+
+
+# J = int_[0.25,0.75]^3 u with the per-qp physical-coordinate inclusion of
+# examples/adjoints/adjoints_ex2/L-qoi.C:29-95 (element_qoi / element_qoi_derivative).
+# With the box faces on element faces every qp of an element lies inside or
+# outside together, so J is the exact integral of the interpolant over the box.
+_QOI_CASES = [
+    (lambda x, y, z: np.ones_like(x), 0.125),
+    (lambda x, y, z: x, 0.0625),          # int x over [.25,.75] = 0.25, times 0.5^2
+    (lambda x, y, z: x * y, 0.03125),     # 0.25^2 * 0.5
+    (lambda x, y, z: x * y * z, 0.015625),  # 0.25^3 (trilinear: HEX only)
+]
+
+
+@pytest.mark.parametrize("t,n,npts", [(O.HEX27, 4, 3), (O.HEX27, 8, 3), (O.HEX8, 4, 2), (O.TET10, 4, 5)])
+def test_qoi_box_closed_forms(t, n, npts):
+    m = O.Mesh(n, t)
+    s = O.System(m, 1, dirichlet_mask=0)
+    cases = _QOI_CASES if t != O.TET10 else _QOI_CASES[:3]
+    for fn, exact in cases:
+        u = _nodal(m, s, fn)
+        assert abs(s.qoi_box(npts, u) - exact) < 1e-14, (fn, exact)
+    # dJ/du without constraints: sum_i dq_i = int_box sum_i phi_i = |box| (partition of unity),
+    # and dq . u = J for every u (J is linear in u)
+    dq = s.qoi_box_derivative(npts)
+    assert abs(dq.sum() - 0.125) < 1e-14
+    u = _nodal(m, s, lambda x, y, z: x * y)
+    assert abs(dq @ u - 0.03125) < 1e-14
+    # every DoF whose support misses the box gets exactly 0
+    X = m.coords()
+    Xd = np.zeros((s.n_dofs, 3))
+    Xd[s.elem_dofs().ravel()] = X[m.conn().ravel()]
+    far = ((Xd <= 0.25 - 1.0 / n + 1e-12) | (Xd >= 0.75 + 1.0 / n - 1e-12)).any(axis=1)
+    assert far.any() and np.all(dq[far] == 0.0)
+
+
+def test_qoi_box_planted_shift_fails():
+    """A box shifted by one element keeps |box|, so u = 1 cannot see it; u = x does."""
+    n = 4
+    m = O.Mesh(n, O.HEX27)
+    s = O.System(m, 1, dirichlet_mask=0)
+    u1 = _nodal(m, s, lambda x, y, z: np.ones_like(x))
+    ux = _nodal(m, s, lambda x, y, z: x)
+    h = 1.0 / n
+    assert abs(s.qoi_box(3, u1, 0.25 + h, 0.75 + h) - 0.125) < 1e-14
+    assert abs(s.qoi_box(3, ux, 0.25 + h, 0.75 + h) - 0.0625) > 1e-3
+
+
+def test_qoi_box_unaligned_against_affine_quadrature():
+    """Box faces inside elements: J is the Gauss sum over the qps whose physical
+    coordinates fall in the box.  On the unperturbed mesh every element is an
+    axis-aligned cube of side 1/n, so the physical qps are x0 + (1 + xi) / (2n) and
+    JxW = w / (2n)^3; this is summed here independently of the oracle's FE map.
+    An inclusion test in reference coordinates, or no test at all, fails it."""
+    n = 3
+    m = O.Mesh(n, O.HEX27)
+    s = O.System(m, 1, dirichlet_mask=0)
+    pts, w = O.qgauss_1d(3)
+    lo, hi = 0.25, 0.75
+    h = 1.0 / n
+    # 1D: which qps of each element lie in [lo, hi], with their weights and positions
+    q1 = [(e * h + (1 + p) * h / 2, wi * h / 2) for e in range(n) for p, wi in zip(pts, w)]
+    ins = [(x, wx) for x, wx in q1 if lo <= x <= hi]
+    vol = sum(wx for _, wx in ins) ** 3
+    mx = sum(x * wx for x, wx in ins) * sum(wx for _, wx in ins) ** 2
+    for fn, exact in ((lambda x, y, z: np.ones_like(x), vol), (lambda x, y, z: x, mx)):
+        J = s.qoi_box(3, _nodal(m, s, fn), lo, hi)
+        assert abs(J - exact) < 1e-14, (J, exact)
+    # differs from the true box integral (0.125): the per-qp rule, not the exact cut
+    assert abs(vol - 0.125) > 1e-3
