@@ -35,6 +35,17 @@ MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_it_spmv16_hex27_64cube_pmc.txt")
 
 
+def sig(v, digits=6):
+    """Floats to `digits` significant digits (residuals like 9.7e-11 keep their value)."""
+    if isinstance(v, float):
+        return float(f"{v:.{digits}g}")
+    if isinstance(v, dict):
+        return {k: sig(x, digits) for k, x in v.items()}
+    if isinstance(v, (list, tuple)):
+        return [sig(x, digits) for x in v]
+    return v
+
+
 def spmv_bytes(nnz, n, val_bytes):
     """Algorithmic bytes of one CSR SpMV: values + int32 columns + row_ptr + x read + y write."""
     return (val_bytes + 4) * nnz + 4 * (n + 1) + 2 * val_bytes * n
@@ -230,6 +241,9 @@ def bench_condensed(args, L, ctx, kappa, log):
     t_mixed = time.perf_counter() - T0
     if m_reason.value != L.CONVERGED_RTOL:
         raise RuntimeError(f"condensed mixed solve did not converge: reason={m_reason.value}")
+    diff_mixed = float(np.abs(x.download() - xf).max() / np.abs(xf).max())
+    if diff_mixed > 1e-8 or diff > 1e-8:
+        raise RuntimeError(f"condensed solutions differ from the full CG solution: cg {diff:.2e} mixed {diff_mixed:.2e}")
     out = {
         "condensed_dofs": int(sc.n_condensed),
         "reduced_rows": int(sc.n_reduced),
@@ -245,19 +259,14 @@ def bench_condensed(args, L, ctx, kappa, log):
         "condensed_mixed_ms": round(t_mixed * 1e3, 3),
         "condensed_mixed_iterations": int(m_its.value),
         "max_rel_diff_vs_full_cg": diff,
+        "mixed_max_rel_diff_vs_full_cg": diff_mixed,
     }
     sc.close()
     g.close()
     return out
 
 
-def cpu_baseline(n_sample):
-    """The oracle (C++ restatement of libMesh's FP64 path) on host cores, on a
-    smaller instance of the same workload."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+def host_description():
     cpu_model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
@@ -267,26 +276,104 @@ def cpu_baseline(n_sample):
                     break
     except OSError:
         pass
-    t0 = time.perf_counter()
-    m = O.Mesh(n_sample, O.HEX27)
-    m.perturb(SEED_PERTURB, 0.2)
-    kappa = O.kappa_lognormal(m.n_elem, SEED_KAPPA)
-    s = O.System(m)
-    t1 = time.perf_counter()
-    vals, rhs = s.assemble_poisson(3, kappa, 0)
-    t2 = time.perf_counter()
-    rp, ci = s.pattern()
-    x, its, err = O.cg_eigen(rp, ci, vals, rhs, 1e-10, 20000)
-    t3 = time.perf_counter()
-    return {
-        "value": s.n_dofs / ((t2 - t1) + (t3 - t2)) / 1e6,
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return cpu_model, usable
+
+
+def cpu_threads():
+    """Threads of the all-core CPU runs: OMP_NUM_THREADS when set, else the CPUs this process may use."""
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return env or host_description()[1]
+
+
+class OracleRun:
+    """The CPU reference path on the bench workload itself (n^3 HEX27, same
+    seeds): the oracle's serial element loop (the restated libMesh assembly of
+    introduction_ex4.C:313-560 with the kappa-weighted form) for the matrix and
+    right-hand side, then the reference's own vendored Eigen 3.3.9
+    ConjugateGradient + DiagonalPreconditioner (oracle/_ref, driven as
+    EigenSparseLinearSolver::solve does) to the same rtol.  Mesh, DoF numbering
+    and the sparsity pattern are setup and stay off the clock, as on the GPU."""
+
+    def __init__(self, n):
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        self.O = O
+        t0 = time.perf_counter()
+        m = O.Mesh(n, O.HEX27)
+        m.perturb(SEED_PERTURB, 0.2)
+        self.kappa = O.kappa_lognormal(m.n_elem, SEED_KAPPA)
+        self.mesh = m
+        self.sys = O.System(m)
+        self.rp, self.ci = self.sys.pattern()
+        self.t_setup = time.perf_counter() - t0
+        self.n_dofs = self.sys.n_dofs
+        if O.ref() is None:
+            raise RuntimeError("oracle/_ref/libeigen_ref.so is not built (the reference tree was absent at build time)")
+
+    def run(self, rtol, threads):
+        t0 = time.perf_counter()
+        vals, rhs = self.sys.assemble_poisson(3, self.kappa, 0)
+        t_asm = time.perf_counter() - t0
+        x, its, err, t_solve = self.O.ref_cg_eigen_timed(self.rp, self.ci, vals, rhs, rtol, 20000, threads)
+        return {"t_assemble_s": t_asm, "t_solve_s": t_solve, "iterations": its, "rel_residual": err,
+                "x": x, "vals": vals, "rhs": rhs}
+
+
+def cpu_baseline(orc, rtol, log):
+    """One run of the CPU reference path on the full workload, all usable cores
+    for Eigen's threaded SpMV (the element loop is serial, as in the reference's
+    example).  Returns the baseline record and the run (its solution is the FP64
+    oracle solution the GPU results are compared with)."""
+    threads = cpu_threads()
+    model, usable = host_description()
+    r = orc.run(rtol, threads)
+    log(f"cpu baseline: assembly {r['t_assemble_s']:.2f} s, Eigen CG {r['iterations']} its {r['t_solve_s']:.2f} s "
+        f"({threads} threads)")
+    rec = {
+        "value": orc.n_dofs / (r["t_assemble_s"] + r["t_solve_s"]) / 1e6,
         "unit": "Mdof/s",
         "cores": threads,
-        "kind": "port",
-        "sample": f"{n_sample}^3 HEX27 ({s.n_dofs} dofs, same recipe), oracle assembly (serial element loop) "
-                  f"{t2 - t1:.2f}s + Eigen-restated Jacobi-CG {its} its {t3 - t2:.2f}s (OpenMP SpMV), setup "
-                  f"{t1 - t0:.2f}s excluded; host {cpu_model}, {os.cpu_count()} logical CPUs visible",
+        "kind": "reference",
+        "sample": f"the full workload ({orc.n_dofs} dofs), one run: assembly by the oracle's serial element loop "
+                  f"{r['t_assemble_s']:.2f} s, then the reference's vendored Eigen 3.3.9 Jacobi-CG (oracle/_ref) "
+                  f"{r['iterations']} its to rtol {rtol:g} in {r['t_solve_s']:.2f} s on {threads} OpenMP threads "
+                  f"(threaded SpMV, serial vector ops); mesh/DoF/pattern setup {orc.t_setup:.2f} s excluded; "
+                  f"host {model}, {usable} usable CPUs",
+        "t_assemble_s": r["t_assemble_s"],
+        "t_solve_s": r["t_solve_s"],
+        "iterations": r["iterations"],
+        "rel_residual": r["rel_residual"],
     }
+    return rec, r
+
+
+def cpu_full(args, log):
+    """Per-phase medians of the CPU reference path at 1 thread and at all
+    usable cores (1 warm-up, then `runs` timed runs each); one JSON line."""
+    orc = OracleRun(args.n)
+    model, usable = host_description()
+    allc = cpu_threads()
+    out = {"workload": f"{args.n}^3 HEX27 C1 recipe ({orc.n_dofs} dofs)", "host": model, "usable_cpus": usable,
+           "setup_s": orc.t_setup, "rtol": args.rtol, "runs": args.cpu_runs, "threads": {}}
+    for th in sorted({1, allc}):
+        orc.run(args.rtol, th)  # warm-up
+        rs = []
+        for k in range(args.cpu_runs):
+            r = orc.run(args.rtol, th)
+            rs.append((r["t_assemble_s"], r["t_solve_s"], r["iterations"]))
+            log(f"threads {th} run {k}: assembly {r['t_assemble_s']:.3f} s solve {r['t_solve_s']:.3f} s "
+                f"its {r['iterations']}")
+        ta = float(np.median([a for a, _, _ in rs]))
+        ts = float(np.median([b for _, b, _ in rs]))
+        out["threads"][str(th)] = {"assembly_s": [a for a, _, _ in rs], "solve_s": [b for _, b, _ in rs],
+                                   "median_assembly_s": ta, "median_solve_s": ts, "iterations": rs[-1][2],
+                                   "mdofs": orc.n_dofs / (ta + ts) / 1e6}
+    print(json.dumps(sig(out)), flush=True)
 
 
 SEED_BATCH = 0x5EED0003
@@ -317,13 +404,13 @@ def batch_cpu_baseline(nmin, nmax, count):
                       f"FP64 element loop + constraints + scatter, {t:.2f}s"}
 
 
-def bench_batched(args, L, log):
+def bench_batched(args, L, log, ctx, steps, warmup):
     """Batched FP32 elasticity assembly: `batch_count` problems (E=1, nu=0.3,
     body force (0,0,-1), clamped x=0 face), edge N log-uniform in [8, nmax],
     element type uniform over HEX8/HEX27/TET4/TET10 from a seed; one step = one
-    hipGraph launch assembling every problem (element kernel + scatter)."""
+    hipGraph launch assembling every problem (element kernel + scatter).
+    Returns the record (a bench line of its own with --workload batched)."""
     edge, kind = L.batch_recipe(SEED_BATCH, args.batch_count, 8, args.batch_nmax)
-    ctx = L.Context(0)
     t0 = time.perf_counter()
     gps, n_elem, n_dofs, n_nodes, npe = [], 0, 0, [], []
     for i, (n, k) in enumerate(zip(edge, kind)):
@@ -343,16 +430,20 @@ def bench_batched(args, L, log):
     best = b.autotune(1.0, 0.3, -1.0, 2)
     lpe, pp = b.tiles()
     log(f"autotuned lanes/element {lpe.tolist()} passes/tile {pp.tolist()} best ms {best.round(3).tolist()}")
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         b.assemble()
     ctx.sync()
-    ctx.profile_reset()
-    ctx.profile(True)
     T0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         b.assemble()
     ctx.sync()
     T1 = time.perf_counter()
+    # the same launches again with HIP events around each graph launch
+    ctx.profile_reset()
+    ctx.profile(True)
+    for _ in range(steps):
+        b.assemble()
+    ctx.sync()
     ctx.profile(False)
     gms, gcnt = ctx.profile_query("batch_graph")
     # compulsory bytes: FP32 values written once, col_idx and row_ptr read,
@@ -367,9 +458,9 @@ def bench_batched(args, L, log):
         "value": round(n_elem / (gms * 1e-3), 1),
         "unit": "elements/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round((T1 - T0) / args.steps * 1e3, 3),
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round((T1 - T0) / steps * 1e3, 3),
         "graph_ms": round(gms, 4),
         "higher_is_better": True,
         "scaling": "weak",
@@ -395,11 +486,13 @@ def bench_batched(args, L, log):
         "setup_s": round(t_setup, 2),
         "mdofs": round(n_dofs / (gms * 1e-3) / 1e6, 3),
     }
-    if args.cpu_sample > 0:
-        log("cpu baseline ...")
+    if args.cpu != "skip":
+        log("batched cpu baseline ...")
         out["cpu_baseline"] = batch_cpu_baseline(4, 8, 32)
     b.close()
-    print(json.dumps(out), flush=True)
+    for g in gps:
+        g.close()
+    return out
 
 
 def main():
@@ -411,7 +504,15 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--inner", type=float, default=0.1, help="reliable-update trigger of the mixed solver")
     ap.add_argument("--scatter", choices=["gather", "colored"], default="gather")
-    ap.add_argument("--cpu-sample", type=int, default=40, help="cube edge of the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu", choices=["run", "skip"], default="run",
+                    help="run: time the CPU reference path once on the full workload (all usable cores) and "
+                         "compare every GPU solution with its FP64 solution")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="only the CPU reference path: per-phase medians of --cpu-runs runs at 1 thread and at all "
+                         "usable cores (no GPU)")
+    ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--batched", choices=["run", "skip"], default="run",
+                    help="run: add the batched elasticity block (64 problems, one hipGraph launch per step)")
     ap.add_argument("--copy-mib", type=int, default=1024, help="size of the copy-kernel calibration (0: skip)")
     ap.add_argument("--workload", choices=["poisson", "batched"], default="poisson",
                     help="poisson: 64^3 HEX27 assembly + CG (the headline); batched: 64 elasticity problems")
@@ -427,8 +528,12 @@ def main():
     import libmesh_amd as L
 
     log = (lambda *a: None) if args.quiet else (lambda *a: print(*a, file=sys.stderr, flush=True))
+    if args.cpu_full:
+        cpu_full(args, log)
+        return
     if args.workload == "batched":
-        bench_batched(args, L, log)
+        ctx = L.Context(0)
+        print(json.dumps(sig(bench_batched(args, L, log, ctx, args.steps, args.warmup))), flush=True)
         return
     scatter = L.SCATTER_GATHER if args.scatter == "gather" else L.SCATTER_COLORED
 
@@ -584,6 +689,7 @@ def main():
         ms, c = ctx.profile_query(k)
         if c:
             asm_k[k] = ms * c
+    rhs_h = b.download()  # the FP64 right-hand side, compared with the oracle's below
     # FP32 MFMA contraction + FP32 scatter (writes the FP32 operator only)
     ctx.profile_reset()
     ctx.profile(True)
@@ -649,10 +755,13 @@ def main():
             "avg_launch_ms": round(mx["spmv_avg_ms"], 5),
             "avg_working_launch_ms": round(working_ms(mx), 5),
             "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, ROOT) if traffic else None,
+            # HBM bytes the counters saw per launch over the same launch time
+            "traffic_frac": (round(traffic / (working_ms(mx) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                             if traffic else None),
         },
-        "fp64": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in results["fp64"].items()},
-        "mixed": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in results["mixed"].items()},
-        "mixed_mfma32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in results["mixed_mfma32"].items()},
+        "fp64": results["fp64"],
+        "mixed": results["mixed"],
+        "mixed_mfma32": results["mixed_mfma32"],
         "speedup_mixed_mfma32_vs_fp64": round(results["mixed_mfma32"]["mdofs"] / results["fp64"]["mdofs"], 4),
         "mixed_mfma32_rel_diff_vs_fp64_solution": err_mfma32_vs_fp64,
         "speedup_mixed_vs_fp64": round(mx["mdofs"] / results["fp64"]["mdofs"], 4),
@@ -684,10 +793,34 @@ def main():
                                                "frac": round(gbs / HBM_PEAK_GBS, 4), "compulsory_bytes": sb}
     if copy:
         out["copy_calibration"] = copy
-    if args.cpu_sample > 0:
-        log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(args.cpu_sample)
-    print(json.dumps(out), flush=True)
+    gvals = g.values() if args.cpu == "run" else None
+    g.close()
+    del kd, b, x
+    if args.batched == "run":
+        log("batched elasticity block ...")
+        rec = bench_batched(args, L, log, ctx, args.steps, args.warmup)
+        out["batched"] = {k: v for k, v in rec.items()
+                          if k not in ("n_gpus", "higher_is_better", "scaling", "vs_baseline", "data")}
+    if args.cpu == "run":
+        log("cpu baseline (reference path on the full workload) ...")
+        orc = OracleRun(args.n)
+        rec, r = cpu_baseline(orc, args.rtol, log)
+        out["cpu_baseline"] = rec
+        xo = r["x"]
+        no = float(np.linalg.norm(xo))
+        # error of every GPU result against the FP64 oracle: the oracle's own
+        # assembly solved by the reference's Eigen CG to the same rtol
+        out["rel_err_vs_fp64_oracle"] = {
+            "fp64": float(np.linalg.norm(x64 - xo) / no),
+            "mixed": float(np.linalg.norm(xmx - xo) / no),
+            "mixed_mfma32": float(np.linalg.norm(x32m - xo) / no),
+            "oracle_iterations": r["iterations"],
+            "oracle_rel_residual": r["rel_residual"],
+        }
+        out["matrix_rel_diff_vs_oracle"] = float(np.abs(gvals - r["vals"]).max() / np.abs(r["vals"]).max())
+        out["rhs_rel_diff_vs_oracle"] = float(np.abs(rhs_h - r["rhs"]).max() / np.abs(r["rhs"]).max())
+        log(f"vs oracle: {out['rel_err_vs_fp64_oracle']} matrix {out['matrix_rel_diff_vs_oracle']:.2e}")
+    print(json.dumps(sig(out)), flush=True)
 
 
 if __name__ == "__main__":

@@ -253,13 +253,15 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
                     int64_t maxits, int64_t *iters, double *rel_res, int *reason, double *stats_h);
 /* Build the FP32 operator by rounding the FP64 values. */
 int lmh_matrix_make_fp32(lmh_matrix *A);
-/* Column storage of the FP32 operator: on (default) stores, per SpMV row
+/* Column storage of the solvers' SpMV: on (default) stores, per SpMV row
  * block, the block's distinct columns once (16-bit offsets against two bases)
  * and a 16-bit index into that list per entry (6 bytes per entry with the
- * FP32 value instead of 8; x is gathered once per distinct column) whenever
- * every block fits; off keeps the int32 CSR columns.
- * lmh_matrix_fp32_compression reports 1 when the compressed columns are in
- * use, 0 otherwise (builds them if needed). */
+ * FP32 value instead of 8, 10 with the FP64 value instead of 12; x is
+ * gathered once per distinct column) whenever every block fits; off keeps
+ * the int32 CSR columns.  Used by the FP32 inner iterations of
+ * lmh_mixed_solve and by the iterations of lmh_cg_solve; the results are
+ * bit-identical either way.  lmh_matrix_fp32_compression reports 1 when the
+ * compressed columns are in use, 0 otherwise (builds them if needed). */
 int lmh_matrix_set_fp32_compression(lmh_matrix *A, int on);
 int lmh_matrix_fp32_compression(lmh_matrix *A);
 /* Preconditioner of lmh_cg_solve, lmh_bicgstab_solve and lmh_mixed_solve on
@@ -335,9 +337,14 @@ int lmh_profile_reset(lmh_context *ctx);
  * and the values S = A_uu - A_uc A_cc^-1 A_cu, which for DoFs that each live
  * on one element equals the sum of the element Schur complements
  * (static_condensation.C:205-211).  The condensed block must be diagonal (one
- * condensed DoF per coupled block, e.g. a scalar variable on HEX27): otherwise
- * LMH_ERR_UNSUPPORTED; a zero pivot gives LMH_ERR_DEGENERATE.  A keeps its
- * values and must outlive the condensation. */
+ * condensed DoF per coupled block, e.g. a scalar variable on HEX27; exact
+ * zeros between condensed DoFs are admitted): otherwise LMH_ERR_UNSUPPORTED;
+ * a zero pivot gives LMH_ERR_DEGENERATE.  Every column of a condensed row c
+ * must lie, structurally, in every reduced row that holds column c (true for
+ * element-internal DoFs), whatever the values at creation: otherwise
+ * LMH_ERR_UNSUPPORTED.  A keeps its values and must outlive every call on the
+ * condensation except lmh_condensation_destroy, which may follow A's
+ * lmh_matrix_destroy (the context must still exist). */
 
 /* DoFs of the internal nodes of dm's elements (HEX27: local node 26, every
  * variable), element by element.  *n receives the count; dofs_h (may be NULL

@@ -636,9 +636,8 @@ class StaticCondensation:
         self.nnz = hip().lmh_matrix_nnz(self.matrix)
 
     def close(self):
-        """Frees the device data; a no-op once the context or the problem's matrix is gone."""
-        m = getattr(self.problem, "matrix", None)
-        if self.h.value and self.problem.ctx.h and m is not None and m.value:
+        """Frees the device data (the full matrix may be gone already); a no-op once the context is gone."""
+        if self.h.value and self.problem.ctx.h:
             check(hip().lmh_condensation_destroy(self.h))
         self.h = _P()
 

@@ -219,11 +219,16 @@ __global__ __launch_bounds__(256) void k_poisson_hex8(const double *__restrict__
 // ---------------------------------------------------------------------------
 // Q2 HEX27, 3x3x3 Gauss, MFMA contraction.  One element per wave, 4 waves per
 // block, persistent.  Ke (27x27, padded 32x32) = Gref^T (32x84) * H (84x32):
-// tiles (0,0), (0,1), (1,1) on v_mfma_*_16x16x4 (21 k-steps each), tile (1,0)
-// is the transpose of (0,1).  MFMA operand layout (16x16x4): lane l supplies
-// A[l%16][l/16] and B[l/16][l%16]; accumulator element r of lane l holds
-// D[drow(l/16, r)][l%16] (see MfmaT::drow).
-// The constant Gref^T operand is read per k-step from the LDS derivative table.
+// all four 16x16 tiles run on v_mfma_*_16x16x4 (21 k-steps each), so every
+// lane stores its accumulators straight to Ke in row-major order (16
+// consecutive entries per lane group: coalesced), with no LDS staging of Ke.
+// MFMA operand layout (16x16x4): lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; accumulator element r of lane l holds D[drow(l/16, r)][l%16]
+// (see MfmaT::drow).  The constant Gref^T operand is read per k-step from the
+// LDS derivative table; the shape values for Fe come from the FE table in
+// global memory (27x27, cache resident).  LDS holds the derivative table and
+// per wave the coordinates, the 27 M_q and JxW_q f_q: 29 KB per block, so the
+// occupancy is set by registers, not LDS.
 // ---------------------------------------------------------------------------
 template <typename CT>
 struct MfmaT;
@@ -248,33 +253,28 @@ struct MfmaT<float> {
 
 constexpr int Q2N = 27, Q2Q = 27, Q2K = 81, Q2KS = 21;
 
-template <typename CT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_poisson_hex27_mfma(const double *__restrict__ tab,
-                                                            const double *__restrict__ xyz,
-                                                            const int32_t *__restrict__ conn,
-                                                            const int32_t *__restrict__ edofs,
-                                                            const uint8_t *__restrict__ cons,
-                                                            const double *__restrict__ kappa, int f_kind,
-                                                            int constrain, int64_t ne, CT *__restrict__ Ke,
-                                                            double *__restrict__ Fe, int *__restrict__ err) {
+// The forcing kind is a template argument: the f = 1 launch carries none of
+// the FP64 sine's code or registers.
+template <typename CT, int FK>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FK ? 2 : 3))) void k_poisson_hex27_mfma(
+    const double *__restrict__ tab, const double *__restrict__ xyz, const int32_t *__restrict__ conn,
+    const int32_t *__restrict__ edofs, const uint8_t *__restrict__ cons, const double *__restrict__ kappa,
+    int constrain, int64_t ne, CT *__restrict__ Ke, double *__restrict__ Fe, int *__restrict__ err) {
   typedef typename MfmaT<CT>::acc_t acc_t;
   __shared__ double s_w[Q2Q];
-  __shared__ double s_phi[Q2N * Q2Q];
   __shared__ double s_dref[Q2N * Q2Q * 3];
   __shared__ double s_X[4][Q2N * 3];
   __shared__ double s_M[4][Q2Q * 9];
   __shared__ double s_Jf[4][Q2Q];
-  __shared__ CT s_K[4][Q2N * 28];
+  const double *g_phi = tab + Q2Q;  // phi[a * nq + q]
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   for (int i = tid; i < Q2Q; i += 256) s_w[i] = tab[i];
-  for (int i = tid; i < Q2N * Q2Q; i += 256) s_phi[i] = tab[Q2Q + i];
   for (int i = tid; i < Q2N * Q2Q * 3; i += 256) s_dref[i] = tab[Q2Q + Q2N * Q2Q + i];
   __syncthreads();
 
   double *X = s_X[wv];
   double *M = s_M[wv];
   double *Jf = s_Jf[wv];
-  CT *Ks = s_K[wv];
   const int64_t groups = (ne + 3) / 4;
   for (int64_t g = blockIdx.x; g < groups; g += gridDim.x) {
     const int64_t e = g * 4 + wv;
@@ -300,73 +300,66 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
       for (int t = 0; t < 9; ++t) M[lane * 9 + t] = Mq[t];
       double f = 1.0;
-      if (f_kind != 0) {
+      if constexpr (FK != 0) {
         double x = 0, y = 0, z = 0;
         for (int a = 0; a < Q2N; ++a) {
-          const double p = s_phi[a * Q2Q + lane];
+          const double p = g_phi[a * Q2Q + lane];
           x += X[a * 3] * p;
           y += X[a * 3 + 1] * p;
           z += X[a * 3 + 2] * p;
         }
-        f = forcing(f_kind, x, y, z);
+        f = forcing(FK, x, y, z);
       }
       Jf[lane] = jxw * f;
     }
     __syncthreads();
     if (valid) {
-      acc_t c00 = {0, 0, 0, 0}, c01 = {0, 0, 0, 0}, c11 = {0, 0, 0, 0};
+      acc_t c00 = {0, 0, 0, 0}, c01 = {0, 0, 0, 0}, c10 = {0, 0, 0, 0}, c11 = {0, 0, 0, 0};
       const int li = lane & 15, lk = lane >> 4;
       const int j0 = li, j1 = 16 + li;
 #pragma unroll 1
       for (int ks = 0; ks < Q2KS; ++ks) {
         const int k = 4 * ks + lk;
         double b0 = 0.0, b1 = 0.0;
+        CT a0 = 0, a1 = 0;
         if (k < Q2K) {
           const int q = k / 3, r = k - 3 * (k / 3);
           const double m0 = M[q * 9 + r * 3], m1 = M[q * 9 + r * 3 + 1], m2 = M[q * 9 + r * 3 + 2];
           const double *d0 = &s_dref[(j0 * Q2Q + q) * 3];
           b0 = m0 * d0[0] + m1 * d0[1] + m2 * d0[2];
+          a0 = (CT)d0[r];
           if (j1 < Q2N) {
             const double *d1 = &s_dref[(j1 * Q2Q + q) * 3];
             b1 = m0 * d1[0] + m1 * d1[1] + m2 * d1[2];
+            a1 = (CT)d1[r];
           }
         }
-        // constant A operand A[I][ks] = Gref^T[16I + l%16][4ks + l/16], read from the
-        // LDS reference-derivative table
-        CT a0 = 0, a1 = 0;
-        if (k < Q2K) {
-          const int q = k / 3, r = k - 3 * (k / 3);
-          a0 = (CT)s_dref[(j0 * Q2Q + q) * 3 + r];
-          if (j1 < Q2N) a1 = (CT)s_dref[(j1 * Q2Q + q) * 3 + r];
-        }
+        // A[I][ks] = Gref^T[16I + l%16][4ks + l/16] (the same LDS reads as the B operand's)
         c00 = MfmaT<CT>::mma(a0, (CT)b0, c00);
         c01 = MfmaT<CT>::mma(a0, (CT)b1, c01);
+        c10 = MfmaT<CT>::mma(a1, (CT)b0, c10);
         c11 = MfmaT<CT>::mma(a1, (CT)b1, c11);
       }
-      // stage in LDS (row stride 28) with the mirrored (1,0) tile
+      CT *Kr = Ke + e * (Q2N * Q2N);
+      const bool cj0 = (cmask >> j0) & 1ull, cj1 = j1 < Q2N && ((cmask >> j1) & 1ull);
+      auto put = [&](int i, int j, bool cj, CT v) {
+        const bool ri = (cmask >> i) & 1ull;
+        if (ri || cj) v = (i == j && ri) ? (CT)1 : (CT)0;
+        Kr[i * Q2N + j] = v;
+      };
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i0 = MfmaT<CT>::drow(lk, r), i1 = 16 + i0;
-        Ks[i0 * 28 + j0] = c00[r];
-        if (j1 < Q2N) {
-          Ks[i0 * 28 + j1] = c01[r];
-          Ks[j1 * 28 + i0] = c01[r];
-          if (i1 < Q2N) Ks[i1 * 28 + j1] = c11[r];
+        put(i0, j0, cj0, c00[r]);
+        if (j1 < Q2N) put(i0, j1, cj1, c01[r]);
+        if (i1 < Q2N) {
+          put(i1, j0, cj0, c10[r]);
+          if (j1 < Q2N) put(i1, j1, cj1, c11[r]);
         }
-      }
-    }
-    __syncthreads();
-    if (valid) {
-      for (int t = lane; t < Q2N * Q2N; t += 64) {
-        const int i = t / Q2N, j = t - (t / Q2N) * Q2N;
-        CT v = Ks[i * 28 + j];
-        const bool ri = (cmask >> i) & 1ull, cj = (cmask >> j) & 1ull;
-        if (ri || cj) v = (i == j && ri) ? (CT)1 : (CT)0;
-        Ke[e * (Q2N * Q2N) + t] = v;
       }
       if (lane < Q2N) {
         double fe = 0.0;
-        for (int q = 0; q < Q2Q; ++q) fe += Jf[q] * s_phi[lane * Q2Q + q];
+        for (int q = 0; q < Q2Q; ++q) fe += Jf[q] * g_phi[lane * Q2Q + q];
         Fe[e * Q2N + lane] = ((cmask >> lane) & 1ull) ? 0.0 : fe;
       }
     }
@@ -409,6 +402,62 @@ __global__ __launch_bounds__(64) void k_scatter_gather(const int32_t *__restrict
     for (int t = lane; t < len; t += 64) vals[b + t] = acc[t];
     if (rhs && lane == 0) rhs[r] = f;
     __syncthreads();
+  }
+}
+
+// The same gather for element DoF counts up to 32: each half-wave owns a row
+// (two rows per wave), and a row's incidences are taken in chunks of GC
+// whose element matrix rows and slot offsets are all loaded before any is
+// added, so a row waits on one round of loads instead of one per incidence.
+// The adds still go in ascending incidence order (one wave executes its LDS
+// operations in program order, so no barrier is needed between them): the
+// sums are bit-identical to k_scatter_gather's.
+template <typename T>
+__global__ __launch_bounds__(64) void k_scatter_gather2(const int32_t *__restrict__ inc_ptr,
+                                                        const int32_t *__restrict__ inc, int dpe, int64_t n_rows,
+                                                        const int32_t *__restrict__ rp,
+                                                        const uint16_t *__restrict__ off, const T *__restrict__ Ke,
+                                                        const double *__restrict__ Fe, T *__restrict__ vals,
+                                                        double *__restrict__ rhs, int stride) {
+  constexpr int GC = 8;
+  extern __shared__ unsigned char smraw[];
+  const int lane = threadIdx.x, h = lane >> 5, hl = lane & 31;
+  T *acc = (T *)smraw + h * stride;
+  int64_t rlo, rhi;
+  block_range(n_rows, rlo, rhi);
+  for (int64_t rb = rlo; rb < rhi; rb += 2) {
+    const int64_t r = rb + h;
+    const bool live = r < rhi;
+    const int32_t b = live ? rp[r] : 0, len = live ? rp[r + 1] - b : 0;
+    for (int t = hl; t < len; t += 32) acc[t] = (T)0;
+    const int32_t k0 = live ? inc_ptr[r] : 0, k1 = live ? inc_ptr[r + 1] : 0;
+    double f = 0.0;
+    for (int32_t kc = k0; kc < k1; kc += GC) {
+      const int nc = k1 - kc < GC ? k1 - kc : GC;
+      int64_t ei[GC];
+      T v[GC];
+      uint16_t o[GC];
+#pragma unroll
+      for (int c = 0; c < GC; ++c) ei[c] = inc[kc + (c < nc ? c : 0)];
+#pragma unroll
+      for (int c = 0; c < GC; ++c) {
+        const int64_t at = ei[c] * dpe + (hl < dpe ? hl : 0);
+        v[c] = Ke[at];
+        o[c] = off[at];
+      }
+      double fv[GC];
+      if (rhs)
+#pragma unroll
+        for (int c = 0; c < GC; ++c) fv[c] = Fe[ei[c]];
+#pragma unroll
+      for (int c = 0; c < GC; ++c)
+        if (c < nc) {
+          if (hl < dpe) acc[o[c]] += v[c];
+          if (rhs) f += fv[c];
+        }
+    }
+    for (int t = hl; t < len; t += 32) vals[b + t] = acc[t];
+    if (rhs && live && hl == 0) rhs[r] = f;
   }
 }
 
@@ -671,10 +720,15 @@ static int element_poisson_impl(lmh_context *ctx, const lmh_mesh *m, const lmh_d
   const int64_t ne = m->n_elem;
   const int maxg = ctx->cu_count * 8;
   if (m->type == LMH_HEX27 && npts == 3) {
-    const int g = grid_for((ne + 3) / 4, 1, ctx->cu_count * 2);
-    LMH_LAUNCH(ctx, sizeof(CT) == 8 ? "element_hex27_mfma_fp64" : "element_hex27_mfma_fp32",
-               k_poisson_hex27_mfma<CT>, dim3(g), dim3(256), 0, t->d_data, m->xyz, m->conn, dm->edofs, dm->cons, kappa,
-               f_kind, constrain, ne, Ke, Fe, ctx->d_flags);
+    // one resident wave of workgroups: 3 per CU at the f = 1 kernel's registers, 2 with the sine
+    const int g = grid_for((ne + 3) / 4, 1, ctx->cu_count * (f_kind == 0 ? 3 : 2));
+    const char *name = sizeof(CT) == 8 ? "element_hex27_mfma_fp64" : "element_hex27_mfma_fp32";
+    if (f_kind == 0)
+      LMH_LAUNCH(ctx, name, (k_poisson_hex27_mfma<CT, 0>), dim3(g), dim3(256), 0, t->d_data, m->xyz, m->conn,
+                 dm->edofs, dm->cons, kappa, constrain, ne, Ke, Fe, ctx->d_flags);
+    else
+      LMH_LAUNCH(ctx, name, (k_poisson_hex27_mfma<CT, 1>), dim3(g), dim3(256), 0, t->d_data, m->xyz, m->conn,
+                 dm->edofs, dm->cons, kappa, constrain, ne, Ke, Fe, ctx->d_flags);
     return check_launch("k_poisson_hex27_mfma");
   }
   if (m->type == LMH_HEX8 && npts == 2 && sizeof(CT) == 8) {
@@ -697,6 +751,14 @@ template <typename T>
 static int scatter_impl(lmh_matrix *A, const T *Ke, const double *Fe, T *vals, double *rhs, int scatter) {
   lmh_context *ctx = A->ctx;
   const lmh_dofmap *dm = A->dm;
+  const int stride = (int)((A->max_row + 1) & ~1);
+  if (scatter == LMH_SCATTER_GATHER && dm->dpe <= 32 && sizeof(T) * 2 * (size_t)stride <= 32768) {
+    const int g = (int)std::min<int64_t>((A->rows + 1) / 2, (int64_t)ctx->cu_count * LMH_SCATTER_PER_CU);
+    LMH_LAUNCH(ctx, sizeof(T) == 8 ? "scatter_gather_fp64" : "scatter_gather_fp32", k_scatter_gather2<T>, dim3(g),
+               dim3(64), sizeof(T) * 2 * stride, dm->inc_ptr, dm->inc, dm->dpe, A->rows, A->rp, A->off, Ke, Fe, vals,
+               rhs, stride);
+    return check_launch("k_scatter_gather2");
+  }
   if (scatter == LMH_SCATTER_GATHER) {
     const int g = (int)std::min<int64_t>(A->rows, (int64_t)ctx->cu_count * LMH_SCATTER_PER_CU);
     LMH_LAUNCH(ctx, sizeof(T) == 8 ? "scatter_gather_fp64" : "scatter_gather_fp32", k_scatter_gather<T>, dim3(g),

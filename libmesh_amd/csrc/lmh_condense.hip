@@ -24,7 +24,8 @@
 #include "lmh_internal.h"
 
 struct lmh_condensation {
-  lmh_matrix *A = nullptr;  // full operator (not owned)
+  lmh_context *ctx = nullptr;  // A's context: destroy needs no access to A
+  lmh_matrix *A = nullptr;     // full operator (not owned)
   lmh_matrix *R = nullptr;  // reduced operator (owned)
   int64_t n_full = 0, n_cond = 0, n_red = 0;
   int32_t *ridx = nullptr;   // n_full: reduced index, or -1 - q for the q-th condensed DoF
@@ -67,7 +68,11 @@ __global__ __launch_bounds__(256) void k_sc_check(int64_t n_cond, const int32_t 
 
 // Reduced values: copy row i of A without its condensed columns, then for
 // each condensed column c of row i subtract (K_ic / K_cc) * row c, merged
-// into the (sorted) reduced row.
+// into the (sorted) reduced row.  Runs once per condensation: it also checks
+// that every uncondensed column j of every condensed row c of row i lies in
+// reduced row i, structurally (whatever the values), so that later refreshes
+// (k_sc_values_wave, which has no such check) can never drop an entry that
+// becomes nonzero after reassembly.
 __global__ __launch_bounds__(256) void k_sc_values(int64_t n_red, const int32_t *__restrict__ rfull,
                                                    const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
                                                    const double *__restrict__ v, const int32_t *__restrict__ ridx,
@@ -82,18 +87,19 @@ __global__ __launch_bounds__(256) void k_sc_values(int64_t n_red, const int32_t 
       if (ridx[ci[k]] >= 0) sv[p++] = v[k];
     for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
       const int32_t c = ci[k], qc = ridx[c];
-      if (qc >= 0 || v[k] == 0.0) continue;
+      if (qc >= 0) continue;
       const double f = v[k] / v[dpos[-1 - qc]];
+      const bool sub = v[k] != 0.0;
       p = s0;
       for (int32_t m = rp[c]; m < rp[c + 1]; ++m) {
         const int32_t rj = ridx[ci[m]];
         if (rj < 0) continue;
         while (p < s1 && sci[p] < rj) ++p;
         if (p == s1 || sci[p] != rj) {
-          if (v[m] != 0.0) atomicOr(flags, SC_FILL);
+          atomicOr(flags, SC_FILL);
           continue;
         }
-        sv[p] -= f * v[m];
+        if (sub) sv[p] -= f * v[m];
       }
     }
   }
@@ -184,16 +190,22 @@ __global__ __launch_bounds__(256) void k_sc_map(int64_t n_red, const int32_t *__
   }
 }
 
-// x_c = (b_c - sum_{j != c} K_cj x_j) / K_cc, after the uncondensed x_j are in place
+// x_c = (b_c - sum_{j uncondensed} K_cj x_j) / K_cc, after the uncondensed x_j
+// are in place.  Columns of other condensed DoFs are skipped: k_sc_check
+// admits them only as exact zeros, and those x_j are being written by other
+// threads of this launch (their old contents may even be NaN).
 __global__ __launch_bounds__(256) void k_sc_backward(int64_t n_cond, const int32_t *__restrict__ cdof,
                                                      const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                                                     const double *__restrict__ v, const int32_t *__restrict__ dpos,
-                                                     const double *__restrict__ b, double *__restrict__ x) {
+                                                     const double *__restrict__ v, const int32_t *__restrict__ ridx,
+                                                     const int32_t *__restrict__ dpos, const double *__restrict__ b,
+                                                     double *__restrict__ x) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n_cond; q += (int64_t)gridDim.x * blockDim.x) {
     const int32_t c = cdof[q], d = dpos[q];
     double s = b[c];
-    for (int32_t k = rp[c]; k < rp[c + 1]; ++k)
-      if (k != d) s -= v[k] * x[ci[k]];
+    for (int32_t k = rp[c]; k < rp[c + 1]; ++k) {
+      const int32_t j = ci[k];
+      if (ridx[j] >= 0) s -= v[k] * x[j];
+    }
     x[c] = s / v[d];
   }
 }
@@ -274,6 +286,7 @@ int lmh_condensation_create(lmh_matrix *A, int64_t n_cond, const int32_t *cond_d
     srp[r + 1] = (int32_t)sci.size();
   }
   std::unique_ptr<lmh_condensation> sc(new lmh_condensation);
+  sc->ctx = ctx;
   sc->A = A;
   sc->n_full = n;
   sc->n_cond = n_cond;
@@ -292,9 +305,10 @@ int lmh_condensation_create(lmh_matrix *A, int64_t n_cond, const int32_t *cond_d
       hipMalloc((void **)&sc->g, sizeof(double) * nr) != hipSuccess ||
       hipMalloc((void **)&sc->y, sizeof(double) * nr) != hipSuccess)
     return fail(set_error(LMH_ERR_ALLOC, "hipMalloc failed for the condensation"));
-  LMH_HIP(hipMemcpy(sc->ridx, ridx.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
-  LMH_HIP(hipMemcpy(sc->rfull, rfull.data(), sizeof(int32_t) * nr, hipMemcpyHostToDevice));
-  if (n_cond) LMH_HIP(hipMemcpy(sc->cdof, cd.data(), sizeof(int32_t) * n_cond, hipMemcpyHostToDevice));
+  if (hipMemcpy(sc->ridx, ridx.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(sc->rfull, rfull.data(), sizeof(int32_t) * nr, hipMemcpyHostToDevice) != hipSuccess ||
+      (n_cond && hipMemcpy(sc->cdof, cd.data(), sizeof(int32_t) * n_cond, hipMemcpyHostToDevice) != hipSuccess))
+    return fail(set_error(LMH_ERR_HIP, "hipMemcpy failed for the condensation"));
   const int rc = sc_setup(sc.get());
   if (rc != LMH_OK) return fail(rc);
   *out = sc.release();
@@ -334,7 +348,7 @@ int lmh_condensation_backward(lmh_condensation *sc, const double *full_rhs_d, co
   LMH_TRY(check_launch("k_sc_map"));
   if (!sc->n_cond) return LMH_OK;
   LMH_LAUNCH(ctx, "sc_backward", k_sc_backward, dim3(grid_for(sc->n_cond, 256, 8192)), dim3(256), 0, sc->n_cond,
-             sc->cdof, A->rp, A->ci, A->vals, sc->dpos, full_rhs_d, full_sol_d);
+             sc->cdof, A->rp, A->ci, A->vals, sc->ridx, sc->dpos, full_rhs_d, full_sol_d);
   return check_launch("k_sc_backward");
 }
 
@@ -352,7 +366,8 @@ int lmh_condensation_solve(lmh_condensation *sc, const double *b_d, double *x_d,
 
 int lmh_condensation_destroy(lmh_condensation *sc) {
   if (!sc) return LMH_OK;
-  (void)hipStreamSynchronize(sc->A->ctx->stream);
+  (void)hipSetDevice(sc->ctx->device);
+  (void)hipStreamSynchronize(sc->ctx->stream);
   lmh_matrix_destroy(sc->R);
   (void)hipFree(sc->ridx);
   (void)hipFree(sc->rfull);

@@ -229,28 +229,37 @@ __device__ inline void rowblock_spmv(const int32_t *__restrict__ rb, int64_t nrb
 // rowblock_spmv (same products, same summation order, bit-identical sums).
 // Row blocks hold at most RB_CAP - 8 entries, so a block's chunks fit one
 // 8-entry index chunk and two 4-entry value chunks per thread.
+// T = double (the FP64 operator) runs the same scheme with 2-entry value
+// chunks; products and sums are formed as in rowblock_spmv<double>, so the
+// FP64 results are bit-identical to the int32-column kernel too.
 typedef uint32_t vu4 __attribute__((ext_vector_type(4)));
-typedef float vf4 __attribute__((ext_vector_type(4)));
 constexpr int UCAP = 768;  // distinct columns of one row block (x window)
-template <bool DOT>
+template <typename T, bool DOT>
 __device__ inline double rowblock_spmv16(const int32_t *__restrict__ rb, int64_t nrb, const int32_t *__restrict__ rp,
                                          const uint16_t *__restrict__ widx, const uint16_t *__restrict__ woff,
-                                         const int4 *__restrict__ meta, const float *__restrict__ v,
-                                         const float *__restrict__ x, float *__restrict__ y) {
-  __shared__ float prod[RB_CAP + 1];  // + one sink slot for lanes whose chunk slot is outside the block
-  __shared__ float xw[UCAP];
-  __shared__ int32_t rowbuf[2][BLK];  // double-buffered: written before the first barrier
-  __shared__ float xrow[2][BLK];      // x of the block's rows, for the (x, Ax) partials
+                                         const int4 *__restrict__ meta, const T *__restrict__ v,
+                                         const T *__restrict__ x, T *__restrict__ y) {
+  constexpr int CH = 16 / sizeof(T);  // entries per 16-byte value chunk
+  typedef T vch __attribute__((ext_vector_type(CH)));
+  __shared__ T prod[RB_CAP + 1];  // + one sink slot for lanes whose chunk slot is outside the block
+  __shared__ T xw[UCAP];
+  // Row offsets and the x of the block's rows (for the (x, Ax) partials).
+  // FP32: double-buffered, written before the first barrier.  FP64: one
+  // buffer, written after it (the previous block's rows are reduced by then),
+  // which keeps the LDS under 32 KB for 5 resident workgroups per CU.
+  constexpr int NB = sizeof(T) == 8 ? 1 : 2;
+  __shared__ int32_t rowbuf[NB][BLK];
+  __shared__ T xrow[NB][BLK];
   __shared__ vu4 colbuf[BLK];         // 8 window indices per thread
   const uint16_t *cb = (const uint16_t *)colbuf;
-  constexpr int PER = RB_CAP / BLK, UPER = UCAP / BLK;
+  constexpr int PER = RB_CAP / BLK, UPER = UCAP / BLK, NCH = PER / CH;
   static_assert(PER == 8 && UCAP % BLK == 0, "one 8-entry index chunk per thread");
   int64_t lo, hi;
   block_range(nrb, lo, hi);
   const int tid = threadIdx.x;
   int32_t r0 = 0, r1 = 0, k0 = 0, cnt = 0, rpn = 0;
   int4 m{};
-  vf4 vv[PER / 4];  // values of entries a0 + 4 tid + 4 BLK j + e, a0 = block start rounded down to 8
+  vch vv[NCH];  // values of entries a0 + CH tid + CH BLK j + e, a0 = block start rounded down to 8
   uint32_t uo[UPER];
   vu4 ov;
   double acc = 0;
@@ -265,9 +274,9 @@ __device__ inline double rowblock_spmv16(const int32_t *__restrict__ rb, int64_t
     const int32_t a0 = k0 & ~7, span = (k0 - a0) + cnt, nu = m.z >> 16;
     ov = __builtin_nontemporal_load((const vu4 *)(widx + a0 + 8 * (8 * tid < span ? tid : 0)));
 #pragma unroll
-    for (int j = 0; j < PER / 4; ++j) {
-      const int32_t c = 4 * tid + 4 * BLK * j;  // chunks starting past the block re-read chunk 0
-      vv[j] = __builtin_nontemporal_load((const vf4 *)(v + a0 + (c < span ? c : 0)));
+    for (int j = 0; j < NCH; ++j) {
+      const int32_t c = CH * tid + CH * BLK * j;  // chunks starting past the block re-read chunk 0
+      vv[j] = __builtin_nontemporal_load((const vch *)(v + a0 + (c < span ? c : 0)));
     }
 #pragma unroll
     for (int j = 0; j < UPER; ++j) {
@@ -282,53 +291,55 @@ __device__ inline double rowblock_spmv16(const int32_t *__restrict__ rb, int64_t
   int buf = 0;
   auto body = [&](int64_t bk, auto more) {
     int32_t *rows = rowbuf[buf];
-    float *xr = xrow[buf];
-    buf ^= 1;
+    T *xr = xrow[buf];
+    buf ^= NB - 1;
     const int32_t R0 = r0, R1 = r1, K0 = k0, CNT = cnt, RPN = rpn, lead = K0 & 7;
     const int4 M = m;
     const int32_t NU = M.z >> 16, S0 = M.z & 0xffff;
     const bool staged = R1 - R0 < BLK;
-    float xu[UPER];
+    T xu[UPER];
 #pragma unroll
     for (int j = 0; j < UPER; ++j) {
       const int32_t u = tid + j * BLK, o = (int32_t)uo[j];
       xu[j] = x[u < NU ? o + (o <= S0 ? M.x : M.y) : 0];
     }
-    float xo = 0;
+    T xo = 0;
     if (DOT) xo = x[R0 + (tid < R1 - R0 ? tid : 0)];
     colbuf[tid] = ov;
-    if (staged && tid <= R1 - R0) rows[tid] = RPN - K0;
-    vf4 cv[PER / 4];
+    if (NB == 2 && staged && tid <= R1 - R0) rows[tid] = RPN - K0;
+    vch cv[NCH];
 #pragma unroll
-    for (int j = 0; j < PER / 4; ++j) cv[j] = vv[j];
+    for (int j = 0; j < NCH; ++j) cv[j] = vv[j];
     if constexpr (decltype(more)::value) fetch(bk + 1);
 #pragma unroll
     for (int j = 0; j < UPER; ++j)
       if (tid + j * BLK < NU) xw[tid + j * BLK] = xu[j];
-    if (DOT) xr[tid] = xo;
+    if (NB == 2 && DOT) xr[tid] = xo;
     __syncthreads();  // window, indices and row data visible; the previous block's rows are reduced
-    float xs[PER];
+    if (NB == 1 && staged && tid <= R1 - R0) rows[tid] = RPN - K0;
+    if (NB == 1 && DOT) xr[tid] = xo;
+    T xs[PER];
 #pragma unroll
-    for (int j = 0; j < PER / 4; ++j)
+    for (int j = 0; j < NCH; ++j)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int32_t pos = 4 * tid + 4 * BLK * j + e, t = pos - lead;
+      for (int e = 0; e < CH; ++e) {
+        const int32_t pos = CH * tid + CH * BLK * j + e, t = pos - lead;
         const int32_t wi = cb[pos];
-        xs[4 * j + e] = xw[t >= 0 && t < CNT ? wi : 0];
+        xs[CH * j + e] = xw[t >= 0 && t < CNT ? wi : 0];
       }
 #pragma unroll
-    for (int j = 0; j < PER / 4; ++j)
+    for (int j = 0; j < NCH; ++j)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int32_t pos = 4 * tid + 4 * BLK * j + e, t = pos - lead;
-        prod[t >= 0 && t < CNT ? t : RB_CAP] = cv[j][e] * xs[4 * j + e];
+      for (int e = 0; e < CH; ++e) {
+        const int32_t pos = CH * tid + CH * BLK * j + e, t = pos - lead;
+        prod[t >= 0 && t < CNT ? t : RB_CAP] = cv[j][e] * xs[CH * j + e];
       }
     __syncthreads();
     const int g = tid >> 3, sub = tid & 7;
     if (staged) {
       for (int32_t r = R0 + g; r < R1; r += BLK / 8) {
         const int32_t a = rows[r - R0], e = rows[r - R0 + 1];
-        float s = 0;
+        T s = 0;
         for (int32_t t = a + sub; t < e; t += 8) s += prod[t];
         s += __shfl_xor(s, 4, 8);
         s += __shfl_xor(s, 2, 8);
@@ -341,7 +352,7 @@ __device__ inline double rowblock_spmv16(const int32_t *__restrict__ rb, int64_t
     } else {
       for (int32_t r = R0 + g; r < R1; r += BLK / 8) {
         const int32_t a = rp[r] - K0, e = rp[r + 1] - K0;
-        float s = 0;
+        T s = 0;
         for (int32_t t = a + sub; t < e; t += 8) s += prod[t];
         s += __shfl_xor(s, 4, 8);
         s += __shfl_xor(s, 2, 8);
@@ -501,7 +512,20 @@ __global__ __launch_bounds__(BLK, 8) void k_it_spmv16(const int32_t *__restrict_
                                                    float *__restrict__ q, double *__restrict__ part,
                                                    const double *__restrict__ S, int bank) {
   if (S[bank * B_SIZE + B_DONE] != 0.0) return;
-  double acc[1] = {rowblock_spmv16<true>(rb, nrb, rp, widx, woff, meta, v, p, q)};
+  double acc[1] = {rowblock_spmv16<float, true>(rb, nrb, rp, widx, woff, meta, v, p, q)};
+  publish<1>(acc, part);
+}
+
+// q = A p on the FP64 operator through the same x windows; partials (p, q).
+// Bit-identical to k_it_spmv<double>.
+__global__ __launch_bounds__(BLK) void k_it_spmv16d(const int32_t *__restrict__ rb, int64_t nrb,
+                                                    const int32_t *__restrict__ rp, const uint16_t *__restrict__ widx,
+                                                    const uint16_t *__restrict__ woff, const int4 *__restrict__ meta,
+                                                    const double *__restrict__ v, const double *__restrict__ p,
+                                                    double *__restrict__ q, double *__restrict__ part,
+                                                    const double *__restrict__ S, int bank) {
+  if (S[bank * B_SIZE + B_DONE] != 0.0) return;
+  double acc[1] = {rowblock_spmv16<double, true>(rb, nrb, rp, widx, woff, meta, v, p, q)};
   publish<1>(acc, part);
 }
 
@@ -510,7 +534,7 @@ __global__ __launch_bounds__(BLK, 8) void k_spmv16(const int32_t *__restrict__ r
                                                 const uint16_t *__restrict__ woff, const int4 *__restrict__ meta,
                                                 const float *__restrict__ v, const float *__restrict__ x,
                                                 float *__restrict__ y) {
-  rowblock_spmv16<false>(rb, nrb, rp, widx, woff, meta, v, x, y);
+  rowblock_spmv16<float, false>(rb, nrb, rp, widx, woff, meta, v, x, y);
 }
 
 // Build the x windows, one row block per workgroup: sort the block's
@@ -803,12 +827,13 @@ __global__ __launch_bounds__(BLK) void k_refresh_fin(const double *__restrict__ 
     Sb[B_RZ] = t[2];
     Sb[B_RR] = rr;
     Sb[B_MAXRR] = rr;
-    if (t[1] == 0.0 || rr <= S[G_THR]) {
-      Sb[B_DONE] = 1;
-      Sb[B_REASON] = LMH_CONVERGED_RTOL;
-    } else if (!isfinite(rr) || !isfinite(t[1])) {
+    // non-finite first: an Inf in b makes the threshold Inf too, and rr <= Inf would pass
+    if (!isfinite(rr) || !isfinite(t[1])) {
       Sb[B_DONE] = 1;
       Sb[B_REASON] = LMH_DIVERGED_NANORINF;
+    } else if (t[1] == 0.0 || rr <= S[G_THR]) {
+      Sb[B_DONE] = 1;
+      Sb[B_REASON] = LMH_CONVERGED_RTOL;
     }
     return;
   }
@@ -1222,6 +1247,8 @@ int lmh_cg_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int
   LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
   lmh_context *ctx = A->ctx;
   LMH_TRY(ensure_work(A, false));
+  LMH_TRY(ensure_windows(A));
+  const bool c16 = use_windows(A);
   const int64_t n = A->rows;
   double *r = A->w64, *p = r + n, *q = p + n, *dinv = q + n;
   double *S = ctx->d_scal;
@@ -1245,8 +1272,12 @@ int lmh_cg_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int
     const int64_t nb = std::min<int64_t>(batch, maxits - K);
     for (int64_t i = 0; i < nb; ++i, ++K) {
       const int bank = (int)(K & 1);
-      LMH_LAUNCH(ctx, "cg_spmv_fp64", k_it_spmv<double>, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci,
-                 A->vals, p, q, part1, S, bank);
+      if (c16)
+        LMH_LAUNCH(ctx, "cg_spmv_fp64", k_it_spmv16d, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->widx16,
+                   A->woff16, A->rbmeta, A->vals, p, q, part1, S, bank);
+      else
+        LMH_LAUNCH(ctx, "cg_spmv_fp64", k_it_spmv<double>, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci,
+                   A->vals, p, q, part1, S, bank);
       LMH_LAUNCH(ctx, "cg_update_fp64", (k_it_update<double, double>), dim3(g), dim3(BLK), 0, n, part1, gs, dinv, p,
                  q, x_d, r, part2, S, bank);
       LMH_LAUNCH(ctx, "cg_pupdate_fp64", k_it_pupdate<double>, dim3(g), dim3(BLK), 0, n, part2, g, dinv, r, p, S, bank,

@@ -8,8 +8,11 @@
 // TEST INFRASTRUCTURE ONLY: built by oracle/Makefile against the headers under
 // the reference tree into oracle/_ref/, and used to pin the oracle's CG
 // restatement and as the "reference" CPU baseline.
+#include <omp.h>
+
 #include <Eigen/IterativeLinearSolvers>
 #include <Eigen/Sparse>
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 
@@ -90,5 +93,36 @@ extern "C" int ref_eigen_bicgstab_identity(int64_t n, const int32_t *rp, const i
   std::memcpy(x, xx.data(), sizeof(double) * n);
   *iters = solver.iterations();
   *err = solver.error();
+  return (int)solver.info();
+}
+
+// The CG + JACOBI_PRECOND solve above with the matrix copy kept out of the
+// clock: the solve alone (solveWithGuess) is timed with steady_clock on
+// `threads` OpenMP threads (Eigen's row-major sparse * dense product is the
+// threaded part, SparseDenseProduct.h:41-51; vector operations stay serial).
+// Used as the CPU baseline's solve phase.
+
+extern "C" int ref_eigen_cg_timed(int64_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *b,
+                                  double *x, double tol, int64_t maxits, int threads, int64_t *iters, double *err,
+                                  double *seconds) {
+  Eigen::Map<const EigenSM> view((Eigen::Index)n, (Eigen::Index)n, (Eigen::Index)rp[n], rp, ci, v);
+  EigenSM A(view);
+  Eigen::Map<const Eigen::VectorXd> bb(b, n);
+  Eigen::VectorXd xx = Eigen::Map<const Eigen::VectorXd>(x, n);
+  const int before = omp_get_max_threads();
+  omp_set_num_threads(threads > 0 ? threads : before);
+  Eigen::setNbThreads(threads > 0 ? threads : before);
+  Eigen::ConjugateGradient<EigenSM, Eigen::Lower | Eigen::Upper, Eigen::DiagonalPreconditioner<double>> solver(A);
+  solver.setMaxIterations((Eigen::Index)maxits);
+  solver.setTolerance(tol);
+  const auto t0 = std::chrono::steady_clock::now();
+  xx = solver.solveWithGuess(bb, xx);
+  const auto t1 = std::chrono::steady_clock::now();
+  omp_set_num_threads(before);
+  Eigen::setNbThreads(before);
+  std::memcpy(x, xx.data(), sizeof(double) * n);
+  *iters = solver.iterations();
+  *err = solver.error();
+  *seconds = std::chrono::duration<double>(t1 - t0).count();
   return (int)solver.info();
 }

@@ -107,6 +107,8 @@ def ref():
         for name in ("ref_eigen_cg_identity", "ref_eigen_bicgstab_identity"):
             getattr(R, name).restype = C.c_int
             getattr(R, name).argtypes = [_I64, _P, _P, _P, _P, _P, _D, _I64, _P, _P]
+        R.ref_eigen_cg_timed.restype = C.c_int
+        R.ref_eigen_cg_timed.argtypes = [_I64, _P, _P, _P, _P, _P, _D, _I64, C.c_int, _P, _P, _P]
         R.ref_eigen_spmv.restype = None
         R.ref_eigen_spmv.argtypes = [_I64, _P, _P, _P, _P, _P]
         _ref = R
@@ -359,6 +361,20 @@ def ref_cg_eigen(rp, ci, vals, b, tol=1e-12, maxits=5000, x0=None, precond="jaco
         return None
     fn = R.ref_eigen_cg if precond == "jacobi" else R.ref_eigen_cg_identity
     return _cg(fn, rp, ci, vals, b, tol, maxits, x0)
+
+
+def ref_cg_eigen_timed(rp, ci, vals, b, tol, maxits, threads, x0=None):
+    """The reference's Eigen CG + Jacobi with only the solve on the clock:
+    (x, iterations, error, seconds), or None without oracle/_ref."""
+    R = ref()
+    if R is None:
+        return None
+    n = len(b)
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    its, err, sec = C.c_int64(), C.c_double(), C.c_double()
+    R.ref_eigen_cg_timed(n, _ptr(rp), _ptr(ci), _ptr(vals), _ptr(np.ascontiguousarray(b)), _ptr(x), tol, maxits,
+                         threads, C.byref(its), C.byref(err), C.byref(sec))
+    return x, its.value, err.value, sec.value
 
 
 def spmv(rp, ci, vals, x):

@@ -132,3 +132,57 @@ def test_no_internal_nodes_and_errors(ctx):
         L.StaticCondensation(g, np.array([i, i], dtype=np.int32))
     with pytest.raises(L.LmhError, match="out of range"):
         L.StaticCondensation(g, np.array([p.n_dofs], dtype=np.int32))
+
+
+def _csr_problem(ctx, K, pattern):
+    """A matrix with a caller-given pattern (lmh_matrix_create_csr) and K's values on it."""
+    import types
+
+    rows = [np.flatnonzero(pattern[i]).astype(np.int32) for i in range(K.shape[0])]
+    rp = np.zeros(K.shape[0] + 1, dtype=np.int32)
+    rp[1:] = np.cumsum([r.size for r in rows])
+    ci = np.concatenate(rows).astype(np.int32)
+    h = L._P()
+    L.check(L.hip().lmh_matrix_create_csr(ctx.h, K.shape[0], L._ptr(rp), L._ptr(ci), C.byref(h)))
+    v = np.ascontiguousarray(np.concatenate([K[i, r] for i, r in enumerate(rows)]))
+    L.check(L.hip().lmh_matrix_set_values(h, L._ptr(v)))
+    return types.SimpleNamespace(ctx=ctx, matrix=h), rp, ci
+
+
+def test_structural_zero_coupling_and_structural_fill(ctx):
+    """Condensed DoFs 0 and 1 share a structural entry holding an exact zero (the
+    condensed block is still diagonal, so it is admitted).  Back substitution
+    must not read the other condensed DoF's output: the output vector starts as
+    NaN.  A condensed row reaching a column outside a reduced row it couples to
+    is refused at creation even when that entry is zero then, since a later
+    reassembly could make it nonzero and the refresh kernel has no such check."""
+    n = 6
+    rng = np.random.default_rng(3)
+    M = rng.standard_normal((n, n))
+    K = M @ M.T + n * np.eye(n)
+    K[0, 1] = K[1, 0] = 0.0
+    prob, rp, ci = _csr_problem(ctx, K, np.ones((n, n), dtype=bool))
+    sc = L.StaticCondensation(prob, np.array([1, 0], dtype=np.int32))
+    S = K[2:, 2:] - K[2:, :2] @ np.linalg.inv(K[:2, :2]) @ K[:2, 2:]
+    assert rel(sc.values(), S.ravel()) < 1e-13
+    b = rng.standard_normal(n)
+    bd = L.DeviceArray(ctx, n, data=b)
+    g = L.DeviceArray(ctx, n - 2)
+    sc.forward(bd, g)
+    assert rel(g.download(), b[2:] - K[2:, :2] @ np.linalg.solve(K[:2, :2], b[:2])) < 1e-13
+    yd = L.DeviceArray(ctx, n - 2, data=np.linalg.solve(S, g.download()))
+    x = L.DeviceArray(ctx, n, data=np.full(n, np.nan))
+    sc.backward(bd, yd, x)
+    assert rel(x.download(), np.linalg.solve(K, b)) < 1e-12
+    sc.close()
+    L.check(L.hip().lmh_matrix_destroy(prob.matrix))
+    # row 0 couples to column 5, reduced row 2 (which holds column 0) does not: fill,
+    # refused although K[0, 5] is zero now
+    pat = np.ones((n, n), dtype=bool)
+    pat[2, 5] = pat[5, 2] = False
+    K2 = K.copy()
+    K2[0, 5] = K2[5, 0] = 0.0
+    prob, rp, ci = _csr_problem(ctx, K2, pat)
+    with pytest.raises(L.LmhError, match="outside the matrix pattern"):
+        L.StaticCondensation(prob, np.array([0], dtype=np.int32))
+    L.check(L.hip().lmh_matrix_destroy(prob.matrix))

@@ -87,7 +87,7 @@ def test_zero_rhs(ctx):
 
 def test_solver_failure_reasons(ctx):
     """Reasons the solvers report instead of a result: an iteration limit
-    (DIVERGED_ITS with the residual reached), a NaN in the right-hand side
+    (DIVERGED_ITS with the residual reached), a NaN or Inf in the right-hand side
     (DIVERGED_NANORINF before the first iteration, as PETSc reports it), and a
     zero operator (DIVERGED_BREAKDOWN: (p, Ap) = 0).  Every solve returns."""
     p, s, g = pair(ctx, L.HEX27, 4)
@@ -102,14 +102,17 @@ def test_solver_failure_reasons(ctx):
     x.zero()
     its, res, reason, st = g.mixed(b, x, 1e-14, 0.1, 0, 3)
     assert reason == L.DIVERGED_ITS and res > 1e-14
-    bh = b.download()
-    bh[p.n_dofs // 2] = np.nan
-    bn = L.DeviceArray(ctx, p.n_dofs, data=bh)
-    for solve in (lambda: g.cg(bn, x, 1e-10, 100), lambda: g.bicgstab(bn, x, 1e-10, 100),
-                  lambda: g.mixed(bn, x, 1e-10, 0.1, 0, 100)[:3]):
-        x.zero()
-        its, res, reason = solve()
-        assert reason == L.DIVERGED_NANORINF and its == 0, (its, reason)
+    for bad in (np.nan, np.inf, -np.inf):
+        # an Inf makes |b|^2 and the threshold rtol^2 |b|^2 infinite: rr <= thr must not pass
+        bh = b.download()
+        bh[p.n_dofs // 2] = bad
+        bn = L.DeviceArray(ctx, p.n_dofs, data=bh)
+        for name, solve in (("cg", lambda: g.cg(bn, x, 1e-10, 100)), ("bicgstab", lambda: g.bicgstab(bn, x, 1e-10, 100)),
+                            ("mixed", lambda: g.mixed(bn, x, 1e-10, 0.1, 0, 100)[:3]),
+                            ("mixed-classic", lambda: g.mixed(bn, x, 1e-10, 1e-5, 1, 100)[:3])):
+            x.zero()
+            its, res, reason = solve()
+            assert reason == L.DIVERGED_NANORINF and its == 0, (bad, name, its, reason)
     g.set_values(np.zeros(g.nnz))
     x.zero()
     its, res, reason = g.cg(b, x, 1e-10, 100)
