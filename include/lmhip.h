@@ -221,6 +221,22 @@ int lmh_axpy(lmh_context *ctx, int64_t n, double a, const double *x_d, double *y
  * the iteration count and sqrt(|r|^2/|b|^2) exactly as Eigen reports them. */
 int lmh_cg_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int64_t maxits, int64_t *iters,
                  double *rel_res, int *reason);
+/* Allocation-free lmh_cg_solve.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * stopping rule and results as lmh_cg_solve; it calls no allocator.  The work
+ * vectors (r, p, q, 1/a_ii) and the diagonal slots live in one fixed device
+ * array of the library sized for LMH_CG_NOALLOC_MAX_ROWS rows (4 doubles and
+ * one int32 per row: 144 MiB), which every GPU the library runs on holds from
+ * its first context until the process ends.  A matrix with more rows:
+ * LMH_ERR_INVALID.  A's compressed column windows are used when an earlier
+ * lmh_cg_solve or lmh_mixed_solve on A built them (the iterates are the same
+ * either way).  All calls on one GPU share the array and keep the last call's
+ * contents: calls that overlap on one GPU (from several contexts or threads)
+ * race.  Not recorded by lmh_profile_enable. */
+#define LMH_CG_NOALLOC_MAX_ROWS 4194304
+int lmh_cg_solve_noalloc(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int64_t maxits,
+                         int64_t *iters, double *rel_res, int *reason);
 
 /* Jacobi-preconditioned BiCGSTAB, FP64 (EigenSparseLinearSolver::solve with
  * BICGSTAB + JACOBI_PRECOND -> Eigen::BiCGSTAB, eigen_sparse_linear_solver.C:175-205,

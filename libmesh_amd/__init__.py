@@ -29,6 +29,7 @@ ALL_FACES = 0x3F
 CONVERGED_RTOL, DIVERGED_ITS = 2, -3
 DIVERGED_BREAKDOWN, DIVERGED_NANORINF = -5, -9  # LMH_DIVERGED_* (include/lmhip.h)
 PRECOND_IDENTITY, PRECOND_JACOBI = 0, 1  # libMesh PreconditionerType values
+CG_NOALLOC_MAX_ROWS = 4194304  # LMH_CG_NOALLOC_MAX_ROWS (include/lmhip.h)
 
 _P = C.c_void_p
 _I64 = C.c_int64
@@ -92,6 +93,7 @@ HIP_API = {
     "lmh_residual": (_I, [_P, _P, _P, _P]),
     "lmh_norm2": (_D, [_P, _P, _I64]),
     "lmh_cg_solve": (_I, [_P, _P, _P, _D, _I64, C.POINTER(_I64), C.POINTER(_D), C.POINTER(_I)]),
+    "lmh_cg_solve_noalloc": (_I, [_P, _P, _P, _D, _I64, C.POINTER(_I64), C.POINTER(_D), C.POINTER(_I)]),
     "lmh_mixed_solve": (_I, [_P, _P, _P, _D, _D, _I, _I64, C.POINTER(_I64), C.POINTER(_D), C.POINTER(_I), _P]),
     "lmh_matrix_make_fp32": (_I, [_P]),
     "lmh_matrix_set_fp32_compression": (_I, [_P, _I]),
@@ -476,6 +478,16 @@ class GpuProblem:
         reason = _I()
         check(hip().lmh_cg_solve(self.matrix, b_d.ptr, x_d.ptr, rtol, maxits, C.byref(its), C.byref(res),
                                  C.byref(reason)))
+        return its.value, res.value, reason.value
+
+    def cg_noalloc(self, b_d, x_d, rtol, maxits=5000):
+        """cg without device allocation: work vectors in the library's fixed
+        scratch array (at most CG_NOALLOC_MAX_ROWS rows)."""
+        its = _I64()
+        res = _D()
+        reason = _I()
+        check(hip().lmh_cg_solve_noalloc(self.matrix, b_d.ptr, x_d.ptr, rtol, maxits, C.byref(its), C.byref(res),
+                                         C.byref(reason)))
         return its.value, res.value, reason.value
 
     def condensed_dofs(self):

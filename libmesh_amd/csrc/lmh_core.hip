@@ -349,6 +349,9 @@ int lmh_context_create(int device, lmh_context **out) {
   LMH_HIP(hipMalloc((void **)&c->d_flags, 16 * sizeof(int)));
   LMH_HIP(hipMemset(c->d_flags, 0, 16 * sizeof(int)));
   LMH_HIP(hipHostMalloc((void **)&c->h_pinned, 64 * sizeof(double), hipHostMallocDefault));
+  LMH_HIP(hipEventCreateWithFlags(&c->ev_poll[0], hipEventDisableTiming));
+  LMH_HIP(hipEventCreateWithFlags(&c->ev_poll[1], hipEventDisableTiming));
+  LMH_TRY(cg_scratch_setup(c.get()));
   *out = c.release();
   return LMH_OK;
 }
@@ -364,6 +367,8 @@ int lmh_context_destroy(lmh_context *ctx) {
       (void)hipEventDestroy(pr.second);
     }
   for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+  for (auto e : ctx->ev_poll)
+    if (e) (void)hipEventDestroy(e);
   (void)hipFree(ctx->d_scal);
   (void)hipFree(ctx->d_part);
   (void)hipFree(ctx->d_count);

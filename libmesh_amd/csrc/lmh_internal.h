@@ -26,6 +26,9 @@ constexpr int64_t LMH_PAD = 8;
 #ifndef LMH_SPMV_GRID_PER_CU
 #define LMH_SPMV_GRID_PER_CU 16
 #endif
+// devices whose scratch addresses the library keeps
+#define LMH_MAX_DEVICES 64
+
 #ifndef LMH_GRID_PER_CU
 #define LMH_GRID_PER_CU 4
 #endif
@@ -90,6 +93,8 @@ struct lmh_context {
   unsigned *d_count = nullptr;  // last-block counters (zeroed, self-resetting)
   int *d_flags = nullptr;       // error flags
   double *h_pinned = nullptr;   // pinned host scalars
+  hipEvent_t ev_poll[2] = {nullptr, nullptr};  // solver convergence polls
+  void *cg_scratch = nullptr;   // this device's address of the allocation-free CG's scratch
   int n_part = 0;
   int cu_count = 256;
 };
@@ -217,6 +222,8 @@ hipEvent_t get_event(lmh_context *ctx);
   } while (0)
 
 int check_launch(const char *what);
+// per-device address of the allocation-free CG's scratch into ctx (context creation)
+int cg_scratch_setup(lmh_context *ctx);
 
 // device helpers shared across translation units
 int exclusive_scan_i32(lmh_context *ctx, const int32_t *in, int32_t *out, int64_t n, int64_t *total);

@@ -12,6 +12,7 @@
 #include <climits>
 #include <cmath>
 #include <functional>
+#include <mutex>
 #include <type_traits>
 
 #include "lmh_internal.h"
@@ -1239,31 +1240,66 @@ struct Poll {
   int64_t end = 0;  // iterations launched when the copy was enqueued
 };
 
-int lmh_cg_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int64_t maxits, int64_t *iters,
-                 double *rel_res, int *reason) {
-  LMH_CHECK(A && b_d && x_d, LMH_ERR_INVALID, "null argument");
-  LMH_CHECK(rtol >= 0 && maxits >= 0, LMH_ERR_INVALID, "bad tolerance or iteration limit");
-  LMH_CHECK(A->rows == A->cols, LMH_ERR_INVALID, "CG needs a square matrix");
-  LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
+}  // extern "C"
+
+namespace lmh {
+
+// Work vectors of the allocation-free CG: r, p, q, 1/a_ii (4 x LMH_CG_NOALLOC_MAX_ROWS
+// doubles), then the diagonal slots (LMH_CG_NOALLOC_MAX_ROWS int32).  Each GPU
+// holds its own copy until the process ends; host code reaches it only through
+// the address hipGetSymbolAddress returns for that device (cg_scratch_setup).
+static __device__ __align__(256) double scratch[4 * (size_t)LMH_CG_NOALLOC_MAX_ROWS + LMH_CG_NOALLOC_MAX_ROWS / 2];
+
+static std::mutex g_scratch_mutex;
+static void *g_scratch_ptr[LMH_MAX_DEVICES] = {};
+
+int cg_scratch_setup(lmh_context *ctx) {
+  LMH_CHECK(ctx->device >= 0 && ctx->device < LMH_MAX_DEVICES, LMH_ERR_UNSUPPORTED, "device %d beyond %d",
+            ctx->device, LMH_MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[ctx->device]) {
+    // the caller made ctx->device current
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return set_error(LMH_ERR_HIP, "hipGetSymbolAddress of the CG scratch failed on device %d: %s", ctx->device,
+                       hipGetErrorString(e));
+    g_scratch_ptr[ctx->device] = ptr;
+  }
+  ctx->cg_scratch = g_scratch_ptr[ctx->device];
+  return LMH_OK;
+}
+
+// Jacobi/identity CG iterations on A (FP64).  r, p, q, dinv: n doubles each;
+// diag: A's diagonal slots.  NA (no allocation): launches without the
+// profiling events and uses the x windows only when they are already built.
+template <bool NA>
+static int cg_impl(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int64_t maxits, double *r, double *p,
+                   double *q, double *dinv, const int32_t *diag, int64_t *iters, double *rel_res, int *reason) {
+#define CG_LAUNCH(name, kernel, grid, ...)                                                   \
+  do {                                                                                      \
+    if (NA)                                                                                 \
+      hipLaunchKernelGGL(kernel, grid, dim3(BLK), 0, ctx->stream, __VA_ARGS__);             \
+    else                                                                                    \
+      LMH_LAUNCH(ctx, name, kernel, grid, dim3(BLK), 0, __VA_ARGS__);                       \
+  } while (0)
   lmh_context *ctx = A->ctx;
-  LMH_TRY(ensure_work(A, false));
-  LMH_TRY(ensure_windows(A));
+  if (!NA) LMH_TRY(ensure_windows(A));
   const bool c16 = use_windows(A);
   const int64_t n = A->rows;
-  double *r = A->w64, *p = r + n, *q = p + n, *dinv = q + n;
   double *S = ctx->d_scal;
   double *part1 = ctx->d_part, *part2 = ctx->d_part + 2 * ctx->n_part;  // 1 and 2 values per block
   const int g = vec_grid(ctx), gs = spmv_grid(ctx);
-  LMH_TRY(ensure_diag(A));
-  LMH_LAUNCH(ctx, "invdiag", (k_invdiag<double, double>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, n, A->diag,
-             A->vals, A->precond == LMH_PRECOND_JACOBI, dinv);
-  LMH_LAUNCH(ctx, "residual_fp64", k_residual, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci, A->vals, b_d,
-             x_d, r, part2);
-  LMH_LAUNCH(ctx, "cg_init", k_cg_init, dim3(g), dim3(BLK), 0, n, dinv, r, p, part1);
-  LMH_LAUNCH(ctx, "cg_init", k_cg_init_fin, dim3(g), dim3(BLK), 0, part2, gs, part1, g, rtol, S, x_d, n);
+  CG_LAUNCH("invdiag", (k_invdiag<double, double>), dim3(grid_for(n, BLK, 8192)), n, diag, A->vals,
+            A->precond == LMH_PRECOND_JACOBI, dinv);
+  CG_LAUNCH("residual_fp64", k_residual, dim3(gs), A->rb, A->nrb, A->rp, A->ci, A->vals, b_d, x_d, r, part2);
+  CG_LAUNCH("cg_init", k_cg_init, dim3(g), n, dinv, r, p, part1);
+  CG_LAUNCH("cg_init", k_cg_init_fin, dim3(g), part2, gs, part1, g, rtol, S, x_d, n);
   LMH_TRY(check_launch("cg setup"));
+  // The scalar banks are copied to pinned memory after every batch of
+  // iterations; the host reads a batch's copy while the next batch runs.
   const int batch = 4;
-  Poll poll[2];
+  int64_t end[2] = {0, 0};
   double *snap = ctx->h_pinned;  // 2 x (2 banks)
   int64_t K = 0;
   int slot = 0;
@@ -1273,41 +1309,76 @@ int lmh_cg_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int
     for (int64_t i = 0; i < nb; ++i, ++K) {
       const int bank = (int)(K & 1);
       if (c16)
-        LMH_LAUNCH(ctx, "cg_spmv_fp64", k_it_spmv16d, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->widx16,
-                   A->woff16, A->rbmeta, A->vals, p, q, part1, S, bank);
+        CG_LAUNCH("cg_spmv_fp64", k_it_spmv16d, dim3(gs), A->rb, A->nrb, A->rp, A->widx16, A->woff16, A->rbmeta,
+                  A->vals, p, q, part1, S, bank);
       else
-        LMH_LAUNCH(ctx, "cg_spmv_fp64", k_it_spmv<double>, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci,
-                   A->vals, p, q, part1, S, bank);
-      LMH_LAUNCH(ctx, "cg_update_fp64", (k_it_update<double, double>), dim3(g), dim3(BLK), 0, n, part1, gs, dinv, p,
-                 q, x_d, r, part2, S, bank);
-      LMH_LAUNCH(ctx, "cg_pupdate_fp64", k_it_pupdate<double>, dim3(g), dim3(BLK), 0, n, part2, g, dinv, r, p, S, bank,
-                 0);
+        CG_LAUNCH("cg_spmv_fp64", k_it_spmv<double>, dim3(gs), A->rb, A->nrb, A->rp, A->ci, A->vals, p, q, part1, S,
+                  bank);
+      CG_LAUNCH("cg_update_fp64", (k_it_update<double, double>), dim3(g), n, part1, gs, dinv, p, q, x_d, r, part2, S,
+                bank);
+      CG_LAUNCH("cg_pupdate_fp64", k_it_pupdate<double>, dim3(g), n, part2, g, dinv, r, p, S, bank, 0);
     }
     LMH_HIP(hipMemcpyAsync(snap + 32 * slot, S, sizeof(double) * 2 * B_SIZE, hipMemcpyDeviceToHost, ctx->stream));
-    if (!poll[slot].ev) poll[slot].ev = get_event(ctx);
-    LMH_HIP(hipEventRecord(poll[slot].ev, ctx->stream));
-    poll[slot].end = K;
+    LMH_HIP(hipEventRecord(ctx->ev_poll[slot], ctx->stream));
+    end[slot] = K;
     if (pending) {
       const int o = slot ^ 1;
-      LMH_HIP(hipEventSynchronize(poll[o].ev));
-      done = snap[32 * o + (poll[o].end & 1) * B_SIZE + B_DONE] != 0.0;
+      LMH_HIP(hipEventSynchronize(ctx->ev_poll[o]));
+      done = snap[32 * o + (end[o] & 1) * B_SIZE + B_DONE] != 0.0;
     }
     pending = true;
     slot ^= 1;
   }
-  LMH_HIP(hipStreamSynchronize(ctx->stream));
+#undef CG_LAUNCH
   LMH_TRY(check_launch("cg iterations"));
-  for (auto &pl : poll)
-    if (pl.ev) ctx->event_pool.push_back(pl.ev);
-  double h[2 * B_SIZE], gbb = 0;
-  LMH_HIP(hipMemcpy(h, S, sizeof(h), hipMemcpyDeviceToHost));
-  LMH_HIP(hipMemcpy(&gbb, S + G_BB, sizeof(double), hipMemcpyDeviceToHost));
-  const double *Bk = h + (K & 1) * B_SIZE;
+  // both banks and |b|^2 (G_BB follows the banks) into pinned memory
+  static_assert(G_BB < 64, "scalar slots beyond the pinned copy");
+  LMH_HIP(hipMemcpyAsync(snap, S, sizeof(double) * (G_BB + 1), hipMemcpyDeviceToHost, ctx->stream));
+  LMH_HIP(hipStreamSynchronize(ctx->stream));
+  const double *Bk = snap + (K & 1) * B_SIZE, gbb = snap[G_BB];
   const bool conv = Bk[B_DONE] != 0.0;
   if (iters) *iters = conv ? (int64_t)Bk[B_ITS] : maxits;
   if (rel_res) *rel_res = gbb > 0 ? std::sqrt(Bk[B_RR] / gbb) : 0.0;
   if (reason) *reason = conv ? (int)Bk[B_REASON] : LMH_DIVERGED_ITS;
   return LMH_OK;
+}
+
+}  // namespace lmh
+
+extern "C" {
+
+int lmh_cg_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int64_t maxits, int64_t *iters,
+                 double *rel_res, int *reason) {
+  LMH_CHECK(A && b_d && x_d, LMH_ERR_INVALID, "null argument");
+  LMH_CHECK(rtol >= 0 && maxits >= 0, LMH_ERR_INVALID, "bad tolerance or iteration limit");
+  LMH_CHECK(A->rows == A->cols, LMH_ERR_INVALID, "CG needs a square matrix");
+  LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
+  LMH_TRY(ensure_work(A, false));
+  LMH_TRY(ensure_diag(A));
+  const int64_t n = A->rows;
+  double *r = A->w64;
+  return cg_impl<false>(A, b_d, x_d, rtol, maxits, r, r + n, r + 2 * n, r + 3 * n, A->diag, iters, rel_res, reason);
+}
+
+int lmh_cg_solve_noalloc(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int64_t maxits,
+                         int64_t *iters, double *rel_res, int *reason) {
+  LMH_CHECK(A && b_d && x_d, LMH_ERR_INVALID, "null argument");
+  LMH_CHECK(rtol >= 0 && maxits >= 0, LMH_ERR_INVALID, "bad tolerance or iteration limit");
+  LMH_CHECK(A->rows == A->cols, LMH_ERR_INVALID, "CG needs a square matrix");
+  LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
+  LMH_CHECK(A->rows <= LMH_CG_NOALLOC_MAX_ROWS, LMH_ERR_INVALID,
+            "lmh_cg_solve_noalloc: %lld rows exceed its scratch limit of %d rows (use lmh_cg_solve)",
+            (long long)A->rows, (int)LMH_CG_NOALLOC_MAX_ROWS);
+  lmh_context *ctx = A->ctx;
+  LMH_CHECK(ctx->cg_scratch, LMH_ERR_HIP, "CG scratch address not set up for device %d", ctx->device);
+  const int64_t n = A->rows;
+  const size_t cap = LMH_CG_NOALLOC_MAX_ROWS;
+  double *w = static_cast<double *>(ctx->cg_scratch);
+  int32_t *diag = reinterpret_cast<int32_t *>(w + 4 * cap);
+  hipLaunchKernelGGL(k_diag_index, dim3(grid_for(n, 256, 8192)), dim3(256), 0, ctx->stream, n, A->rp, A->ci, diag);
+  LMH_TRY(check_launch("k_diag_index"));
+  return cg_impl<true>(A, b_d, x_d, rtol, maxits, w, w + cap, w + 2 * cap, w + 3 * cap, diag, iters, rel_res,
+                       reason);
 }
 
 }  // extern "C"
