@@ -214,6 +214,30 @@ double lmh_norm2(lmh_context *ctx, const double *x_d, int64_t n);
 /* y += a x (NumericVector::add(a, v), numeric_vector.h). */
 int lmh_axpy(lmh_context *ctx, int64_t n, double a, const double *x_d, double *y_d);
 
+/* Whole-vector updates of the device-resident NumericVector
+ * (include/numerics/numeric_vector.h:288 operator=(T), :503 add(T), :618
+ * scale, :517 add(a, v), :457 operator*=(v), :474 operator/=(v), :480
+ * reciprocal, :623 abs, :730 pointwise_mult, :738 pointwise_divide, :119
+ * operator=(v)), on n entries of z:
+ *   SET z = a; ADD_SCALAR z += a; SCALE z *= a; AXPY z += a x; MUL z *= x;
+ *   DIV z /= x; RECIPROCAL z = 1/z; ABS z = |z|; PMUL z = x y; PDIV z = x / y;
+ *   COPY z = x.  Operands that an operation does not read may be NULL. */
+enum {
+  LMH_VEC_SET = 0, LMH_VEC_ADD_SCALAR, LMH_VEC_SCALE, LMH_VEC_AXPY, LMH_VEC_MUL, LMH_VEC_DIV,
+  LMH_VEC_RECIPROCAL, LMH_VEC_ABS, LMH_VEC_PMUL, LMH_VEC_PDIV, LMH_VEC_COPY
+};
+int lmh_vector_op(lmh_context *ctx, int op, int64_t n, double a, const double *x_d, const double *y_d, double *z_d);
+/* Reductions of the same vector (numeric_vector.h:312 sum, :318 l1_norm, :324
+ * l2_norm, :330 linfty_norm, :301 min, :307 max, :631 dot, and
+ * l1_norm_diff / l2_norm_diff) over n entries of x (and y), deterministic
+ * (fixed slices per block, fixed fold order); the value is written to *out_h.
+ * n = 0 gives 0 (MIN: +inf, MAX: -inf). */
+enum {
+  LMH_VEC_SUM = 0, LMH_VEC_L1, LMH_VEC_L2, LMH_VEC_LINF, LMH_VEC_MIN, LMH_VEC_MAX, LMH_VEC_DOT,
+  LMH_VEC_L1_DIFF, LMH_VEC_L2_DIFF
+};
+int lmh_vector_reduce(lmh_context *ctx, int op, int64_t n, const double *x_d, const double *y_d, double *out_h);
+
 /* Jacobi-preconditioned CG, FP64 (EigenSparseLinearSolver::solve with CG +
  * JACOBI_PRECOND -> Eigen::internal::conjugate_gradient,
  * eigen_sparse_linear_solver.C:82-170, ConjugateGradient.h:28-90).  x_d holds

@@ -29,6 +29,10 @@ ALL_FACES = 0x3F
 CONVERGED_RTOL, DIVERGED_ITS = 2, -3
 DIVERGED_BREAKDOWN, DIVERGED_NANORINF = -5, -9  # LMH_DIVERGED_* (include/lmhip.h)
 PRECOND_IDENTITY, PRECOND_JACOBI = 0, 1  # libMesh PreconditionerType values
+# lmh_vector_op operations and lmh_vector_reduce reductions (include/lmhip.h)
+(VEC_SET, VEC_ADD_SCALAR, VEC_SCALE, VEC_AXPY, VEC_MUL, VEC_DIV, VEC_RECIPROCAL, VEC_ABS, VEC_PMUL, VEC_PDIV,
+ VEC_COPY) = range(11)
+(VEC_SUM, VEC_L1, VEC_L2, VEC_LINF, VEC_MIN, VEC_MAX, VEC_DOT, VEC_L1_DIFF, VEC_L2_DIFF) = range(9)
 CG_NOALLOC_MAX_ROWS = 4194304  # LMH_CG_NOALLOC_MAX_ROWS (include/lmhip.h)
 
 _P = C.c_void_p
@@ -69,6 +73,8 @@ HIP_API = {
     "lmh_enforce_constraints": (_I, [_P, _P, _I]),
     "lmh_assemble_residual_poisson": (_I, [_P, _P, _P, _I, _P, _I, _P, _P]),
     "lmh_axpy": (_I, [_P, _I64, _D, _P, _P]),
+    "lmh_vector_op": (_I, [_P, _I, _I64, _D, _P, _P, _P]),
+    "lmh_vector_reduce": (_I, [_P, _I, _I64, _P, _P, C.POINTER(_D)]),
     "lmh_bicgstab_solve": (_I, [_P, _P, _P, _D, _I64, C.POINTER(_I64), C.POINTER(_D), C.POINTER(_I)]),
     "lmh_matrix_create": (_I, [_P, _P, C.POINTER(_P)]),
     "lmh_matrix_create_csr": (_I, [_P, _I64, _P, _P, C.POINTER(_P)]),

@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "libmesh/libmesh.h"
 #include "libmesh/system.h"
 
 using namespace libMesh;
@@ -39,15 +40,20 @@ void declare(EquationSystems &es, const char *aux_var_b) {
 void fill(EquationSystems &es, const Mesh &mesh) {
   System &p = es.get_system<System>("Poisson"), &aux = es.get_system<System>("aux");
   const auto &dp = p.get_dof_map(), &da = aux.get_dof_map();
-  const unsigned npe = mesh.nodes_per_elem();
-  for (dof_id_type e = 0; e < mesh.n_elem(); ++e)
+  std::vector<dof_id_type> du, dab;
+  for (dof_id_type e = 0; e < mesh.n_elem(); ++e) {
+    const Elem &elem = mesh.elem_ref(e);
+    dp.dof_indices(&elem, du);
+    da.dof_indices(&elem, dab);
+    const unsigned npe = elem.n_nodes();
     for (unsigned k = 0; k < npe; ++k) {
-      const Point x = mesh.point(mesh.node_id(e, k));
-      p.solution->set(dp.elem_dofs()[(size_t)e * dp.dofs_per_elem() + k], f_u(x));
-      p.get_vector("u_old").set(dp.elem_dofs()[(size_t)e * dp.dofs_per_elem() + k], 2. * f_u(x) + 1.);
-      aux.solution->set(da.elem_dofs()[(size_t)e * da.dofs_per_elem() + k], f_a(x));
-      aux.solution->set(da.elem_dofs()[(size_t)e * da.dofs_per_elem() + npe + k], f_b(x));
+      const Point &x = elem.point(k);
+      p.solution->set(du[k], f_u(x));
+      p.get_vector("u_old").set(du[k], 2. * f_u(x) + 1.);
+      aux.solution->set(dab[k], f_a(x));
+      aux.solution->set(dab[npe + k], f_b(x));
     }
+  }
   p.solution->close();
   p.get_vector("u_old").close();
   aux.solution->close();
@@ -79,8 +85,9 @@ int main(int argc, char **argv) {
   }
   const unsigned n = std::atoi(argv[1]);
   const std::string prefix = argv[2];
+  LibMeshInit init(argc, argv);
   try {
-    Mesh mesh;
+    Mesh mesh(init.comm());
     MeshTools::Generation::build_cube(mesh, n, n, n, 0., 1., 0., 1., 0., 1., HEX27);
     MeshTools::Modification::perturb_interior(mesh, 0x5EED0001, 0.2);
     EquationSystems es(mesh);

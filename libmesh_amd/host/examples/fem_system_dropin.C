@@ -23,6 +23,8 @@
 #include "libmesh/hip_assembler.h"
 #include "libmesh/hip_csr_matrix.h"
 #include "libmesh/hip_linear_solver.h"
+#include "libmesh/libmesh.h"
+#include "box_qoi.h"
 #include "periodic_x.h"
 
 using namespace libMesh;
@@ -46,8 +48,9 @@ int main(int argc, char **argv) {
     bicgstab |= !std::strcmp(argv[a], "bicgstab");
     if (!std::strncmp(argv[a], "rtol=", 5)) rtol = std::atof(argv[a] + 5);
   }
+  LibMeshInit init(argc, argv);
   try {
-    Mesh mesh;
+    Mesh mesh(init.comm());
     MeshTools::Generation::build_cube(mesh, n, n, n, 0., 1., 0., 1., 0., 1., type);
     if (perturb) MeshTools::Modification::perturb_interior(mesh, 0x5EED0001, 0.2);
     EquationSystems es(mesh);
@@ -58,13 +61,14 @@ int main(int argc, char **argv) {
     if (periodic) {
       sys.get_dof_map().add_dirichlet_boundary(DirichletBoundary({0, 1, 3, 5}, {0}));
       sys.get_dof_map().set_error_on_constraint_loop(true);
-      sys.get_dof_map().set_asymmetric_constraint_rows(bicgstab);
       sys.attach_constraint_object(periodic_rows);
     } else {
       sys.get_dof_map().add_dirichlet_boundary(DirichletBoundary({0, 1, 2, 3, 4, 5}, {0}));
     }
-    sys.add_matrix("System Matrix", std::make_unique<HipCsrMatrix<Number>>());
-    sys.linear_solver = std::make_unique<HipLinearSolver<Number>>();
+    auto matrix = std::make_unique<HipCsrMatrix<Number>>(init.comm());
+    matrix->set_asymmetric_constraint_rows(bicgstab);
+    sys.add_matrix("System Matrix", std::move(matrix));
+    sys.linear_solver = std::make_unique<HipLinearSolver<Number>>(init.comm());
     if (bicgstab) sys.linear_solver->set_solver_type(BICGSTAB);
     const FEType fe_type(order, LAGRANGE);
     const QGauss qrule(3, fe_type.default_quadrature_order());
@@ -85,12 +89,21 @@ int main(int argc, char **argv) {
     std::ofstream f(prefix + "_u.bin", std::ios::binary);
     f.write((const char *)u.data(), sizeof(double) * u.size());
     if (adjoint) {
-      // the system matrix holds the Jacobian of the last Newton step
-      HipVector<Number> dq(sys.get_dof_map().n_dofs()), z(sys.get_dof_map().n_dofs());
-      physics.qoi_box_derivative(0.25, 0.75, dq);
-      const auto r = sys.adjoint_solve(z, dq);
-      const std::vector<Real> g = physics.kappa_sensitivities(*sys.solution, z);
+      // ImplicitSystem::adjoint_qoi_parameter_sensitivity through the
+      // FEMSystem's Jacobian, J = integral of u over [0.25,0.75]^3
+      BoxQoIDerivative qoi_derivative(sys, physics);
+      sys.init_qois(1);
+      sys.attach_QOI_derivative_object(qoi_derivative);
+      if (physics.coefficient().empty()) physics.set_coefficient(std::vector<Real>(mesh.n_elem(), 1.0));
+      const auto r = sys.adjoint_solve();
+      sys.set_adjoint_already_solved(true);
       std::printf("adjoint iterations %u residual %.6e\n", r.first, r.second);
+      ParameterVector params;
+      for (auto &k : physics.coefficient()) params.push_back(std::make_unique<ParameterPointer<Number>>(&k));
+      SensitivityData sens;
+      sys.adjoint_qoi_parameter_sensitivity(QoISet(), params, sens);
+      std::vector<double> g(params.size());
+      for (unsigned int j = 0; j < params.size(); ++j) g[j] = sens[0][j];
       std::ofstream fg(prefix + "_grad.bin", std::ios::binary);
       fg.write((const char *)g.data(), sizeof(double) * g.size());
     }

@@ -24,8 +24,9 @@ class PeriodicX : public System::Constraint {
     DofMap &dm = _sys.get_dof_map();
     std::vector<dof_id_type> node_dof(_mesh.n_nodes()), di;
     for (dof_id_type e = 0; e < _mesh.n_elem(); ++e) {
-      dm.dof_indices(e, di);
-      for (unsigned a = 0; a < _mesh.nodes_per_elem(); ++a) node_dof[_mesh.node_id(e, a)] = di[a];
+      const Elem &elem = _mesh.elem_ref(e);
+      dm.dof_indices(&elem, di);
+      for (unsigned a = 0; a < elem.n_nodes(); ++a) node_dof[elem.node_id(a)] = di[a];
     }
     auto key = [&](const Point &p) {
       return std::make_pair((long)std::lround(p.y * 2 * _cells), (long)std::lround(p.z * 2 * _cells));

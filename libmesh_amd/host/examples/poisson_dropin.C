@@ -24,7 +24,9 @@
 #include "libmesh/hip_assembler.h"
 #include "libmesh/hip_csr_matrix.h"
 #include "libmesh/hip_linear_solver.h"
+#include "libmesh/libmesh.h"
 #include "libmesh/system.h"
+#include "box_qoi.h"
 #include "periodic_x.h"
 
 using namespace libMesh;
@@ -55,8 +57,9 @@ int main(int argc, char **argv) {
     bicgstab |= !std::strcmp(argv[a], "bicgstab");
     identity |= !std::strcmp(argv[a], "identity");
   }
+  LibMeshInit init(argc, argv);
   try {
-    Mesh mesh;
+    Mesh mesh(init.comm());
     MeshTools::Generation::build_cube(mesh, n, n, n, 0., 1., 0., 1., 0., 1., type);
     if (perturb) MeshTools::Modification::perturb_interior(mesh, 0x5EED0001, 0.2);
     if (distort) {
@@ -75,13 +78,14 @@ int main(int argc, char **argv) {
     if (periodic) {
       sys.get_dof_map().add_dirichlet_boundary(DirichletBoundary({0, 1, 3, 5}, {0}));
       sys.get_dof_map().set_error_on_constraint_loop(true);
-      sys.get_dof_map().set_asymmetric_constraint_rows(bicgstab);
       sys.attach_constraint_object(periodic_rows);
     } else {
       sys.get_dof_map().add_dirichlet_boundary(DirichletBoundary({0, 1, 2, 3, 4, 5}, {0}));
     }
-    sys.add_matrix("System Matrix", std::make_unique<HipCsrMatrix<Number>>());
-    auto solver = std::make_unique<HipLinearSolver<Number>>();
+    auto matrix = std::make_unique<HipCsrMatrix<Number>>(init.comm());
+    matrix->set_asymmetric_constraint_rows(bicgstab);
+    sys.add_matrix("System Matrix", std::move(matrix));
+    auto solver = std::make_unique<HipLinearSolver<Number>>(init.comm());
     solver->set_mixed_precision(mixed);
     if (bicgstab) solver->set_solver_type(BICGSTAB);
     if (identity) solver->set_preconditioner_type(IDENTITY_PRECOND);
@@ -106,15 +110,24 @@ int main(int argc, char **argv) {
     write_bin(prefix + "_u.bin", u);
 
     if (adjoint) {
-      // adjoint_qoi_parameter_sensitivity for per-element kappa, J = integral of u over [0.25,0.75]^3
-      HipVector<Number> dq(sys.get_dof_map().n_dofs()), z(sys.get_dof_map().n_dofs());
-      assembler.qoi_box_derivative(0.25, 0.75, dq);
-      const auto r = sys.adjoint_solve(z, dq);
-      const std::vector<Real> g = assembler.kappa_sensitivities(*sys.solution, z);
+      // ImplicitSystem::adjoint_solve and adjoint_qoi_parameter_sensitivity
+      // for every per-element kappa, J = integral of u over [0.25,0.75]^3
+      BoxQoIDerivative qoi_derivative(sys, assembler);
+      sys.init_qois(1);
+      sys.attach_QOI_derivative_object(qoi_derivative);
+      if (assembler.coefficient().empty()) assembler.set_coefficient(std::vector<Real>(mesh.n_elem(), 1.0));
+      const auto r = sys.adjoint_solve();
+      sys.set_adjoint_already_solved(true);
       std::printf("adjoint iterations %u residual %.6e\n", r.first, r.second);
+      ParameterVector params;
+      for (auto &k : assembler.coefficient()) params.push_back(std::make_unique<ParameterPointer<Number>>(&k));
+      SensitivityData sens;
+      sys.adjoint_qoi_parameter_sensitivity(QoISet(), params, sens);
+      std::vector<double> g(params.size());
+      for (unsigned int j = 0; j < params.size(); ++j) g[j] = sens[0][j];
       write_bin(prefix + "_grad.bin", g);
       std::vector<double> zh;
-      z.localize(zh);
+      sys.get_adjoint_solution(0).localize(zh);
       write_bin(prefix + "_z.bin", zh);
     }
   } catch (const DegenerateMap &e) {

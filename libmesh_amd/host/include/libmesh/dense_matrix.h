@@ -47,4 +47,24 @@ class DenseVector {
   std::vector<T> _val;
 };
 
+// DenseSubVector (include/numerics/dense_subvector.h): a window of
+// `n` entries of a parent DenseVector starting at entry `ioff`
+template <typename T>
+class DenseSubVector {
+ public:
+  DenseSubVector(DenseVector<T> &parent, unsigned ioff = 0, unsigned n = 0) : _parent(parent), _i(ioff), _n(n) {}
+  void reposition(unsigned ioff, unsigned n) {
+    _i = ioff;
+    _n = n;
+  }
+  T &operator()(unsigned i) { return _parent(_i + i); }
+  T operator()(unsigned i) const { return _parent(_i + i); }
+  unsigned size() const { return _n; }
+  unsigned i_off() const { return _i; }
+
+ private:
+  DenseVector<T> &_parent;
+  unsigned _i, _n;
+};
+
 }  // namespace libMesh

@@ -25,8 +25,12 @@ class System;
 template <typename T>
 class NumericVector;
 
-// DofConstraintRow (include/base/dof_map.h): constraining DoF -> coefficient
+// DofConstraintRow, DofConstraints, DofConstraintValueMap
+// (include/base/dof_map.h:97-125): constraining DoF -> coefficient; the rows
+// by constrained DoF; the nonzero right-hand sides by constrained DoF
 typedef std::map<dof_id_type, Real> DofConstraintRow;
+class DofConstraints : public std::map<dof_id_type, DofConstraintRow> {};
+class DofConstraintValueMap : public std::map<dof_id_type, Number> {};
 
 // DirichletBoundary with a zero function (include/base/dirichlet_boundaries.h)
 struct DirichletBoundary {
@@ -41,55 +45,61 @@ class DofMap {
   explicit DofMap(unsigned sys_number = 0) : _sys(sys_number) {}
 
   unsigned add_variable(const std::string &name, const FEType &type);
-  unsigned n_variables() const { return (unsigned)_names.size(); }
-  const std::string &variable_name(unsigned v) const { return _names.at(v); }
-  const FEType &variable_type(unsigned v) const { return _types.at(v); }
-  void add_dirichlet_boundary(const DirichletBoundary &d) { _dirichlet.push_back(d); }
+  unsigned int n_variables() const { return (unsigned)_names.size(); }
+  const std::string &variable_name(const unsigned int i) const { return _names.at(i); }
+  const FEType &variable_type(const unsigned int i) const { return _types.at(i); }
+  void add_dirichlet_boundary(const DirichletBoundary &dirichlet_boundary) { _dirichlet.push_back(dirichlet_boundary); }
 
   // DofMap::distribute_dofs (dof_map.C:949) -> distribute_local_dofs_var_major
   // (dof_map.C:1417-1530), then create_dof_constraints for the Dirichlet
-  // boundaries (dof_map_constraints.C:1736, 592-740).
-  void distribute_dofs(const MeshBase &mesh);
+  // boundaries (dof_map_constraints.C:1736, 592-740): an empty row with a
+  // zero right-hand side per boundary DoF.
+  void distribute_dofs(MeshBase &mesh);
   dof_id_type n_dofs() const { return _n_dofs; }
-  unsigned dofs_per_elem() const { return _dpe; }
-  // DofMap::dof_indices (dof_map.C:2201-2316): variable-major local order
-  void dof_indices(dof_id_type e, std::vector<dof_id_type> &di) const;
-  bool is_constrained_dof(dof_id_type d) const { return _cons[d] != 0; }
-  dof_id_type n_constrained_dofs() const;
+  // DofMap::dof_indices (dof_map.C:2201-2316): all variables, variable-major
+  // local order; or one variable's DoFs
+  void dof_indices(const Elem *const elem, std::vector<dof_id_type> &di) const;
+  void dof_indices(const Elem *const elem, std::vector<dof_id_type> &di, const unsigned int vn) const;
+  bool is_constrained_dof(const dof_id_type dof) const { return _dof_constraints.count(dof) != 0; }
+  dof_id_type n_constrained_dofs() const { return (dof_id_type)_dof_constraints.size(); }
 
-  // DofMap::add_constraint_row (dof_map_constraints.C:2180-2208): x_dof =
-  // constraint_rhs + sum of row coefficients times their DoFs.  Rows are
-  // processed (process_constraints, :4341-4497) on the device when the system
-  // matrix is initialised, so rows are added after distribute_dofs and before
-  // the matrices are built: from a System::Constraint object, as libMesh
-  // applications do.
-  void add_constraint_row(dof_id_type dof_number, const DofConstraintRow &constraint_row, Number constraint_rhs,
-                          bool forbid_constraint_overwrite);
-  void add_constraint_row(dof_id_type dof_number, const DofConstraintRow &constraint_row,
-                          bool forbid_constraint_overwrite = true) {
-    add_constraint_row(dof_number, constraint_row, 0, forbid_constraint_overwrite);
+  // DofMap::add_constraint_row (dof_map_constraints.C:2180-2208, declared
+  // dof_map.h:1130, 1156): x_dof = constraint_rhs + sum of row coefficients
+  // times their DoFs.  Rows are added after distribute_dofs, from a
+  // System::Constraint object as libMesh applications do, and processed by
+  // process_constraints.
+  void add_constraint_row(const dof_id_type dof_number, const DofConstraintRow &constraint_row,
+                          const Number constraint_rhs, const bool forbid_constraint_overwrite);
+  void add_constraint_row(const dof_id_type dof_number, const DofConstraintRow &constraint_row,
+                          const bool forbid_constraint_overwrite = true) {
+    add_constraint_row(dof_number, constraint_row, 0., forbid_constraint_overwrite);
   }
+  // dof_map.h:1164-1175, 1263
+  DofConstraints::const_iterator constraint_rows_begin() const { return _dof_constraints.begin(); }
+  DofConstraints::const_iterator constraint_rows_end() const { return _dof_constraints.end(); }
+  const DofConstraints &get_dof_constraints() const { return _dof_constraints; }
+  DofConstraintValueMap &get_primal_constraint_values() { return _primal_constraint_values; }
   // DofMap::set_error_on_constraint_loop (check_for_constraint_loops, :4509-4597)
   void set_error_on_constraint_loop(bool error_on_constraint_loop) { _error_on_loop = error_on_constraint_loop; }
-  bool get_error_on_constraint_loop() const { return _error_on_loop; }
-  // The asymmetric_constraint_rows argument that the device assembly passes to
-  // heterogeneously_constrain_element_matrix_and_vector (:2518-2616); default
-  // true as there.  Symmetric rows keep the system SPD for CG; then call
-  // enforce_constraints_exactly after the solve.
-  void set_asymmetric_constraint_rows(bool asymmetric) { _asymmetric = asymmetric; }
-  bool asymmetric_constraint_rows() const { return _asymmetric; }
+  // DofMap::process_constraints (dof_map_constraints.C:4341-4497): rows are
+  // substituted until they refer to unconstrained DoFs only (on the device);
+  // with set_error_on_constraint_loop(true) a constraint loop is an error.
+  void process_constraints(MeshBase &mesh);
   // DofMap::enforce_constraints_exactly (:3088-3170) on the system's solution
   // (or v), on the device, with the processed rows of the system matrix.
   void enforce_constraints_exactly(const System &system, NumericVector<Number> *v = nullptr,
                                    bool homogeneous = false) const;
-  // rows added with add_constraint_row, in order (DoF, row, value)
-  const std::vector<std::tuple<dof_id_type, DofConstraintRow, Number>> &added_constraint_rows() const {
-    return _rows;
-  }
+  // DofMap::enforce_adjoint_constraints_exactly (dof_map.h:1557): the
+  // homogeneous form of the same rows on an adjoint solution
+  void enforce_adjoint_constraints_exactly(NumericVector<Number> &v, unsigned int q) const;
 
+  // This library's mesh generation, files and C host API read the numbering
+  // as flat arrays; the drop-in matrix reads it through the accessors above.
+  unsigned dofs_per_elem() const { return _dpe; }
   const std::vector<int32_t> &elem_dofs() const { return _edofs; }
-  const std::vector<uint8_t> &constrained() const { return _cons; }
+  std::vector<uint8_t> constrained() const;
   const MeshBase *mesh() const { return _mesh; }
+  bool has_general_constraints() const;
 
  private:
   unsigned _sys;
@@ -97,16 +107,15 @@ class DofMap {
   std::vector<FEType> _types;
   std::vector<DirichletBoundary> _dirichlet;
   const MeshBase *_mesh = nullptr;
+  const System *_system = nullptr;
   dof_id_type _n_dofs = 0;
   unsigned _dpe = 0;
   std::vector<int32_t> _edofs;
-  std::vector<uint8_t> _cons;
-  std::vector<uint8_t> _dirichlet_cons;  // the Dirichlet set alone
-  std::vector<std::tuple<dof_id_type, DofConstraintRow, Number>> _rows;
-  bool _error_on_loop = false, _asymmetric = true;
+  DofConstraints _dof_constraints;
+  DofConstraintValueMap _primal_constraint_values;
+  bool _error_on_loop = false;
 
- public:
-  const std::vector<uint8_t> &dirichlet_constrained() const { return _dirichlet_cons; }
+  friend class System;
 };
 
 }  // namespace libMesh

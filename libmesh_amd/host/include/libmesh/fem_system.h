@@ -87,10 +87,13 @@ class FEMSystem : public ImplicitSystem {
   // C^T (K u - F) at the enforced solution and the Jacobian C^T K C in the
   // DofMap's row form (CG for symmetric rows, BICGSTAB for asymmetric).
   void assembly(bool get_residual, bool get_jacobian, bool apply_heterogeneous_constraints = false,
-                bool apply_no_constraints = false);
+                bool apply_no_constraints = false) override;
   // DifferentiableSystem::solve -> time_solver->solve()
   void solve() override;
   std::unique_ptr<SteadySolver> time_solver;
+
+ protected:
+  HipAssembler *device_assembler() const override { return _physics; }
 
  private:
   HipAssembler *_physics = nullptr;

@@ -24,8 +24,16 @@ class HipAssembler : public System::Assembly {
                const QGauss &qrule, Forcing f = UNIT);
   ~HipAssembler() override;
 
-  // per-element coefficient kappa_e (empty: kappa = 1)
+  // per-element coefficient kappa_e (empty: kappa = 1).  The assembler keeps
+  // the values on the host and uploads them at every assembly, so parameters
+  // that point into coefficient() (ParameterPointer) act on the next one.
   void set_coefficient(const std::vector<Real> &kappa);
+  std::vector<Number> &coefficient() { return _kappa_h; }
+  // the element whose coefficient *p is, or (dof_id_type)-1
+  dof_id_type coefficient_index(const Number *p) const {
+    if (_kappa_h.empty() || p < _kappa_h.data() || p >= _kappa_h.data() + _kappa_h.size()) return (dof_id_type)-1;
+    return (dof_id_type)(p - _kappa_h.data());
+  }
   // FP32 element contraction + FP32 operator for the mixed-precision solver
   void set_fp32_operator(bool on) { _fp32 = on; }
   void set_scatter(int mode) { _scatter = mode; }
@@ -53,8 +61,9 @@ class HipAssembler : public System::Assembly {
   Forcing _f;
   bool _fp32 = false;
   int _scatter = 1;
+  void upload_coefficient();
+  std::vector<Number> _kappa_h;
   double *_kappa_d = nullptr;
-  size_t _kappa_n = 0;
 };
 
 }  // namespace libMesh

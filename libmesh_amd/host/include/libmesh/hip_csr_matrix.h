@@ -1,10 +1,15 @@
-// hip_csr_matrix.h -- HipCsrMatrix<T>: a SparseMatrix backend whose CSR
-// pattern is built on the GPU from the attached DofMap (the rule of
-// SparsityPattern::Build, src/base/sparsity_pattern.C:96-365) and whose values
-// live in device memory.  Host-side set/add/add_matrix calls are staged and
-// applied at close() in call order; element matrices go through the batched
-// deterministic device scatter.  Device assembly (HipAssembler) writes the
-// values directly.
+// hip_csr_matrix.h -- HipCsrMatrix<T>: a SparseMatrix backend whose values
+// live in device memory.  Its CSR pattern comes from one of:
+//  * the GPU, from the attached DofMap and the mesh given to attach_mesh (the
+//    rule of SparsityPattern::Build, src/base/sparsity_pattern.C:96-365),
+//    read through libMesh's MeshBase / Elem / DofMap accessors; this is the
+//    pattern device assembly (HipAssembler) writes into;
+//  * libMesh's own graph: without a mesh the matrix needs the full sparsity
+//    pattern and takes the one DofMap hands to update_sparsity_pattern;
+//  * the entries inserted before the first close() (init(m, n, ...) mode).
+// Host-side set/add/add_matrix calls are staged and applied at close() in
+// call order; element matrices go through the batched deterministic device
+// scatter.
 #pragma once
 
 #include <map>
@@ -18,13 +23,30 @@ namespace libMesh {
 template <typename T>
 class HipCsrMatrix : public SparseMatrix<T> {
  public:
-  explicit HipCsrMatrix(int device = 0);
+  explicit HipCsrMatrix(const Parallel::Communicator &comm, int device = 0);
   ~HipCsrMatrix() override;
   HipCsrMatrix(const HipCsrMatrix &) = delete;
   HipCsrMatrix &operator=(const HipCsrMatrix &) = delete;
 
   SolverPackage solver_package() override { return HIP_SOLVERS; }
-  bool need_full_sparsity_pattern() const override { return true; }
+  // Without a mesh the pattern must come from DofMap's graph.
+  bool need_full_sparsity_pattern() const override { return _mesh_src == nullptr; }
+  // SparseMatrix::update_sparsity_pattern (sparse_matrix.h:175): the rows of
+  // libMesh's graph, used by init() when no mesh is attached
+  void update_sparsity_pattern(const SparsityPattern::Graph &sparsity_pattern) override;
+  // The mesh whose elements the attached DofMap numbers: init() then builds
+  // the pattern, the element scatter maps and the processed constraint rows
+  // on the GPU.  Not part of libMesh's SparseMatrix: the drop-in System and
+  // HipAssembler call it.
+  void attach_mesh(const MeshBase &mesh) { _mesh_src = &mesh; }
+  // The asymmetric_constraint_rows argument that libMesh assembly passes to
+  // heterogeneously_constrain_element_matrix_and_vector
+  // (dof_map_constraints.C:2518-2616), here fixed per matrix before init():
+  // true (default) keeps the constraint equations in the constrained rows;
+  // false keeps only their diagonals, so the system stays SPD for CG (call
+  // DofMap::enforce_constraints_exactly after the solve).
+  void set_asymmetric_constraint_rows(bool asymmetric) { _asymmetric = asymmetric; }
+  bool asymmetric_constraint_rows() const { return _asymmetric; }
 
   void init(numeric_index_type m, numeric_index_type n, numeric_index_type m_l, numeric_index_type n_l,
             numeric_index_type nnz = 30, numeric_index_type noz = 10, numeric_index_type blocksize = 1) override;
@@ -54,10 +76,15 @@ class HipCsrMatrix : public SparseMatrix<T> {
   void get_diagonal(NumericVector<T> &dest) const override;
   void get_transpose(SparseMatrix<T> &dest) const override;
   void get_row(numeric_index_type i, std::vector<numeric_index_type> &indices, std::vector<T> &values) const override;
-  void vector_mult(NumericVector<T> &dest, const NumericVector<T> &arg) const override;
+  void zero_rows(std::vector<numeric_index_type> &rows, T diag_value = 0.0) override;
+  void scale(const T scale) override;
   std::size_t n_nonzeros() const override { return (std::size_t)_nnz; }
+  // dest = A arg on the device (dest sized to the rows); dest = A^T arg on the host mirror
+  void device_vector_mult(HipVector<T> &dest, const HipVector<T> &arg) const;
+  void vector_mult_transpose(NumericVector<T> &dest, const NumericVector<T> &arg) const;
   // Matlab text (17 significant digits), Matlab text or PETSc binary in
   void print_matlab(const std::string &name = "") const override;
+  void print_petsc_binary(const std::string &filename) const override;
   void read(const std::string &filename) override;
   void read_matlab(const std::string &filename) override;
   void read_petsc_binary(const std::string &filename) override;
@@ -68,6 +95,7 @@ class HipCsrMatrix : public SparseMatrix<T> {
   lmh_dofmap *device_dofmap() const { return _dm; }
   lmh_context *context() const { return _ctx; }
   const DofMap *dof_map() const { return this->_dof_map; }
+  const MeshBase *attached_mesh() const { return _mesh_src; }
   // values were written on the device (host mirror stale)
   void mark_device_modified() {
     _vals_host_valid = false;
@@ -83,6 +111,9 @@ class HipCsrMatrix : public SparseMatrix<T> {
 
   int _device;
   lmh_context *_ctx;
+  const MeshBase *_mesh_src = nullptr;
+  bool _asymmetric = true;
+  std::vector<int32_t> _graph_rp, _graph_ci;  // libMesh's graph (update_sparsity_pattern)
   lmh_mesh *_mesh = nullptr;
   lmh_dofmap *_dm = nullptr;
   lmh_matrix *_A = nullptr;

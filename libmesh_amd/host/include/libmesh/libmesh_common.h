@@ -4,6 +4,7 @@
 // match libMesh so code written against libMesh compiles unchanged.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <sstream>
 #include <stdexcept>
@@ -47,6 +48,8 @@ enum LinearConvergenceReason : int {
 enum SolverPackage : int { PETSC_SOLVERS = 0, TRILINOS_SOLVERS, LASPACK_SOLVERS, SLEPC_SOLVERS, EIGEN_SOLVERS,
                            NLOPT_SOLVERS, INVALID_SOLVER_PACKAGE, HIP_SOLVERS = 100 };
 enum ParallelType : int { AUTOMATIC = 0, SERIAL, PARALLEL, GHOSTED, INVALID_PARALLELIZATION };
+// include/enums/enum_subset_solve_mode.h
+enum SubsetSolveMode : int { SUBSET_ZERO = 0, SUBSET_COPY_RHS, SUBSET_DONT_TOUCH };
 
 // include/base/libmesh_exceptions.h:65, 116
 class LogicError : public std::logic_error {

@@ -43,5 +43,11 @@ Entries read(const std::string &filename);
 void write_matlab(std::ostream &os, std::size_t m, std::size_t n, const std::vector<int32_t> &row_ptr,
                   const std::vector<int32_t> &col_idx, const std::vector<double> &vals);
 
+// PETSc binary of a CSR matrix, the layout read_petsc_binary reads (every
+// stored entry, explicit zeros included; index width 4 or 8 bytes)
+void write_petsc_binary(const std::string &filename, int index_bytes, std::size_t m, std::size_t n,
+                        const std::vector<int32_t> &row_ptr, const std::vector<int32_t> &col_idx,
+                        const std::vector<double> &vals);
+
 }  // namespace MatrixIO
 }  // namespace libMesh

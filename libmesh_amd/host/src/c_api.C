@@ -13,8 +13,14 @@
 
 using namespace libMesh;
 
+// one serial communicator for the meshes of this API
+static const Parallel::Communicator &serial_comm() {
+  static const Parallel::Communicator comm;
+  return comm;
+}
+
 struct lmesh_problem {
-  ReplicatedMesh mesh;
+  ReplicatedMesh mesh{serial_comm()};
   DofMap dof_map;
 };
 

@@ -1,6 +1,7 @@
 // dof_map.C -- DofMap numbering and Dirichlet constraint set for one group of
 // Lagrange variables (dof_map.C:949-1530, dof_map_constraints.C:592-740).
 #include "libmesh/dof_map.h"
+#include "libmesh/device_dof_data.h"
 
 namespace libMesh {
 namespace {
@@ -21,7 +22,7 @@ unsigned DofMap::add_variable(const std::string &name, const FEType &type) {
   return (unsigned)_names.size() - 1;
 }
 
-void DofMap::distribute_dofs(const MeshBase &mesh) {
+void DofMap::distribute_dofs(MeshBase &mesh) {
   if (_types.empty()) libmesh_error_msg("DofMap::distribute_dofs: no variables");
   const ElemType t = mesh.elem_type();
   const Order elem_order = (t == HEX8 || t == TET4) ? FIRST : SECOND;
@@ -48,8 +49,9 @@ void DofMap::distribute_dofs(const MeshBase &mesh) {
     for (unsigned v = 0; v < nv; ++v)
       for (unsigned a = 0; a < npe; ++a) _edofs[(size_t)e * _dpe + v * npe + a] = (int32_t)(base[mesh.node_id(e, a)] + v);
   // Dirichlet: every listed variable's DoF at every node of a side whose
-  // boundary id is in the boundary's set
-  _cons.assign(_n_dofs, 0);
+  // boundary id is in the boundary's set gets an empty row (u = 0)
+  _dof_constraints.clear();
+  _primal_constraint_values.clear();
   const bool hex = (t == HEX8 || t == HEX27);
   const int side_n = t == HEX8 ? 4 : t == HEX27 ? 9 : t == TET4 ? 3 : 6;
   for (const auto &d : _dirichlet)
@@ -59,15 +61,13 @@ void DofMap::distribute_dofs(const MeshBase &mesh) {
         if (b < 0 || !d.b.count((boundary_id_type)b)) continue;
         for (int k = 0; k < side_n; ++k) {
           const int a = hex ? kHexSide[s][k] : kTetSide[s][k];
-          for (unsigned v : d.variables) _cons[_edofs[(size_t)e * _dpe + v * npe + a]] = 1;
+          for (unsigned v : d.variables) _dof_constraints[(dof_id_type)_edofs[(size_t)e * _dpe + v * npe + a]];
         }
       }
-  _dirichlet_cons = _cons;
-  _rows.clear();
 }
 
-void DofMap::add_constraint_row(dof_id_type dof_number, const DofConstraintRow &constraint_row, Number constraint_rhs,
-                                bool forbid_constraint_overwrite) {
+void DofMap::add_constraint_row(const dof_id_type dof_number, const DofConstraintRow &constraint_row,
+                                const Number constraint_rhs, const bool forbid_constraint_overwrite) {
   if (dof_number >= _n_dofs) libmesh_error_msg("add_constraint_row: DoF " << dof_number << " out of range");
   if (forbid_constraint_overwrite && is_constrained_dof(dof_number))
     libmesh_error_msg("ERROR: DOF " << dof_number << " was already constrained!");
@@ -76,18 +76,61 @@ void DofMap::add_constraint_row(dof_id_type dof_number, const DofConstraintRow &
                                                        << dof_number);
   for (const auto &it : constraint_row)
     if (it.first >= _n_dofs) libmesh_error_msg("add_constraint_row: DoF " << it.first << " out of range");
-  _rows.emplace_back(dof_number, constraint_row, constraint_rhs);
-  _cons[dof_number] = 1;
+  _dof_constraints[dof_number] = constraint_row;
+  if (constraint_rhs != Number(0))
+    _primal_constraint_values[dof_number] = constraint_rhs;
+  else
+    _primal_constraint_values.erase(dof_number);
 }
 
-void DofMap::dof_indices(dof_id_type e, std::vector<dof_id_type> &di) const {
+void DofMap::dof_indices(const Elem *const elem, std::vector<dof_id_type> &di) const {
+  const dof_id_type e = elem->id();
+  if (!_dpe || (size_t)(e + 1) * _dpe > _edofs.size()) libmesh_error_msg("dof_indices: element " << e << " unknown");
   di.assign(_edofs.begin() + (size_t)e * _dpe, _edofs.begin() + (size_t)(e + 1) * _dpe);
 }
 
-dof_id_type DofMap::n_constrained_dofs() const {
-  dof_id_type c = 0;
-  for (uint8_t x : _cons) c += x;
+void DofMap::dof_indices(const Elem *const elem, std::vector<dof_id_type> &di, const unsigned int vn) const {
+  const dof_id_type e = elem->id();
+  const unsigned nv = n_variables(), npe = nv ? _dpe / nv : 0;
+  if (vn >= nv) libmesh_error_msg("dof_indices: variable " << vn << " of " << nv);
+  if (!_dpe || (size_t)(e + 1) * _dpe > _edofs.size()) libmesh_error_msg("dof_indices: element " << e << " unknown");
+  const size_t o = (size_t)e * _dpe + (size_t)vn * npe;
+  di.assign(_edofs.begin() + o, _edofs.begin() + o + npe);
+}
+
+std::vector<uint8_t> DofMap::constrained() const {
+  std::vector<uint8_t> c(_n_dofs, 0);
+  for (const auto &r : _dof_constraints) c[r.first] = 1;
   return c;
+}
+
+bool DofMap::has_general_constraints() const {
+  if (!_primal_constraint_values.empty()) return true;
+  for (const auto &r : _dof_constraints)
+    if (!r.second.empty()) return true;
+  return false;
+}
+
+// Processing on the device (lmh_dofmap_set_constraints substitutes the rows)
+// and the processed rows read back, so that the constraint accessors return
+// what libMesh's DofMap holds after process_constraints.
+void DofMap::process_constraints(MeshBase &mesh) {
+  if (!has_general_constraints()) return;
+  DeviceDofData dev;
+  build_device_dofs(HipDevice::context(0), mesh, *this, _error_on_loop ? LMH_CONSTRAINT_ERROR_ON_LOOP : 0, dev);
+  int64_t nr = 0, ne = 0;
+  hip_check(lmh_dofmap_constraint_counts(dev.dofmap, &nr, &ne, nullptr), "DofMap::process_constraints");
+  std::vector<int32_t> dof(nr), ptr(nr + 1), col(ne);
+  std::vector<double> coef(ne), rhs(nr);
+  hip_check(lmh_dofmap_get_constraints(dev.dofmap, dof.data(), ptr.data(), col.data(), coef.data(), rhs.data()),
+            "DofMap::process_constraints");
+  _dof_constraints.clear();
+  _primal_constraint_values.clear();
+  for (int64_t k = 0; k < nr; ++k) {
+    DofConstraintRow &row = _dof_constraints[(dof_id_type)dof[k]];
+    for (int32_t t = ptr[k]; t < ptr[k + 1]; ++t) row[(dof_id_type)col[t]] = coef[t];
+    if (rhs[k] != 0.0) _primal_constraint_values[(dof_id_type)dof[k]] = rhs[k];
+  }
 }
 
 }  // namespace libMesh

@@ -21,4 +21,12 @@ void DofMap::enforce_constraints_exactly(const System &system, NumericVector<Num
   x->mark_device_modified();
 }
 
+// homogeneous rows on an adjoint solution; with empty rows only (homogeneous
+// Dirichlet) the adjoint solve already leaves those entries at zero
+void DofMap::enforce_adjoint_constraints_exactly(NumericVector<Number> &v, unsigned int) const {
+  if (!has_general_constraints()) return;
+  if (!_system) libmesh_error_msg("enforce_adjoint_constraints_exactly: DofMap of no system");
+  enforce_constraints_exactly(*_system, &v, true);
+}
+
 }  // namespace libMesh

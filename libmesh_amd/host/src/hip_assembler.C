@@ -12,16 +12,27 @@ HipAssembler::~HipAssembler() {
 }
 
 void HipAssembler::set_coefficient(const std::vector<Real> &kappa) {
+  if (!kappa.empty() && kappa.size() != _mesh.n_elem()) libmesh_error_msg("coefficient needs one value per element");
+  _kappa_h.assign(kappa.begin(), kappa.end());
+  upload_coefficient();
+}
+
+// the host coefficients to the device (2 MB at 262,144 elements)
+void HipAssembler::upload_coefficient() {
   lmh_context *ctx = HipDevice::context();
-  if (_kappa_d) lmh_free(ctx, _kappa_d);
-  _kappa_d = nullptr;
-  _kappa_n = kappa.size();
-  if (kappa.empty()) return;
-  if (kappa.size() != _mesh.n_elem()) libmesh_error_msg("coefficient needs one value per element");
-  void *p = nullptr;
-  hip_check(lmh_malloc(ctx, sizeof(double) * kappa.size(), &p), "HipAssembler::set_coefficient");
-  hip_check(lmh_memcpy_h2d(ctx, p, kappa.data(), sizeof(double) * kappa.size()), "HipAssembler::set_coefficient");
-  _kappa_d = (double *)p;
+  if (_kappa_h.empty()) {
+    if (_kappa_d) lmh_free(ctx, _kappa_d);
+    _kappa_d = nullptr;
+    return;
+  }
+  if (_kappa_h.size() != _mesh.n_elem()) libmesh_error_msg("coefficient needs one value per element");
+  if (!_kappa_d) {
+    void *p = nullptr;
+    hip_check(lmh_malloc(ctx, sizeof(double) * _kappa_h.size(), &p), "HipAssembler (coefficient)");
+    _kappa_d = (double *)p;
+  }
+  hip_check(lmh_memcpy_h2d(ctx, _kappa_d, _kappa_h.data(), sizeof(double) * _kappa_h.size()),
+            "HipAssembler (coefficient)");
 }
 
 void HipAssembler::assemble() {
@@ -31,6 +42,7 @@ void HipAssembler::assemble() {
   if (!A->device_matrix() || !A->device_dofmap()) libmesh_error_msg("HipAssembler: matrix not initialised from the DofMap");
   if (&_dof_map != A->dof_map()) libmesh_error_msg("HipAssembler: DofMap differs from the matrix's");
   if (_fe_type.order != _dof_map.variable_type(0).order) libmesh_error_msg("HipAssembler: FEType mismatch");
+  upload_coefficient();
   const int arg = _qrule.kernel_arg(_mesh.elem_type());
   hip_check(lmh_assemble_poisson(A->device_matrix(), A->device_mesh(), A->device_dofmap(), arg, _kappa_d, (int)_f,
                                  (double *)b->device_data(), LMH_FP64, _scatter),
@@ -46,6 +58,7 @@ void HipAssembler::assemble_residual(const NumericVector<Number> &u_in, NumericV
   auto *R = dynamic_cast<HipVector<Number> *>(&R_in);
   if (!A || !u || !R) libmesh_error_msg("assemble_residual: HipCsrMatrix / HipVector required");
   if (!A->device_matrix()) libmesh_error_msg("assemble_residual: matrix not initialised from the DofMap");
+  upload_coefficient();
   const int arg = _qrule.kernel_arg(_mesh.elem_type());
   hip_check(lmh_assemble_residual_poisson(A->device_matrix(), A->device_mesh(), A->device_dofmap(), arg, _kappa_d,
                                           (int)_f, (const double *)u->device_data(), (double *)R->device_data()),
@@ -56,6 +69,7 @@ void HipAssembler::assemble_residual(const NumericVector<Number> &u_in, NumericV
 void HipAssembler::assemble_jacobian() {
   auto *A = dynamic_cast<HipCsrMatrix<Number> *>(&_system.get_system_matrix());
   if (!A || !A->device_matrix()) libmesh_error_msg("assemble_jacobian: HipCsrMatrix initialised from the DofMap required");
+  upload_coefficient();
   const int arg = _qrule.kernel_arg(_mesh.elem_type());
   hip_check(lmh_assemble_poisson(A->device_matrix(), A->device_mesh(), A->device_dofmap(), arg, _kappa_d, (int)_f,
                                  nullptr, LMH_FP64, _scatter),
@@ -68,7 +82,8 @@ void HipAssembler::qoi_box_derivative(Real lo, Real hi, NumericVector<Number> &d
   auto *A = dynamic_cast<HipCsrMatrix<Number> *>(&_system.get_system_matrix());
   auto *dq = dynamic_cast<HipVector<Number> *>(&dq_in);
   if (!A || !dq) libmesh_error_msg("qoi_box_derivative: HipCsrMatrix / HipVector required");
-  if (dq->size() != _dof_map.n_dofs()) dq->init(_dof_map.n_dofs());
+  const numeric_index_type nd = _dof_map.n_dofs();
+  if (dq->size() != nd) dq->init(nd, nd);
   const int arg = _qrule.kernel_arg(_mesh.elem_type());
   hip_check(lmh_qoi_box_derivative(A->device_mesh(), A->device_dofmap(), arg, lo, hi, (double *)dq->device_data()),
             "qoi_box_derivative");
@@ -82,6 +97,7 @@ std::vector<Real> HipAssembler::kappa_sensitivities(const NumericVector<Number> 
   const auto *z = dynamic_cast<const HipVector<Number> *>(&z_in);
   if (!A || !u || !z) libmesh_error_msg("kappa_sensitivities: HipCsrMatrix / HipVector required");
   lmh_context *ctx = HipDevice::context();
+  upload_coefficient();
   void *g = nullptr;
   hip_check(lmh_malloc(ctx, sizeof(double) * _mesh.n_elem(), &g), "kappa_sensitivities");
   const int arg = _qrule.kernel_arg(_mesh.elem_type());

@@ -17,6 +17,7 @@
 #include <map>
 
 #include "libmesh/hip_csr_matrix.h"
+#include "libmesh/device_dof_data.h"
 
 #include <tuple>
 
@@ -28,7 +29,8 @@ namespace libMesh {
 
 
 template <typename T>
-HipCsrMatrix<T>::HipCsrMatrix(int device) : _device(device), _ctx(HipDevice::context(device)) {}
+HipCsrMatrix<T>::HipCsrMatrix(const Parallel::Communicator &comm, int device)
+    : SparseMatrix<T>(comm), _device(device), _ctx(HipDevice::context(device)) {}
 
 template <typename T>
 HipCsrMatrix<T>::~HipCsrMatrix() {
@@ -60,41 +62,46 @@ void HipCsrMatrix<T>::clear() {
 }
 
 template <typename T>
+void HipCsrMatrix<T>::update_sparsity_pattern(const SparsityPattern::Graph &sparsity_pattern) {
+  _graph_rp.assign(1, 0);
+  _graph_ci.clear();
+  for (const auto &row : sparsity_pattern) {
+    for (std::size_t k = 0; k < row.size(); ++k) {
+      if (k && row[k] <= row[k - 1]) libmesh_error_msg("update_sparsity_pattern: row columns must ascend");
+      _graph_ci.push_back((int32_t)row[k]);
+    }
+    _graph_rp.push_back((int32_t)_graph_ci.size());
+  }
+}
+
+template <typename T>
 void HipCsrMatrix<T>::init(ParallelType) {
   const DofMap *dm = this->_dof_map;
-  if (!dm || !dm->mesh()) libmesh_error_msg("HipCsrMatrix::init: attach a distributed DofMap first");
+  if (!dm) libmesh_error_msg("HipCsrMatrix::init: attach a distributed DofMap first");
+  const std::vector<int32_t> graph_rp = _graph_rp, graph_ci = _graph_ci;
   clear();
-  const MeshBase &mesh = *dm->mesh();
-  hip_check(lmh_mesh_create(_ctx, (int)mesh.elem_type(), mesh.n_nodes(), mesh.coords().data(), mesh.n_elem(),
-                            mesh.connectivity().data(), &_mesh),
-            "HipCsrMatrix::init (mesh)");
-  hip_check(lmh_dofmap_create(_ctx, _mesh, (int)dm->n_variables(), dm->n_dofs(), dm->elem_dofs().data(),
-                              dm->dirichlet_constrained().data(), &_dm),
-            "HipCsrMatrix::init (dof map)");
-  // constraint rows of the DofMap (processed on the device), then the pattern
-  // that couples each element's DoFs with the DoFs they are constrained by
-  const auto &rows = dm->added_constraint_rows();
-  const int flags = (dm->get_error_on_constraint_loop() ? LMH_CONSTRAINT_ERROR_ON_LOOP : 0) |
-                    (dm->asymmetric_constraint_rows() ? 0 : LMH_CONSTRAINT_SYMMETRIC_ROWS);
-  if (!rows.empty() || flags) {
-    std::vector<int32_t> dof, ptr{0}, col;
-    std::vector<double> coef, val;
-    for (const auto &r : rows) {
-      dof.push_back((int32_t)std::get<0>(r));
-      for (const auto &it : std::get<1>(r)) {
-        col.push_back((int32_t)it.first);
-        coef.push_back(it.second);
-      }
-      ptr.push_back((int32_t)col.size());
-      val.push_back(std::get<2>(r));
-    }
-    hip_check(lmh_dofmap_set_constraints(_dm, (int64_t)dof.size(), dof.data(), ptr.data(), col.data(), coef.data(),
-                                         val.data(), flags),
-              "HipCsrMatrix::init (constraint rows)");
-  }
-  hip_check(lmh_matrix_create(_ctx, _dm, &_A), "HipCsrMatrix::init (sparsity)");
   _rows = dm->n_dofs();
   _cols = _rows;
+  if (_mesh_src) {
+    // pattern, scatter maps and constraint rows on the device, from the mesh
+    // and DoF map read through their libMesh accessors
+    DeviceDofData dev;
+    build_device_dofs(_ctx, *_mesh_src, *dm, _asymmetric ? 0 : LMH_CONSTRAINT_SYMMETRIC_ROWS, dev);
+    hip_check(lmh_matrix_create(_ctx, dev.dofmap, &_A), "HipCsrMatrix::init (sparsity)");
+    _mesh = dev.mesh;
+    _dm = dev.dofmap;
+    dev.mesh = nullptr;
+    dev.dofmap = nullptr;
+  } else {
+    // libMesh's graph (DofMap::update_sparsity_pattern before init)
+    if (graph_rp.size() != _rows + 1)
+      libmesh_error_msg("HipCsrMatrix::init: no pattern (attach_mesh, or update_sparsity_pattern with "
+                        << _rows << " rows; got " << (graph_rp.empty() ? 0 : graph_rp.size() - 1) << ")");
+    hip_check(lmh_matrix_create_csr2(_ctx, _rows, _cols, graph_rp.data(), graph_ci.data(), &_A),
+              "HipCsrMatrix::init (graph)");
+    _graph_rp = graph_rp;
+    _graph_ci = graph_ci;
+  }
   _nnz = lmh_matrix_nnz(_A);
   _rp.resize(_rows + 1);
   _ci.resize(_nnz);
@@ -323,7 +330,7 @@ void HipCsrMatrix<T>::print_personal(std::ostream &os) const {
 template <typename T>
 void HipCsrMatrix<T>::get_diagonal(NumericVector<T> &dest) const {
   pull_values();
-  if (dest.size() != _rows) dest.init(_rows);
+  if (dest.size() != _rows) dest.init(_rows, _rows);
   for (numeric_index_type r = 0; r < _rows; ++r) {
     const int64_t k = slot(r, r);
     dest.set(r, k < 0 ? T(0) : _vals[k]);
@@ -356,22 +363,60 @@ void HipCsrMatrix<T>::get_row(numeric_index_type i, std::vector<numeric_index_ty
 }
 
 template <typename T>
-void HipCsrMatrix<T>::vector_mult(NumericVector<T> &dest, const NumericVector<T> &arg) const {
-  auto *d = dynamic_cast<HipVector<T> *>(&dest);
-  const auto *a = dynamic_cast<const HipVector<T> *>(&arg);
-  if (!d || !a) libmesh_error_msg("vector_mult: HipVector arguments required");
+void HipCsrMatrix<T>::device_vector_mult(HipVector<T> &dest, const HipVector<T> &arg) const {
   if (!_closed) libmesh_error_msg("vector_mult before close()");
-  if (a->size() != _cols) libmesh_error_msg("vector_mult: argument has " << a->size() << " entries, need " << _cols);
-  if (d->size() != _rows) d->init(_rows);
-  hip_check(lmh_spmv(_A, (const double *)a->device_data(), (double *)d->device_data()), "vector_mult");
-  d->mark_device_modified();
+  if (arg.size() != _cols) libmesh_error_msg("vector_mult: argument has " << arg.size() << " entries, need " << _cols);
+  if (dest.size() != _rows) dest.init(_rows, _rows);
+  hip_check(lmh_spmv(_A, (const double *)arg.device_data(), (double *)dest.device_data()), "vector_mult");
+  dest.mark_device_modified();
+}
+
+template <typename T>
+void HipCsrMatrix<T>::vector_mult_transpose(NumericVector<T> &dest, const NumericVector<T> &arg) const {
+  if (!_closed) libmesh_error_msg("vector_mult_transpose before close()");
+  if (arg.size() != _rows) libmesh_error_msg("vector_mult_transpose: argument has " << arg.size() << " entries, need " << _rows);
+  pull_values();
+  std::vector<T> a, y(_cols, T(0));
+  arg.localize(a);
+  for (numeric_index_type r = 0; r < _rows; ++r)
+    for (int32_t k = _rp[r]; k < _rp[r + 1]; ++k) y[_ci[k]] += _vals[k] * a[r];
+  if (dest.size() != _cols) dest.init(_cols, _cols);
+  dest = y;
+}
+
+// rows emptied, diag_value on their diagonals (PetscMatrix::zero_rows semantics)
+template <typename T>
+void HipCsrMatrix<T>::zero_rows(std::vector<numeric_index_type> &rows, T diag_value) {
+  close();
+  pull_values();
+  for (numeric_index_type r : rows) {
+    if (r >= _rows) libmesh_error_msg("zero_rows: row " << r << " out of range");
+    for (int32_t k = _rp[r]; k < _rp[r + 1]; ++k) _vals[k] = (numeric_index_type)_ci[k] == r ? diag_value : T(0);
+  }
+  _vals_host_dirty = true;
+  push_values();
+}
+
+// every stored value times `scale`, on the device
+template <typename T>
+void HipCsrMatrix<T>::scale(const T scale) {
+  close();
+  const int32_t *rp = nullptr, *ci = nullptr;
+  double *vals = nullptr;
+  hip_check(lmh_matrix_device_arrays(_A, &rp, &ci, &vals), "HipCsrMatrix::scale");
+  hip_check(lmh_vector_op(_ctx, LMH_VEC_SCALE, _nnz, (double)scale, nullptr, nullptr, vals), "HipCsrMatrix::scale");
+  _vals_host_valid = false;
 }
 
 template <typename T>
 std::unique_ptr<SparseMatrix<T>> HipCsrMatrix<T>::zero_clone() const {
-  auto c = std::make_unique<HipCsrMatrix<T>>(_device);
-  if (this->_dof_map && _dm) {
+  auto c = std::make_unique<HipCsrMatrix<T>>(this->comm(), _device);
+  if (this->_dof_map && (_dm || !_graph_rp.empty())) {
     c->attach_dof_map(*this->_dof_map);
+    if (_mesh_src) c->attach_mesh(*_mesh_src);
+    c->set_asymmetric_constraint_rows(_asymmetric);
+    c->_graph_rp = _graph_rp;
+    c->_graph_ci = _graph_ci;
     c->init();
   } else {
     c->init(_rows, _cols, _rows, _cols);
@@ -406,6 +451,15 @@ void HipCsrMatrix<T>::print_matlab(const std::string &name) const {
   std::ofstream os(name);
   if (!os.good()) libmesh_error_msg("HipCsrMatrix::print_matlab: cannot write " << name);
   MatrixIO::write_matlab(os, _rows, _cols, _rp, _ci, v);
+}
+
+template <typename T>
+void HipCsrMatrix<T>::print_petsc_binary(const std::string &filename) const {
+  if (!_closed) libmesh_error_msg("HipCsrMatrix::print_petsc_binary before close()");
+  pull_values();
+  const bool wide = filename.size() >= 8 && filename.compare(filename.size() - 8, 8, ".petsc64") == 0;
+  std::vector<double> v(_vals.begin(), _vals.end());
+  MatrixIO::write_petsc_binary(filename, wide ? 8 : 4, _rows, _cols, _rp, _ci, v);
 }
 
 // A read matrix takes the pattern of the file's entries (explicit zeros kept),

@@ -158,5 +158,33 @@ void write_matlab(std::ostream &os, std::size_t m, std::size_t n, const std::vec
      << "Mat_sparse = spconvert(zzz);" << std::endl;
 }
 
+// Big-endian words of `bytes` bytes (the layout read_petsc_binary reads).
+static void put_be(std::vector<unsigned char> &out, uint64_t v, int bytes) {
+  for (int k = bytes - 1; k >= 0; --k) out.push_back((unsigned char)(v >> (8 * k)));
+}
+
+void write_petsc_binary(const std::string &filename, int ib, std::size_t m, std::size_t n,
+                        const std::vector<int32_t> &rp, const std::vector<int32_t> &ci,
+                        const std::vector<double> &vals) {
+  if (ib != 4 && ib != 8) libmesh_error_msg("PETSc binary: index width must be 4 or 8 bytes");
+  const std::size_t nz = vals.size();
+  std::vector<unsigned char> b;
+  b.reserve(4 * ib + m * ib + nz * (ib + 8));
+  put_be(b, 1211216, ib);
+  put_be(b, m, ib);
+  put_be(b, n, ib);
+  put_be(b, nz, ib);
+  for (std::size_t r = 0; r < m; ++r) put_be(b, (uint64_t)(rp[r + 1] - rp[r]), ib);
+  for (std::size_t k = 0; k < nz; ++k) put_be(b, (uint64_t)ci[k], ib);
+  for (std::size_t k = 0; k < nz; ++k) {
+    uint64_t bits;
+    std::memcpy(&bits, &vals[k], 8);
+    put_be(b, bits, 8);
+  }
+  std::ofstream os(filename, std::ios::binary);
+  if (!os.good()) libmesh_error_msg("ERROR: cannot write to file:\n\t" << filename);
+  os.write(reinterpret_cast<const char *>(b.data()), (std::streamsize)b.size());
+}
+
 }  // namespace MatrixIO
 }  // namespace libMesh

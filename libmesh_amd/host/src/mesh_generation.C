@@ -65,7 +65,7 @@ void build_cube(ReplicatedMesh &mesh, unsigned nx, unsigned ny, unsigned nz, Rea
   const int f = linear ? 1 : 2;
   const int N[3] = {f * (int)nx, f * (int)ny, f * (int)nz};
   const Real lo[3] = {xmin, ymin, zmin}, span[3] = {xmax - xmin, ymax - ymin, zmax - zmin};
-  mesh = ReplicatedMesh();
+  mesh.clear();
   mesh._cells = {N[0], N[1], N[2]};
   const size_t nn = (size_t)(N[0] + 1) * (N[1] + 1) * (N[2] + 1);
   mesh._xyz.resize(3 * nn);

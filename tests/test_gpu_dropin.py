@@ -21,10 +21,22 @@ def run(args, tmp_path, expect=0):
     return r.stdout
 
 
+def test_numeric_vector_contract(tmp_path):
+    """tests/numerics/numeric_vector_test.h of the reference on HipVector: every
+    check through the derived and the base class."""
+    out = run(["numeric_vector_contract"], tmp_path)
+    assert "FAIL" not in out and "0 failures" in out, out
+    for name in ("Localize", "LocalizeToOne", "LocalizeIndices", "Norms", "Operations", "WriteAndRead"):
+        assert f"HipVector {name}" in out and f"HipVectorBase {name}" in out, (name, out)
+
+
 def test_sparse_matrix_contract(tmp_path):
     out = run(["sparse_matrix_contract", os.path.join(ROOT, "tests", "golden", "matrices")], tmp_path)
     assert "FAIL" not in out and "0 failures" in out, out
     assert "testReadMatlab geom_4" in out and "PETSc binary" in out, out
+    for what in ("update_sparsity_pattern: element matrices sum", "vector_mult then vector_mult_add",
+                 "print_petsc_binary/read_petsc_binary round trip", "zero_rows with a diagonal value"):
+        assert what in out, (what, out)
 
 
 def test_poisson_c0_dropin(tmp_path):

@@ -1,0 +1,249 @@
+"""The libMesh-API headers of the drop-in classes against the reference's own
+headers (skipped when the reference tree is absent).
+
+For each checked class and member name:
+  * every declaration in this library's header is callable like one of the
+    reference's overloads: the same parameter types in order (names, default
+    values and top-level const of by-value parameters aside), the same
+    const-ness, the reference possibly taking further defaulted parameters;
+  * every pure virtual member of the reference's SparseMatrix, NumericVector,
+    ShellMatrix and LinearSolver is declared by this library's class, so the
+    GPU classes implement the whole abstract interface.
+"""
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MIRROR = os.path.join(ROOT, "libmesh_amd", "host", "include", "libmesh")
+REF = "/root/reference/include"
+
+pytestmark = pytest.mark.skipif(not os.path.isdir(REF), reason="reference tree not present")
+
+# (reference header, reference class, mirror header, mirror class, member names)
+CHECKS = [
+    ("numerics/sparse_matrix.h", "SparseMatrix", "sparse_matrix.h", "SparseMatrix",
+     ["SparseMatrix", "build", "solver_package", "initialized", "attach_dof_map", "need_full_sparsity_pattern",
+      "update_sparsity_pattern", "init", "clear", "zero", "zero_clone", "clone", "zero_rows", "close", "flush", "m",
+      "local_m", "local_n", "n", "row_start", "row_stop", "col_start", "col_stop", "set", "add", "add_matrix",
+      "add_block_matrix", "matrix_matrix_mult", "add_sparse_matrix", "operator()", "l1_norm", "linfty_norm",
+      "l1_norm_diff", "closed", "n_nonzeros", "print", "print_personal", "print_matlab", "print_petsc_binary", "read",
+      "read_matlab", "read_petsc_binary", "create_submatrix", "reinit_submatrix", "vector_mult", "vector_mult_add",
+      "get_diagonal", "get_transpose", "get_row", "scale"]),
+    ("numerics/numeric_vector.h", "NumericVector", "numeric_vector.h", "NumericVector",
+     ["NumericVector", "build", "initialized", "type", "set_type", "closed", "close", "clear", "zero", "zero_clone",
+      "clone", "init", "operator=", "min", "max", "sum", "l1_norm", "l2_norm", "linfty_norm", "subset_l1_norm",
+      "subset_l2_norm", "subset_linfty_norm", "l2_norm_diff", "l1_norm_diff", "size", "local_size",
+      "first_local_index", "last_local_index", "operator()", "el", "get", "operator+=", "operator-=", "operator*=",
+      "operator/=", "reciprocal", "conjugate", "add", "add_vector", "add_vector_transpose", "insert", "scale", "abs",
+      "dot", "localize", "localize_to_one", "compare", "local_relative_compare", "global_relative_compare",
+      "pointwise_mult", "pointwise_divide", "print", "print_global", "print_matlab", "read_matlab",
+      "create_subvector", "get_subvector", "restore_subvector", "swap", "max_allowed_id", "readable", "compatible"]),
+    ("numerics/shell_matrix.h", "ShellMatrix", "shell_matrix.h", "ShellMatrix",
+     ["ShellMatrix", "m", "n", "vector_mult", "vector_mult_add", "get_diagonal", "clear", "init"]),
+    ("solvers/linear_solver.h", "LinearSolver", "linear_solver.h", "LinearSolver",
+     ["LinearSolver", "build", "initialized", "clear", "init", "init_systems", "solver_type", "set_solver_type",
+      "preconditioner_type", "set_preconditioner_type", "restrict_solve_to", "solve", "adjoint_solve",
+      "get_converged_reason"]),
+    ("systems/implicit_system.h", "ImplicitSystem", "system.h", "ImplicitSystem",
+     ["adjoint_solve", "adjoint_qoi_parameter_sensitivity", "assemble_residual_derivatives", "assembly",
+      "get_system_matrix", "get_linear_solver"]),
+    ("systems/system.h", "System", "system.h", "System",
+     ["attach_assemble_function", "attach_assemble_object", "attach_constraint_object", "attach_QOI_function",
+      "attach_QOI_object", "attach_QOI_derivative", "attach_QOI_derivative_object", "assemble_qoi",
+      "assemble_qoi_derivative", "n_qois", "init_qois", "set_qoi", "get_qoi_value", "get_qoi_values",
+      "add_adjoint_solution", "get_adjoint_solution", "add_adjoint_rhs", "get_adjoint_rhs", "add_sensitivity_rhs",
+      "get_sensitivity_rhs", "set_adjoint_already_solved", "is_adjoint_already_solved", "add_vector", "get_vector",
+      "have_vector", "n_vectors", "add_matrix", "get_matrix", "have_matrix", "adjoint_solve",
+      "adjoint_qoi_parameter_sensitivity"]),
+    ("systems/qoi_set.h", "QoISet", "qoi_set.h", "QoISet",
+     ["QoISet", "clear", "add_index", "remove_index", "set_weight", "weight", "has_index"]),
+    ("systems/sensitivity_data.h", "SensitivityData", "sensitivity_data.h", "SensitivityData",
+     ["SensitivityData", "clear", "allocate_data", "derivative", "operator[]"]),
+    ("systems/parameter_vector.h", "ParameterVector", "parameter_vector.h", "ParameterVector",
+     ["ParameterVector", "size", "clear", "push_back", "operator[]"]),
+    ("base/dof_map_base.h", "DofMapBase", "dof_map.h", "DofMap", ["n_dofs"]),
+    ("base/dof_map.h", "DofMap", "dof_map.h", "DofMap",
+     ["add_dirichlet_boundary", "distribute_dofs", "n_variables", "variable_name", "variable_type",
+      "dof_indices", "is_constrained_dof", "n_constrained_dofs", "add_constraint_row", "constraint_rows_begin",
+      "constraint_rows_end", "get_primal_constraint_values", "set_error_on_constraint_loop", "process_constraints",
+      "enforce_constraints_exactly", "enforce_adjoint_constraints_exactly"]),
+    ("mesh/mesh_base.h", "MeshBase", "replicated_mesh.h", "ReplicatedMesh",
+     ["n_nodes", "n_elem", "point", "elem_ref", "elem_ptr", "mesh_dimension"]),
+    ("parallel/parallel_object.h", "ParallelObject", "parallel_object.h", "ParallelObject",
+     ["ParallelObject", "comm", "n_processors", "processor_id"]),
+    ("base/libmesh.h", "LibMeshInit", "libmesh.h", "LibMeshInit", ["LibMeshInit", "comm"]),
+]
+# the reference classes whose pure virtuals this library's classes must declare
+ABSTRACT = {"SparseMatrix", "NumericVector", "ShellMatrix", "LinearSolver"}
+# Reference overloads this library leaves out on purpose (serial build, no
+# complex numbers, no PETSc / HDF5 / Exodus backends).
+OMITTED = {
+    ("MeshBase", "elem_ref"),  # the non-const overload: this mesh is built only by mesh generation
+    ("MeshBase", "elem_ptr"),
+}
+
+
+def strip_comments(text):
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", " ", text)
+    text = re.sub(r"^\s*#.*$", " ", text, flags=re.M)  # preprocessor lines
+    return text
+
+
+def class_body(text, name):
+    """Body of `class name ... {` at brace depth 0 of its declaration."""
+    for m in re.finditer(r"\b(class|struct)\s+(?:\w+\s+)?" + re.escape(name) + r"\b[^;{(]*\{", text):
+        i = m.end()
+        depth = 1
+        j = i
+        while depth and j < len(text):
+            if text[j] == "{":
+                depth += 1
+            elif text[j] == "}":
+                depth -= 1
+            j += 1
+        return text[i:j - 1]
+    raise AssertionError(f"class {name} not found")
+
+
+def top_level(body):
+    """The class body with nested braces (inline bodies, nested classes) blanked."""
+    out, depth = [], 0
+    for ch in body:
+        if ch == "{":
+            depth += 1
+            out.append(";" if depth == 1 else " ")
+            continue
+        if ch == "}":
+            depth -= 1
+            out.append(" ")
+            continue
+        out.append(ch if depth == 0 else " ")
+    return "".join(out)
+
+
+def name_regex(name):
+    if name.startswith("operator"):
+        op = name[len("operator"):]
+        return r"\boperator\s*" + r"\s*".join(re.escape(c) for c in op) + r"\s*\("
+    return r"(?<![\w~:.])" + re.escape(name) + r"\s*\("
+
+
+def split_params(s):
+    parts, depth, cur = [], 0, ""
+    for ch in s:
+        if ch in "<([":
+            depth += 1
+        elif ch in ">)]":
+            depth -= 1
+        if ch == "," and depth == 0:
+            parts.append(cur)
+            cur = ""
+        else:
+            cur += ch
+    if cur.strip():
+        parts.append(cur)
+    return parts
+
+
+BUILTIN = {"int", "long", "short", "char", "bool", "double", "float", "void", "unsigned", "signed"}
+QUALIFIER = {"const", "unsigned", "signed", "long", "short", "volatile"}
+
+
+def norm_param(p):
+    """(type, has_default) of one parameter declaration."""
+    parts = re.split(r"(?<![=!<>])=(?!=)", p, maxsplit=1)
+    default = len(parts) > 1
+    p = parts[0].replace("libMesh::", "").replace("TIMPI::", "").replace("std::", "")
+    p = re.sub(r"\*\s*const\b", "*", p)  # "Elem * const elem": the pointer's own const
+    tokens = re.findall(r"[A-Za-z_]\w*(?:::\w+)*|[<>&*,\[\]()]", p)
+    # a trailing identifier that follows a type is the parameter's name
+    if (len(tokens) >= 2 and re.fullmatch(r"[A-Za-z_]\w*", tokens[-1]) and tokens[-1] not in BUILTIN
+            and tokens[-2] not in QUALIFIER):
+        tokens = tokens[:-1]
+    # top-level const of a by-value parameter is not part of the signature
+    if tokens and tokens[0] == "const" and "&" not in tokens and "*" not in tokens:
+        tokens = tokens[1:]
+    return " ".join(tokens), default
+
+
+def declarations(body, name):
+    """(params, const, pure, n_defaults) of each declaration of `name` at class top level."""
+    out = []
+    for m in re.finditer(name_regex(name), body):
+        # skip uses inside expressions: a declaration starts a statement
+        before = body[:m.start()].rstrip()
+        if before and before[-1] not in ";:{}" and not re.search(
+                r"(\bvirtual|\bstatic|\bexplicit|\binline|[\w>&*\]]|\boperator)$", before):
+            continue
+        i, depth = m.end(), 1
+        j = i
+        while depth and j < len(body):
+            if body[j] == "(":
+                depth += 1
+            elif body[j] == ")":
+                depth -= 1
+            j += 1
+        params = body[i:j - 1]
+        tail = body[j:j + 80]
+        tail = tail.split(";")[0]
+        is_const = re.match(r"\s*const\b", tail) is not None
+        pure = re.search(r"=\s*0\s*$", tail.strip()) is not None or re.search(r"=\s*0\b", tail) is not None
+        ps = [norm_param(p) for p in split_params(params) if p.strip() and p.strip() != "void"]
+        out.append((tuple(p for p, _ in ps), is_const, pure, tuple(d for _, d in ps)))
+    return out
+
+
+def compatible(mine, ref):
+    (mp, mc, _, _), (rp, rc, _, rd) = mine, ref
+    if mc != rc or len(mp) > len(rp) or mp != rp[:len(mp)]:
+        return False
+    return all(rd[len(mp):])
+
+
+def load(path, cls):
+    return top_level(class_body(strip_comments(open(path).read()), cls))
+
+
+@pytest.mark.parametrize("ref_h,ref_cls,my_h,my_cls,names", CHECKS, ids=[c[1] for c in CHECKS])
+def test_declarations_match_reference(ref_h, ref_cls, my_h, my_cls, names):
+    ref = load(os.path.join(REF, ref_h), ref_cls)
+    mine = load(os.path.join(MIRROR, my_h), my_cls)
+    problems = []
+    for name in names:
+        ours = declarations(mine, name)
+        theirs = declarations(ref, name)
+        if not ours:
+            problems.append(f"{my_cls}::{name}: not declared")
+            continue
+        if not theirs:
+            problems.append(f"{ref_cls}::{name}: not found in the reference (check list)")
+            continue
+        for d in ours:
+            if not any(compatible(d, r) for r in theirs):
+                problems.append(f"{my_cls}::{name}{d[:2]} matches none of {[r[:2] for r in theirs]}")
+        if ref_cls in ABSTRACT:
+            for r in theirs:
+                if r[2] and not any(d[:2] == r[:2] for d in ours):
+                    problems.append(f"pure virtual {ref_cls}::{name}{r[:2]} missing")
+    assert not problems, "\n".join(problems)
+
+
+@pytest.mark.parametrize("ref_h,cls,my_h", [("numerics/sparse_matrix.h", "SparseMatrix", "sparse_matrix.h"),
+                                            ("numerics/numeric_vector.h", "NumericVector", "numeric_vector.h"),
+                                            ("numerics/shell_matrix.h", "ShellMatrix", "shell_matrix.h"),
+                                            ("solvers/linear_solver.h", "LinearSolver", "linear_solver.h")])
+def test_every_pure_virtual_declared(ref_h, cls, my_h):
+    ref = load(os.path.join(REF, ref_h), cls)
+    mine = load(os.path.join(MIRROR, my_h), cls)
+    names = set(re.findall(r"virtual[^;]*?\b(operator\s*[^\s(]+|\w+)\s*\([^;]*=\s*0\s*;", ref))
+    missing = []
+    for name in sorted(names):
+        name = re.sub(r"\s+", "", name)
+        for r in declarations(ref, name):
+            if r[2] and not any(d[:2] == r[:2] for d in declarations(mine, name)):
+                missing.append(f"{cls}::{name}{r[:2]}")
+    assert names, "no pure virtuals parsed"
+    assert not missing, "\n".join(missing)
