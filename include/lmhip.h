@@ -203,6 +203,43 @@ int lmh_assemble_residual_poisson(lmh_matrix *A, const lmh_mesh *m, const lmh_do
 int lmh_element_poisson(lmh_context *ctx, const lmh_mesh *m, const lmh_dofmap *dm, int npts, const double *kappa_d,
                         int f_kind, int constrain, double *Ke_d, double *Fe_d);
 
+/* ---- user physics: the element loop of FEMSystem::assembly
+ * (src/systems/fem_system.C:875-1200) with element_time_derivative
+ * (include/physics/diff_physics.h:125) written as device code.  The template
+ * element loop is in lmh_fem_physics.h (compiled with the caller's hipcc);
+ * these calls give it the device arrays of a mesh and DoF map and put its
+ * element results into the system matrix and a vector. ---- */
+typedef struct lmh_fem_data {
+  int elem_type;              /* LMH_HEX8 ... */
+  int n_nodes;                /* nodes per element */
+  int n_vars;                 /* variables (one Lagrange type) */
+  int n_dofs;                 /* DoFs per element, n_vars * n_nodes, variable-major */
+  int n_qp;                   /* quadrature points */
+  int64_t n_elem, n_dofs_total;
+  const double *xyz;          /* node coordinates, 3 per node */
+  const int32_t *conn;        /* n_elem * n_nodes, libMesh local node order */
+  const int32_t *elem_dofs;   /* n_elem * n_dofs */
+  const uint8_t *constrained; /* n_dofs_total: nonzero = homogeneous Dirichlet DoF */
+  const double *w;            /* QGauss weights, n_qp */
+  const double *phi;          /* reference shapes, phi[a*n_qp + q] */
+  const double *dref;         /* reference derivatives, dref[(a*n_qp + q)*3 + r] */
+  int *flag;                  /* device int the element loop sets to 1 on a degenerate map */
+  void *stream;               /* the context's hipStream_t */
+} lmh_fem_data;
+/* Fills *out for the mesh and DoF map with the QGauss rule of `npts` points
+ * per direction (hex) or that order (tet), and clears *flag.  Only
+ * homogeneous Dirichlet rows (the element loop's mask): a DoF map with general
+ * constraint rows gives LMH_ERR_UNSUPPORTED. */
+int lmh_fem_element_data(const lmh_mesh *m, const lmh_dofmap *dm, int npts, lmh_fem_data *out);
+/* LMH_ERR_DEGENERATE when the element loop set *flag (then clears it). */
+int lmh_fem_check_flag(const lmh_fem_data *d);
+/* SparseMatrix::add_matrix of every element matrix (n_elem * n_dofs^2 in
+ * element order, dofs of the matrix's DoF map) and NumericVector::add_vector
+ * of every element vector, summed per entry in ascending element order (the
+ * order of the reference's element loop): A's values and R_d (if given) are
+ * overwritten by the sums.  Ke_d may be NULL (R_d only) or Fe_d (A only). */
+int lmh_matrix_scatter_elements(lmh_matrix *A, const double *Ke_d, const double *Fe_d, double *R_d);
+
 /* ---- linear algebra ---- */
 /* y = A x (SparseMatrix::vector_mult, sparse_matrix.C:237-253). */
 int lmh_spmv(lmh_matrix *A, const double *x_d, double *y_d);

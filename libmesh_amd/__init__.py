@@ -92,6 +92,9 @@ HIP_API = {
     "lmh_matrix_get_values_fp32": (_I, [_P, _P]),
     "lmh_matrix_add_element_matrices": (_I, [_P, _I64, _I, _P, _P]),
     "lmh_vector_add_element_vectors": (_I, [_P, _P, _I64, _I, _P, _P]),
+    "lmh_fem_element_data": (_I, [_P, _P, _I, _P]),
+    "lmh_fem_check_flag": (_I, [_P]),
+    "lmh_matrix_scatter_elements": (_I, [_P, _P, _P, _P]),
     "lmh_assemble_poisson": (_I, [_P, _P, _P, _I, _P, _I, _P, _I, _I]),
     "lmh_element_poisson": (_I, [_P, _P, _P, _I, _P, _I, _I, _P, _P]),
     "lmh_spmv": (_I, [_P, _P, _P]),

@@ -960,4 +960,73 @@ int lmh_adjoint_kappa_gradient(const lmh_mesh *m, const lmh_dofmap *dm, int npts
   return read_flag(ctx, "lmh_adjoint_kappa_gradient");
 }
 
+// ---------------------------------------------------------------------------
+// User physics (lmh_fem_physics.h): arrays for the template element loop and
+// the element-order scatter of its results.
+// ---------------------------------------------------------------------------
+int lmh_fem_element_data(const lmh_mesh *m, const lmh_dofmap *dm, int npts, lmh_fem_data *out) {
+  LMH_CHECK(m && dm && out, LMH_ERR_INVALID, "null argument");
+  LMH_CHECK(dm->mesh == m, LMH_ERR_INVALID, "dofmap and mesh do not belong together");
+  LMH_CHECK(dm->cr.n_general == 0, LMH_ERR_UNSUPPORTED,
+            "user physics: the element loop applies homogeneous Dirichlet rows only, not general constraint rows");
+  lmh_context *ctx = m->ctx;
+  const FETable *t = nullptr;
+  LMH_TRY(fe_table(ctx, m->type, npts, &t));
+  out->elem_type = m->type;
+  out->n_nodes = m->npe;
+  out->n_vars = dm->nv;
+  out->n_dofs = dm->dpe;
+  out->n_qp = t->nq;
+  out->n_elem = m->n_elem;
+  out->n_dofs_total = dm->n_dofs;
+  out->xyz = m->xyz;
+  out->conn = m->conn;
+  out->elem_dofs = dm->edofs;
+  out->constrained = dm->cons;
+  out->w = t->d_w();
+  out->phi = t->d_phi();
+  out->dref = t->d_dref();
+  out->flag = ctx->d_flags + 15;  // slot of the user element loop
+  out->stream = (void *)ctx->stream;
+  LMH_HIP(hipMemsetAsync(out->flag, 0, sizeof(int), ctx->stream));
+  return LMH_OK;
+}
+
+int lmh_fem_check_flag(const lmh_fem_data *d) {
+  LMH_CHECK(d && d->flag, LMH_ERR_INVALID, "null argument");
+  hipStream_t s = (hipStream_t)d->stream;
+  int flag = 0;
+  LMH_HIP(hipMemcpyAsync(&flag, d->flag, sizeof(int), hipMemcpyDeviceToHost, s));
+  LMH_HIP(hipStreamSynchronize(s));
+  if (flag) {
+    LMH_HIP(hipMemsetAsync(d->flag, 0, sizeof(int), s));
+    return set_error(LMH_ERR_DEGENERATE, "element loop: Jacobian determinant <= 0 (degenerate map)");
+  }
+  return LMH_OK;
+}
+
+int lmh_matrix_scatter_elements(lmh_matrix *A, const double *Ke_d, const double *Fe_d, double *R_d) {
+  LMH_CHECK(A && A->dm && (Ke_d || Fe_d), LMH_ERR_INVALID, "null argument");
+  LMH_CHECK(!R_d || Fe_d, LMH_ERR_INVALID, "a residual needs element vectors");
+  const lmh_dofmap *dm = A->dm;
+  LMH_CHECK(dm->cr.n_general == 0, LMH_ERR_UNSUPPORTED, "element scatter with general constraint rows");
+  LMH_CHECK(A->cons_version == dm->cr.version, LMH_ERR_INVALID,
+            "the constraints changed after the matrix was created: create the matrix again");
+  lmh_context *ctx = A->ctx;
+  if (Ke_d) {
+    LMH_CHECK(A->vals, LMH_ERR_UNSUPPORTED, "matrix holds FP32 values only");
+    LMH_TRY(scatter_impl<double>(A, Ke_d, Fe_d, A->vals, R_d, LMH_SCATTER_GATHER));
+    A->have32 = false;
+    return LMH_OK;
+  }
+  if (R_d) {
+    // residual only: the DoF map's incidence lists hold (element, local) in
+    // ascending order, so each entry sums in element order
+    LMH_LAUNCH(ctx, "fill", k_fill<double>, dim3(grid_for(A->rows, 256, 8192)), dim3(256), 0, R_d, A->rows, 0.0);
+    LMH_LAUNCH(ctx, "add_element_vectors", k_add_vec_gather, dim3(grid_for(A->rows, 256, 8192)), dim3(256), 0,
+               dm->inc_ptr, dm->inc, A->rows, Fe_d, R_d);
+  }
+  return check_launch("lmh_matrix_scatter_elements");
+}
+
 }  // extern "C"

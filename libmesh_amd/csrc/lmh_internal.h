@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/lmhip.h"
+#include "../../include/lmh_fe_map.h"
 
 namespace lmh {
 
@@ -176,12 +177,13 @@ struct lmh_matrix {
   // FP32 operator column compression (x windows): each row block lists its
   // distinct columns once, sorted, as 16-bit offsets woff16[w0 .. w0+nu),
   // col = off + (off <= s0 ? b0 : base1); every entry stores the 16-bit index
-  // of its column in that list.  rbmeta[bk] = {b0, base1, s0 | nu << 16, w0}.
+  // of its column in that list.  rbmeta[2bk] = {r0, r1, k0, cnt} (the block's rows and
+  // entries), rbmeta[2bk+1] = {b0, base1, s0 | nu << 16, w0}.
   int fp32_index = 1;          // 1: use the compressed columns when they fit
   int idx16_state = 0;         // 0 not built, 1 built, -1 does not fit
   uint16_t *widx16 = nullptr;  // nnz: window index of each entry's column
   uint16_t *woff16 = nullptr;  // sum of nu over the row blocks
-  int4 *rbmeta = nullptr;      // nrb
+  int4 *rbmeta = nullptr;      // 2 nrb
   uint16_t *off = nullptr;  // n_elem*dpe*dpe slot offsets within rows
   int32_t *diag = nullptr;  // rows: position of the diagonal entry, -1 if absent
   int ncolors = 0;
@@ -248,36 +250,7 @@ int release_constraints(lmh_constraints &c);
 // cofactors as FEMap::compute_single_point_map (fe_map.C:1046-1068),
 // G[r][c] = d xi_r / d x_c.  Returns det J; G is only meaningful when det > 0.
 __device__ inline double qp_map(const double *X, const double *dref, int n, int nq, int q, double G[3][3]) {
-  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // J[r][c] = d x_c / d xi_r
-#pragma unroll 4
-  for (int a = 0; a < n; ++a) {
-    const double *d = &dref[(a * nq + q) * 3];
-    const double x = X[a * 3], y = X[a * 3 + 1], z = X[a * 3 + 2];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      J[r][0] += x * d[r];
-      J[r][1] += y * d[r];
-      J[r][2] += z * d[r];
-    }
-  }
-  const double dx_dxi = J[0][0], dy_dxi = J[0][1], dz_dxi = J[0][2];
-  const double dx_deta = J[1][0], dy_deta = J[1][1], dz_deta = J[1][2];
-  const double dx_dzeta = J[2][0], dy_dzeta = J[2][1], dz_dzeta = J[2][2];
-  const double det = (dx_dxi * (dy_deta * dz_dzeta - dz_deta * dy_dzeta) +
-                      dy_dxi * (dz_deta * dx_dzeta - dx_deta * dz_dzeta) +
-                      dz_dxi * (dx_deta * dy_dzeta - dy_deta * dx_dzeta));
-  if (!(det > 0.0)) return det;
-  const double inv = 1.0 / det;
-  G[0][0] = (dy_deta * dz_dzeta - dz_deta * dy_dzeta) * inv;
-  G[0][1] = (dz_deta * dx_dzeta - dx_deta * dz_dzeta) * inv;
-  G[0][2] = (dx_deta * dy_dzeta - dy_deta * dx_dzeta) * inv;
-  G[1][0] = (dz_dxi * dy_dzeta - dy_dxi * dz_dzeta) * inv;
-  G[1][1] = (dx_dxi * dz_dzeta - dz_dxi * dx_dzeta) * inv;
-  G[1][2] = (dy_dxi * dx_dzeta - dx_dxi * dy_dzeta) * inv;
-  G[2][0] = (dy_dxi * dz_deta - dz_dxi * dy_deta) * inv;
-  G[2][1] = (dz_dxi * dx_deta - dx_dxi * dz_deta) * inv;
-  G[2][2] = (dx_dxi * dy_deta - dy_dxi * dx_deta) * inv;
-  return det;
+  return lmh_qp_map(X, dref, n, nq, q, G);
 }
 
 // MI355X dispatches workgroups round-robin over its 8 XCDs; remap so each XCD

@@ -235,14 +235,24 @@ __device__ inline void rowblock_spmv(const int32_t *__restrict__ rb, int64_t nrb
 // FP64 results are bit-identical to the int32-column kernel too.
 typedef uint32_t vu4 __attribute__((ext_vector_type(4)));
 constexpr int UCAP = 768;  // distinct columns of one row block (x window)
+// Pipelining.  Every block's loads are addressed from registers only, so no
+// load waits on another in flight: a 32-byte descriptor per block (rows,
+// entry range, window meta, built with the windows) is loaded three blocks
+// ahead, the block's window offsets two blocks ahead, and its values, window
+// indices, row pointers and x-window gathers one block ahead, while the
+// current block runs its LDS phases.
 template <typename T, bool DOT>
-__device__ inline double rowblock_spmv16(const int32_t *__restrict__ rb, int64_t nrb, const int32_t *__restrict__ rp,
+__device__ inline double rowblock_spmv16(int64_t nrb, const int4 *__restrict__ desc, const int32_t *__restrict__ rp,
                                          const uint16_t *__restrict__ widx, const uint16_t *__restrict__ woff,
-                                         const int4 *__restrict__ meta, const T *__restrict__ v,
-                                         const T *__restrict__ x, T *__restrict__ y) {
+                                         const T *__restrict__ v, const T *__restrict__ x, T *__restrict__ y) {
   constexpr int CH = 16 / sizeof(T);  // entries per 16-byte value chunk
   typedef T vch __attribute__((ext_vector_type(CH)));
-  __shared__ T prod[RB_CAP + 1];  // + one sink slot for lanes whose chunk slot is outside the block
+  // the chunk's CH window indices (8 bytes for FP32, 4 for FP64)
+  typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+  typedef typename std::conditional<CH == 4, u32x2, uint32_t>::type ich;
+  // products at their chunk positions from the block's start rounded down to 8
+  // entries (slots outside the block hold unread values)
+  __shared__ __attribute__((aligned(16))) T prod[RB_CAP];
   __shared__ T xw[UCAP];
   // Row offsets and the x of the block's rows (for the (x, Ax) partials).
   // FP32: double-buffered, written before the first barrier.  FP64: one
@@ -251,90 +261,101 @@ __device__ inline double rowblock_spmv16(const int32_t *__restrict__ rb, int64_t
   constexpr int NB = sizeof(T) == 8 ? 1 : 2;
   __shared__ int32_t rowbuf[NB][BLK];
   __shared__ T xrow[NB][BLK];
-  __shared__ vu4 colbuf[BLK];         // 8 window indices per thread
-  const uint16_t *cb = (const uint16_t *)colbuf;
   constexpr int PER = RB_CAP / BLK, UPER = UCAP / BLK, NCH = PER / CH;
   static_assert(PER == 8 && UCAP % BLK == 0, "one 8-entry index chunk per thread");
   int64_t lo, hi;
   block_range(nrb, lo, hi);
-  const int tid = threadIdx.x;
-  int32_t r0 = 0, r1 = 0, k0 = 0, cnt = 0, rpn = 0;
-  int4 m{};
-  vch vv[NCH];  // values of entries a0 + CH tid + CH BLK j + e, a0 = block start rounded down to 8
-  uint32_t uo[UPER];
-  vu4 ov;
   double acc = 0;
+  if (lo >= hi) return acc;
+  const int tid = threadIdx.x;
+  // descriptor of block bk (clamped to the workgroup's last block): geometry
+  // {r0, r1, k0, cnt} and window {b0, b1, s0 | nu << 16, w0}
+  auto dsc = [&](int64_t bk, int4 &g, int4 &w) {
+    const int64_t b = bk < hi ? bk : hi - 1;
+    g = desc[2 * b];
+    w = desc[2 * b + 1];
+  };
   // branch-free loads: out-of-range slots re-read a valid address and are dropped
-  auto fetch = [&](int64_t bk) {
-    m = meta[bk];
-    r0 = rb[bk];
-    r1 = rb[bk + 1];
-    k0 = rp[r0];
-    cnt = rp[r1] - k0;
-    rpn = rp[r0 + (tid <= r1 - r0 ? tid : 0)];
-    const int32_t a0 = k0 & ~7, span = (k0 - a0) + cnt, nu = m.z >> 16;
-    ov = __builtin_nontemporal_load((const vu4 *)(widx + a0 + 8 * (8 * tid < span ? tid : 0)));
-#pragma unroll
-    for (int j = 0; j < NCH; ++j) {
-      const int32_t c = CH * tid + CH * BLK * j;  // chunks starting past the block re-read chunk 0
-      vv[j] = __builtin_nontemporal_load((const vch *)(v + a0 + (c < span ? c : 0)));
-    }
+  auto load_woff = [&](const int4 &W, uint32_t(&uo)[UPER]) {
+    const int32_t nu = W.z >> 16;
 #pragma unroll
     for (int j = 0; j < UPER; ++j) {
       const int32_t u = tid + j * BLK;
-      uo[j] = __builtin_nontemporal_load(&woff[m.w + (u < nu ? u : 0)]);
+      uo[j] = __builtin_nontemporal_load(&woff[W.w + (u < nu ? u : 0)]);
     }
   };
-  // One row block.  The loads of the next block are issued (unconditionally
-  // in the main loop, so that the compiler's wait counts leave them in
-  // flight) after this block's window gathers and row values, and are waited
-  // for only at the top of the next block.
-  int buf = 0;
-  auto body = [&](int64_t bk, auto more) {
-    int32_t *rows = rowbuf[buf];
-    T *xr = xrow[buf];
-    buf ^= NB - 1;
-    const int32_t R0 = r0, R1 = r1, K0 = k0, CNT = cnt, RPN = rpn, lead = K0 & 7;
-    const int4 M = m;
-    const int32_t NU = M.z >> 16, S0 = M.z & 0xffff;
-    const bool staged = R1 - R0 < BLK;
-    T xu[UPER];
+  auto gather = [&](const int4 &G, const int4 &W, const uint32_t(&uo)[UPER], T(&xu)[UPER], T &xo) {
+    const int32_t NU = W.z >> 16, S0 = W.z & 0xffff;
 #pragma unroll
     for (int j = 0; j < UPER; ++j) {
       const int32_t u = tid + j * BLK, o = (int32_t)uo[j];
-      xu[j] = x[u < NU ? o + (o <= S0 ? M.x : M.y) : 0];
+      xu[j] = x[u < NU ? o + (o <= S0 ? W.x : W.y) : 0];
     }
-    T xo = 0;
-    if (DOT) xo = x[R0 + (tid < R1 - R0 ? tid : 0)];
-    colbuf[tid] = ov;
-    if (NB == 2 && staged && tid <= R1 - R0) rows[tid] = RPN - K0;
-    vch cv[NCH];
+    if (DOT) xo = x[G.x + (tid < G.y - G.x ? tid : 0)];
+  };
+  auto fetch = [&](const int4 &G, vch(&vv)[NCH], ich(&iv)[NCH], int32_t &rpn) {
+    const int32_t r0 = G.x, r1 = G.y, k0 = G.z, cnt = G.w;
+    rpn = rp[r0 + (tid <= r1 - r0 ? tid : 0)];
+    const int32_t a0 = k0 & ~7, span = (k0 - a0) + cnt;
 #pragma unroll
-    for (int j = 0; j < NCH; ++j) cv[j] = vv[j];
-    if constexpr (decltype(more)::value) fetch(bk + 1);
+    for (int j = 0; j < NCH; ++j) {
+      const int32_t c = CH * tid + CH * BLK * j;  // chunks starting past the block re-read chunk 0
+      const int32_t o = c < span ? c : 0;
+      vv[j] = __builtin_nontemporal_load((const vch *)(v + a0 + o));
+      iv[j] = __builtin_nontemporal_load((const ich *)(widx + a0 + o));
+    }
+  };
+  // current block (c), the next (n), the one after (nn)
+  int4 Gc, Wc, Gn, Wn, Gnn, Wnn;
+  vch vvc[NCH], vvn[NCH];
+  ich ivc[NCH], ivn[NCH];
+  int32_t rpnc = 0, rpnn = 0;
+  T xuc[UPER], xun[UPER], xoc = 0, xon = 0;
+  uint32_t uon[UPER], uonn[UPER];
+  dsc(lo, Gc, Wc);
+  dsc(lo + 1, Gn, Wn);
+  dsc(lo + 2, Gnn, Wnn);
+  {
+    uint32_t uoc[UPER];
+    load_woff(Wc, uoc);
+    gather(Gc, Wc, uoc, xuc, xoc);
+  }
+  fetch(Gc, vvc, ivc, rpnc);
+  load_woff(Wn, uon);
+  int buf = 0;
+  // One block from registers: window, indices and row data to LDS, products,
+  // rows reduced in a fixed order.
+  auto compute = [&]() {
+    int32_t *rows = rowbuf[buf];
+    T *xr = xrow[buf];
+    buf ^= NB - 1;
+    const int32_t R0 = Gc.x, R1 = Gc.y, K0 = Gc.z, CNT = Gc.w, RPN = rpnc, lead = K0 & 7, A0 = K0 - lead;
+    const int32_t NU = Wc.z >> 16;
+    const bool staged = R1 - R0 < BLK;
+    if (NB == 2 && staged && tid <= R1 - R0) rows[tid] = RPN - A0;
 #pragma unroll
     for (int j = 0; j < UPER; ++j)
-      if (tid + j * BLK < NU) xw[tid + j * BLK] = xu[j];
-    if (NB == 2 && DOT) xr[tid] = xo;
-    __syncthreads();  // window, indices and row data visible; the previous block's rows are reduced
-    if (NB == 1 && staged && tid <= R1 - R0) rows[tid] = RPN - K0;
-    if (NB == 1 && DOT) xr[tid] = xo;
-    T xs[PER];
+      if (tid + j * BLK < NU) xw[tid + j * BLK] = xuc[j];
+    if (NB == 2 && DOT) xr[tid] = xoc;
+    __syncthreads();  // window and row data visible; the previous block's rows are reduced
+    if (NB == 1 && staged && tid <= R1 - R0) rows[tid] = RPN - A0;
+    if (NB == 1 && DOT) xr[tid] = xoc;
 #pragma unroll
-    for (int j = 0; j < NCH; ++j)
-#pragma unroll
-      for (int e = 0; e < CH; ++e) {
-        const int32_t pos = CH * tid + CH * BLK * j + e, t = pos - lead;
-        const int32_t wi = cb[pos];
-        xs[CH * j + e] = xw[t >= 0 && t < CNT ? wi : 0];
-      }
-#pragma unroll
-    for (int j = 0; j < NCH; ++j)
+    for (int j = 0; j < NCH; ++j) {
+      vch xs;
 #pragma unroll
       for (int e = 0; e < CH; ++e) {
-        const int32_t pos = CH * tid + CH * BLK * j + e, t = pos - lead;
-        prod[t >= 0 && t < CNT ? t : RB_CAP] = cv[j][e] * xs[CH * j + e];
+        const int32_t t = CH * tid + CH * BLK * j + e - lead;
+        uint32_t word;
+        if constexpr (CH == 4)
+          word = ivc[j][e >> 1];
+        else
+          word = ivc[j];
+        const int32_t wi = (int32_t)((word >> (16 * (e & 1))) & 0xffffu);
+        xs[e] = xw[t >= 0 && t < CNT ? wi : 0];
       }
+      *(vch *)&prod[CH * tid + CH * BLK * j] = vvc[j] * xs;
+    }
     __syncthreads();
     const int g = tid >> 3, sub = tid & 7;
     if (staged) {
@@ -352,7 +373,7 @@ __device__ inline double rowblock_spmv16(const int32_t *__restrict__ rb, int64_t
       }
     } else {
       for (int32_t r = R0 + g; r < R1; r += BLK / 8) {
-        const int32_t a = rp[r] - K0, e = rp[r + 1] - K0;
+        const int32_t a = rp[r] - A0, e = rp[r + 1] - A0;
         T s = 0;
         for (int32_t t = a + sub; t < e; t += 8) s += prod[t];
         s += __shfl_xor(s, 4, 8);
@@ -365,11 +386,36 @@ __device__ inline double rowblock_spmv16(const int32_t *__restrict__ rb, int64_t
       }
     }
   };
-  if (lo < hi) {
-    fetch(lo);
-    for (int64_t bk = lo; bk + 1 < hi; ++bk) body(bk, std::true_type{});
-    body(hi - 1, std::false_type{});
+  for (int64_t bk = lo; bk + 1 < hi; ++bk) {
+    // the next block's x window and row x (its offsets arrived with this
+    // block's loads), then its values and indices, the offsets of the block
+    // after it and the descriptor three ahead
+    gather(Gn, Wn, uon, xun, xon);
+    fetch(Gn, vvn, ivn, rpnn);
+    load_woff(Wnn, uonn);
+    int4 G3, W3;
+    dsc(bk + 3, G3, W3);
+    compute();
+    Gc = Gn;
+    Wc = Wn;
+#pragma unroll
+    for (int j = 0; j < NCH; ++j) {
+      vvc[j] = vvn[j];
+      ivc[j] = ivn[j];
+    }
+    rpnc = rpnn;
+#pragma unroll
+    for (int j = 0; j < UPER; ++j) {
+      xuc[j] = xun[j];
+      uon[j] = uonn[j];
+    }
+    xoc = xon;
+    Gn = Gnn;
+    Wn = Wnn;
+    Gnn = G3;
+    Wnn = W3;
   }
+  compute();
   return acc;
 }
 
@@ -508,12 +554,12 @@ __global__ __launch_bounds__(BLK) void k_it_spmv(const int32_t *__restrict__ rb,
 // 64 VGPRs (8 waves per SIMD): the pipelined loop needs every resident wave.
 __global__ __launch_bounds__(BLK, 8) void k_it_spmv16(const int32_t *__restrict__ rb, int64_t nrb,
                                                    const int32_t *__restrict__ rp, const uint16_t *__restrict__ widx,
-                                                   const uint16_t *__restrict__ woff, const int4 *__restrict__ meta,
+                                                   const uint16_t *__restrict__ woff, const int4 *__restrict__ desc,
                                                    const float *__restrict__ v, const float *__restrict__ p,
                                                    float *__restrict__ q, double *__restrict__ part,
                                                    const double *__restrict__ S, int bank) {
   if (S[bank * B_SIZE + B_DONE] != 0.0) return;
-  double acc[1] = {rowblock_spmv16<float, true>(rb, nrb, rp, widx, woff, meta, v, p, q)};
+  double acc[1] = {rowblock_spmv16<float, true>(nrb, desc, rp, widx, woff, v, p, q)};
   publish<1>(acc, part);
 }
 
@@ -521,21 +567,21 @@ __global__ __launch_bounds__(BLK, 8) void k_it_spmv16(const int32_t *__restrict_
 // Bit-identical to k_it_spmv<double>.
 __global__ __launch_bounds__(BLK) void k_it_spmv16d(const int32_t *__restrict__ rb, int64_t nrb,
                                                     const int32_t *__restrict__ rp, const uint16_t *__restrict__ widx,
-                                                    const uint16_t *__restrict__ woff, const int4 *__restrict__ meta,
+                                                    const uint16_t *__restrict__ woff, const int4 *__restrict__ desc,
                                                     const double *__restrict__ v, const double *__restrict__ p,
                                                     double *__restrict__ q, double *__restrict__ part,
                                                     const double *__restrict__ S, int bank) {
   if (S[bank * B_SIZE + B_DONE] != 0.0) return;
-  double acc[1] = {rowblock_spmv16<double, true>(rb, nrb, rp, widx, woff, meta, v, p, q)};
+  double acc[1] = {rowblock_spmv16<double, true>(nrb, desc, rp, widx, woff, v, p, q)};
   publish<1>(acc, part);
 }
 
 __global__ __launch_bounds__(BLK, 8) void k_spmv16(const int32_t *__restrict__ rb, int64_t nrb,
                                                 const int32_t *__restrict__ rp, const uint16_t *__restrict__ widx,
-                                                const uint16_t *__restrict__ woff, const int4 *__restrict__ meta,
+                                                const uint16_t *__restrict__ woff, const int4 *__restrict__ desc,
                                                 const float *__restrict__ v, const float *__restrict__ x,
                                                 float *__restrict__ y) {
-  rowblock_spmv16<float, false>(rb, nrb, rp, widx, woff, meta, v, x, y);
+  rowblock_spmv16<float, false>(nrb, desc, rp, widx, woff, v, x, y);
 }
 
 // Build the x windows, one row block per workgroup: sort the block's
@@ -544,7 +590,7 @@ __global__ __launch_bounds__(BLK, 8) void k_spmv16(const int32_t *__restrict__ r
 // s1, which fit 16-bit offsets when s0 + s1 < 65535 (or the whole block spans
 // < 65536).  COUNT: store each block's window size in wsz.  Otherwise wsz holds
 // their exclusive prefix sums (the window starts): write the window offsets,
-// each entry's window index and the block's meta.  A block that does not fit
+// each entry's window index and the block's descriptor (rows, entries, meta).  A block that does not fit
 // (more than UCAP distinct columns, too wide a span, too many entries) raises
 // *bad.
 template <bool COUNT>
@@ -644,7 +690,10 @@ __global__ __launch_bounds__(BLK) void k_build_windows(const int32_t *__restrict
           widx[k0 + (int32_t)(keys[i] & 4095)] = (uint16_t)u;
         }
       }
-      if (tid == 0) meta[bk] = make_int4(b0, b1, s0 | (nu << 16), w0);
+      if (tid == 0) {
+        meta[2 * bk] = make_int4(rb[bk], rb[bk + 1], k0, cnt);
+        meta[2 * bk + 1] = make_int4(b0, b1, s0 | (nu << 16), w0);
+      }
     }
     __syncthreads();
   }
@@ -941,7 +990,7 @@ static int ensure_windows(lmh_matrix *A) {
   LMH_HIP(hipMemcpyAsync(wsz.p, w0.data(), sizeof(int32_t) * (nrb + 1), hipMemcpyHostToDevice, ctx->stream));
   LMH_HIP(hipMalloc((void **)&A->widx16, sizeof(uint16_t) * (A->nnz + LMH_PAD)));
   LMH_HIP(hipMalloc((void **)&A->woff16, sizeof(uint16_t) * (total + LMH_PAD)));
-  LMH_HIP(hipMalloc((void **)&A->rbmeta, sizeof(int4) * (nrb ? nrb : 1)));
+  LMH_HIP(hipMalloc((void **)&A->rbmeta, sizeof(int4) * 2 * (nrb ? nrb : 1)));
   LMH_LAUNCH(ctx, "build_windows", k_build_windows<false>, grid, dim3(BLK), 0, A->rb, nrb, A->rp, A->ci, wsz.p,
              A->widx16, A->woff16, A->rbmeta, ctx->d_flags);
   LMH_TRY(check_launch("k_build_windows"));
