@@ -10,6 +10,7 @@
 
 #include <memory>
 
+#include "libmesh/device_physics.h"
 #include "libmesh/hip_assembler.h"
 #include "libmesh/system.h"
 
@@ -76,12 +77,26 @@ class SteadySolver {
 class FEMSystem : public ImplicitSystem {
  public:
   using ImplicitSystem::ImplicitSystem;
-  // DifferentiableSystem::attach_physics: the device physics
-  void attach_physics(HipAssembler *physics) { _physics = physics; }
+  ~FEMSystem() override;
+  // DifferentiableSystem::attach_physics (diff_system.h:200): the built-in
+  // device Poisson physics, or a user physics whose element_time_derivative
+  // is device code (DevicePhysicsFor<P>, device_physics_hip.h)
+  void attach_physics(HipAssembler *physics) {
+    _physics = physics;
+    _device_physics = nullptr;
+  }
+  void attach_physics(DevicePhysics *physics) {
+    _device_physics = physics;
+    _physics = nullptr;
+  }
   // DifferentiableSystem::init_data builds the default time solver
   void init() override;
   // FEMSystem::assembly (fem_system.C:875-1010): residual at the current
-  // solution into rhs, Jacobian into the system matrix, constrained as with
+  // solution into rhs, Jacobian into the system matrix.  A DevicePhysics runs
+  // element_time_derivative on every element (quadrature: the variable's
+  // default QGauss rule plus extra_quadrature_order, fem_context.C:155-175),
+  // the element constraint of the Dirichlet rows, and the element-order sum
+  // into the HipCsrMatrix and rhs.  The built-in physics is constrained as with
   // constrain_in_solver (element constraint; the Newton solver enforces the
   // constraints exactly).  With general constraint rows the residual is
   // C^T (K u - F) at the enforced solution and the Jacobian C^T K C in the
@@ -96,7 +111,12 @@ class FEMSystem : public ImplicitSystem {
   HipAssembler *device_assembler() const override { return _physics; }
 
  private:
+  void device_physics_assembly(bool get_residual, bool get_jacobian);
   HipAssembler *_physics = nullptr;
+  DevicePhysics *_device_physics = nullptr;
+  // element results and coefficients of the device physics (kept between assemblies)
+  double *_Fe_d = nullptr, *_Ke_d = nullptr, *_coef_d = nullptr;
+  size_t _Fe_n = 0, _Ke_n = 0, _coef_n = 0;
 };
 
 }  // namespace libMesh

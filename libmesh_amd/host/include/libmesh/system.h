@@ -128,6 +128,8 @@ class System {
 
   std::unique_ptr<NumericVector<Number>> solution;
   NumericVector<Number> *rhs = nullptr;
+  // quadrature order added to the FE type's default rule (system.h:1640)
+  int extra_quadrature_order = 0;
 
   // System::add_vector / get_vector / have_vector / n_vectors
   // (src/systems/system.C:751-800, 893-960): named vectors sized like the

@@ -8,6 +8,8 @@
 #include <cstdio>
 
 #include "libmesh/hip_csr_matrix.h"
+#include "libmesh/hip_device.h"
+#include "libmesh/quadrature_gauss.h"
 
 namespace libMesh {
 
@@ -16,9 +18,88 @@ void FEMSystem::init() {
   if (!time_solver) time_solver = std::make_unique<SteadySolver>(*this);
 }
 
+FEMSystem::~FEMSystem() {
+  lmh_context *ctx = HipDevice::context();
+  for (double *p : {_Fe_d, _Ke_d, _coef_d})
+    if (p) lmh_free(ctx, p);
+}
+
+// (re)allocates a device buffer of at least n doubles
+static double *device_buffer(double *&p, size_t &have, size_t n, const char *what) {
+  if (p && have >= n) return p;
+  lmh_context *ctx = HipDevice::context();
+  if (p) lmh_free(ctx, p);
+  p = nullptr;
+  have = 0;
+  void *q = nullptr;
+  hip_check(lmh_malloc(ctx, sizeof(double) * (n ? n : 1), &q), what);
+  p = (double *)q;
+  have = n;
+  return p;
+}
+
+// FEMSystem::assembly for a DevicePhysics: the element loop of
+// fem_system.C:875-1010 (FEMContext::pre_fe_reinit / elem_fe_reinit,
+// element_time_derivative, the element constraint, add_matrix / add_vector)
+// with all elements on the device, then their sums in element order.
+void FEMSystem::device_physics_assembly(bool get_residual, bool get_jacobian) {
+  auto *A = dynamic_cast<HipCsrMatrix<Number> *>(&get_system_matrix());
+  if (!A || !A->device_matrix() || !A->device_mesh() || !A->device_dofmap())
+    libmesh_error_msg("FEMSystem::assembly (device physics): the system matrix must be a HipCsrMatrix "
+                      "initialised from the DofMap");
+  const MeshBase *mesh = get_dof_map().mesh();
+  if (!mesh) libmesh_error_msg("FEMSystem::assembly (device physics): DoFs not distributed");
+  // FEMContext::use_default_quadrature_rules (fem_context.C:155-175)
+  const FEType fe_type = get_dof_map().variable_type(0);
+  const QGauss qrule(3, static_cast<Order>(fe_type.default_quadrature_order() + extra_quadrature_order));
+  lmh_fem_data d;
+  hip_check(lmh_fem_element_data(A->device_mesh(), A->device_dofmap(), qrule.kernel_arg(mesh->elem_type()), &d),
+            "FEMSystem::assembly (element data)");
+  const size_t ne = (size_t)d.n_elem, nd = (size_t)d.n_dofs;
+  double *Fe = device_buffer(_Fe_d, _Fe_n, ne * nd, "FEMSystem::assembly (element residuals)");
+  double *Ke = get_jacobian ? device_buffer(_Ke_d, _Ke_n, ne * nd * nd, "FEMSystem::assembly (element Jacobians)")
+                            : nullptr;
+  const std::vector<Real> &coef = _device_physics->coefficient();
+  const double *coef_d = nullptr;
+  if (!coef.empty()) {
+    if (coef.size() != ne) libmesh_error_msg("DevicePhysics::coefficient needs one value per element");
+    coef_d = device_buffer(_coef_d, _coef_n, ne, "FEMSystem::assembly (coefficients)");
+    hip_check(lmh_memcpy_h2d(HipDevice::context(), (void *)coef_d, coef.data(), sizeof(double) * ne),
+              "FEMSystem::assembly (coefficients)");
+  }
+  solution->close();
+  auto *u = dynamic_cast<HipVector<Number> *>(solution.get());
+  std::unique_ptr<HipVector<Number>> u_copy;
+  if (!u) {  // any other NumericVector: a device copy of its values
+    u_copy = std::make_unique<HipVector<Number>>(solution->comm(), solution->size(), solution->size());
+    std::vector<Number> vals;
+    solution->localize(vals);
+    *u_copy = vals;
+    u_copy->close();
+    u = u_copy.get();
+  }
+  hip_check(_device_physics->element_loop(d, u->device_data(), coef_d, get_jacobian, Fe, Ke),
+            "FEMSystem::assembly (element_time_derivative)");
+  auto *R = dynamic_cast<HipVector<Number> *>(rhs);
+  if (get_residual && !R) libmesh_error_msg("FEMSystem::assembly (device physics): rhs must be a HipVector");
+  double *R_d = get_residual ? (double *)R->device_data() : nullptr;
+  hip_check(lmh_matrix_scatter_elements(A->device_matrix(), Ke, get_residual ? Fe : nullptr, R_d),
+            "FEMSystem::assembly (element sums)");
+  if (get_jacobian) A->mark_device_modified();
+  if (get_residual) R->mark_device_modified();
+}
+
 void FEMSystem::assembly(bool get_residual, bool get_jacobian, bool apply_heterogeneous_constraints,
                          bool apply_no_constraints) {
   if (!get_residual && !get_jacobian) libmesh_error_msg("FEMSystem::assembly: nothing requested");
+  if (_device_physics) {
+    // Dirichlet rows are empty with value 0, so the heterogeneous and the
+    // homogeneous element constraint coincide
+    (void)apply_heterogeneous_constraints;
+    if (apply_no_constraints) libmesh_not_implemented();
+    device_physics_assembly(get_residual, get_jacobian);
+    return;
+  }
   if (!_physics) libmesh_error_msg("FEMSystem::assembly: no physics attached");
   // Dirichlet rows are empty with value 0, so the heterogeneous and the
   // homogeneous element constraint coincide; unconstrained assembly is not

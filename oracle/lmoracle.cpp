@@ -1005,8 +1005,9 @@ void scatter(const System &s, i64 e, const double *K, const double *F, double *v
   const int n = s.dpe;
   const int32_t *d = &s.edofs[(size_t)e * n];
   for (int i = 0; i < n; ++i) {
-    for (int j = 0; j < n; ++j) vals[slot(s, d[i], d[j])] += K[i * n + j];
-    rhs[d[i]] += F[i];  // NumericVector::add_vector (numeric_vector.C:690-697)
+    if (vals)
+      for (int j = 0; j < n; ++j) vals[slot(s, d[i], d[j])] += K[i * n + j];
+    if (rhs) rhs[d[i]] += F[i];  // NumericVector::add_vector (numeric_vector.C:690-697)
   }
 }
 
@@ -1256,6 +1257,90 @@ int orc_residual_poisson(const orc_system *os, int npts, const double *kappa, in
       for (int j = 0; j < n; ++j) r += K[(size_t)i * n + j] * u[d[j]];
       R[d[i]] += r - F[i];
     }
+  }
+  return 0;
+}
+
+// A user physics through FEMSystem::assembly (fem_system.C:875-1010): the
+// steady reaction-diffusion -div(kappa grad u) + sigma u^3 = f written as
+// FEMSystem physics write element_time_derivative (L-shaped.C:58-108: F and K
+// accumulate JxW-weighted terms at each quadrature point, u and grad u from
+// FEMContext::interior_value / interior_gradient, fem_context.C:314-380, 411-490):
+//   F_e(i)   = sum_qp JxW (kappa_e grad u . grad phi_i + sigma u^3 phi_i - f phi_i)
+//   K_e(i,j) = sum_qp JxW (kappa_e grad phi_j . grad phi_i + 3 sigma u^2 phi_j phi_i)
+// then the element constraint of the empty Dirichlet rows (constrain above).
+// `variant` plants a mistake for the tests that show the pins catch one:
+// 1 = Jacobian reaction factor 2 instead of 3, 2 = residual reaction u^2.
+static bool element_reaction_diffusion(const System &s, FEValues &fe, i64 e, const double *kappa, double sigma,
+                                       int f_kind, const double *u, int variant, double *K, double *F) {
+  if (!fe.reinit(*s.m, e)) return false;
+  const int n = fe.n;
+  const double kap = kappa ? kappa[e] : 1.0;
+  const int32_t *d = &s.edofs[(size_t)e * n];
+  if (K) std::fill(K, K + n * n, 0.);
+  std::fill(F, F + n, 0.);
+  for (int qp = 0; qp < fe.nq; ++qp) {
+    double uq = 0, g[3] = {0, 0, 0};
+    for (int a = 0; a < n; ++a) {
+      const double ua = u ? u[d[a]] : 0.0;
+      uq += ua * fe.ph(a, qp);
+      const double *dp = fe.dp(a, qp);
+      g[0] += ua * dp[0];
+      g[1] += ua * dp[1];
+      g[2] += ua * dp[2];
+    }
+    const double w = fe.JxW[qp], f = forcing(f_kind, &fe.xyz[3 * qp]);
+    const double react = variant == 2 ? sigma * uq * uq : sigma * uq * uq * uq;
+    const double dreact = (variant == 1 ? 2.0 : 3.0) * sigma * uq * uq;
+    for (int i = 0; i < n; ++i) {
+      const double pi = fe.ph(i, qp);
+      F[i] += w * (kap * dot3(g, fe.dp(i, qp)) + react * pi - f * pi);
+      if (K)
+        for (int j = 0; j < n; ++j)
+          K[i * n + j] += w * (kap * dot3(fe.dp(j, qp), fe.dp(i, qp)) + dreact * fe.ph(j, qp) * pi);
+    }
+  }
+  return true;
+}
+
+// Element results (constrained when constrain_flag is set) of elements
+// e0 .. e0+count-1; Ke may be NULL (residual only).
+int orc_fem_element_reaction_diffusion(const orc_system *os, int npts, const double *kappa, double sigma,
+                                       int f_kind, const double *u, int64_t e0, int64_t count, int constrain_flag,
+                                       int variant, double *Ke, double *Fe) {
+  const System &s = os->s;
+  if (s.nv != 1 || s.general) return -2;
+  FEValues fe;
+  fe.init(s.m->type, npts);
+  const int n = s.dpe;
+  for (i64 k = 0; k < count; ++k) {
+    double *K = Ke ? Ke + (size_t)k * n * n : nullptr, *F = Fe + (size_t)k * n;
+    if (!element_reaction_diffusion(s, fe, e0 + k, kappa, sigma, f_kind, u, variant, K, F)) return -1;
+    if (constrain_flag) {
+      std::vector<double> Kt((size_t)n * n);
+      constrain(s, e0 + k, K ? K : Kt.data(), F);
+    }
+  }
+  return 0;
+}
+
+// FEMSystem::assembly(get_residual = R != NULL, get_jacobian = vals != NULL)
+// for that physics: constrained element results summed in element order into
+// the CSR values and the residual (both zeroed first).
+int orc_fem_reaction_diffusion(const orc_system *os, int npts, const double *kappa, double sigma, int f_kind,
+                               const double *u, int variant, double *vals, double *R) {
+  const System &s = os->s;
+  if (s.nv != 1 || s.general) return -2;
+  if (vals) std::fill(vals, vals + s.rp.back(), 0.);
+  if (R) std::fill(R, R + s.ndofs, 0.);
+  FEValues fe;
+  fe.init(s.m->type, npts);
+  const int n = s.dpe;
+  std::vector<double> K((size_t)n * n), F(n);
+  for (i64 e = 0; e < s.m->n_elem; ++e) {
+    if (!element_reaction_diffusion(s, fe, e, kappa, sigma, f_kind, u, variant, K.data(), F.data())) return -1;
+    constrain(s, e, K.data(), F.data());
+    scatter(s, e, K.data(), F.data(), vals, R);
   }
   return 0;
 }

@@ -96,6 +96,15 @@ int orc_assemble_poisson(const orc_system *s, int npts, const double *kappa, int
  * constrain != 0 applies the element constraint step. */
 int orc_element_poisson(const orc_system *s, int npts, const double *kappa, int f_kind,
                         int64_t e0, int64_t count, int constrain, double *Ke, double *Fe);
+/* FEMSystem::assembly of the user physics -div(kappa grad u) + sigma u^3 = f at
+ * the solution u (one variable): constrained element residual Fe and Jacobian Ke
+ * (Ke may be NULL), or their element-order sums into vals / R (either may be
+ * NULL).  variant 0; 1 and 2 plant mistakes for the pin tests. */
+int orc_fem_element_reaction_diffusion(const orc_system *s, int npts, const double *kappa, double sigma,
+                                       int f_kind, const double *u, int64_t e0, int64_t count, int constrain,
+                                       int variant, double *Ke, double *Fe);
+int orc_fem_reaction_diffusion(const orc_system *s, int npts, const double *kappa, double sigma, int f_kind,
+                               const double *u, int variant, double *vals, double *R);
 /* Linear elasticity (systems_of_equations_ex6.C), body force (0,0,fz). */
 int orc_assemble_elasticity(const orc_system *s, int npts, double E, double nu, double fz,
                             double *vals, double *rhs);

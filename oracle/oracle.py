@@ -68,6 +68,9 @@ def lib():
             "orc_assemble_poisson": (C.c_int, [_P, C.c_int, _P, C.c_int, _P, _P]),
             "orc_element_poisson": (C.c_int, [_P, C.c_int, _P, C.c_int, _I64, _I64, C.c_int, _P, _P]),
             "orc_assemble_elasticity": (C.c_int, [_P, C.c_int, _D, _D, _D, _P, _P]),
+            "orc_fem_element_reaction_diffusion": (C.c_int, [_P, C.c_int, _P, _D, C.c_int, _P, _I64, _I64, C.c_int,
+                                                             C.c_int, _P, _P]),
+            "orc_fem_reaction_diffusion": (C.c_int, [_P, C.c_int, _P, _D, C.c_int, _P, C.c_int, _P, _P]),
             "orc_element_elasticity": (C.c_int, [_P, C.c_int, _D, _D, _D, _I64, _I64, C.c_int, _P, _P]),
             "orc_cg_eigen": (None, [_I64, _P, _P, _P, _P, _P, _D, _I64, _P, _P]),
             "orc_cg_laspack": (None, [_I64, _P, _P, _P, _P, _P, _D, _I64, _P, _P]),
@@ -271,6 +274,35 @@ class System:
                                        int(constrain), _ptr(Ke), _ptr(Fe))
         if rc != 0:
             raise RuntimeError("degenerate map")
+        return Ke, Fe
+
+    def fem_reaction_diffusion(self, npts, u, sigma, kappa=None, f_kind=0, jacobian=True, residual=True,
+                               variant=0):
+        """FEMSystem::assembly of -div(kappa grad u) + sigma u^3 = f at u: (CSR
+        values of the constrained Jacobian or None, residual or None)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        vals = np.empty(self.nnz) if jacobian else None
+        R = np.empty(self.n_dofs) if residual else None
+        k = None if kappa is None else np.ascontiguousarray(kappa, dtype=np.float64)
+        rc = lib().orc_fem_reaction_diffusion(self.h, npts, None if k is None else _ptr(k), sigma, f_kind, _ptr(u),
+                                              variant, None if vals is None else _ptr(vals),
+                                              None if R is None else _ptr(R))
+        if rc != 0:
+            raise RuntimeError("degenerate map" if rc == -1 else "one variable, Dirichlet rows only")
+        return vals, R
+
+    def element_reaction_diffusion(self, npts, u, sigma, kappa=None, f_kind=0, e0=0, count=None, constrain=True,
+                                   variant=0):
+        count = self.mesh.n_elem - e0 if count is None else count
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        Ke = np.empty((count, self.dpe, self.dpe))
+        Fe = np.empty((count, self.dpe))
+        k = None if kappa is None else np.ascontiguousarray(kappa, dtype=np.float64)
+        rc = lib().orc_fem_element_reaction_diffusion(self.h, npts, None if k is None else _ptr(k), sigma, f_kind,
+                                                      _ptr(u), e0, count, int(constrain), variant, _ptr(Ke),
+                                                      _ptr(Fe))
+        if rc != 0:
+            raise RuntimeError("degenerate map" if rc == -1 else "one variable, Dirichlet rows only")
         return Ke, Fe
 
     def assemble_elasticity(self, npts, E=1.0, nu=0.3, fz=-1.0):
