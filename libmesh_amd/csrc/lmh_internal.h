@@ -174,16 +174,22 @@ struct lmh_matrix {
   int tpos_state = 0;
   int32_t *tpos = nullptr;
   double *vals_t = nullptr;
-  // FP32 operator column compression (x windows): each row block lists its
-  // distinct columns once, sorted, as 16-bit offsets woff16[w0 .. w0+nu),
-  // col = off + (off <= s0 ? b0 : base1); every entry stores the 16-bit index
-  // of its column in that list.  rbmeta[2bk] = {r0, r1, k0, cnt} (the block's rows and
-  // entries), rbmeta[2bk+1] = {b0, base1, s0 | nu << 16, w0}.
+  // Column compression of the SpMV operators (x windows): each row block of
+  // the window partition rbw lists its distinct columns once, sorted, as
+  // 16-bit offsets woff16[w0 .. w0+nu), col = off + (off <= s0 ? b0 : base1);
+  // every entry stores the index of its column in that list, 8-bit (blocks of
+  // at most 1024 entries and 256 distinct columns) or else 16-bit (the
+  // 2048-entry SpMV blocks, up to 768 columns).  rbmeta[2bk] = {r0, r1, k0,
+  // cnt} (the block's rows and entries), rbmeta[2bk+1] = {b0, base1, s0 | nu << 16, w0}.
   int fp32_index = 1;          // 1: use the compressed columns when they fit
   int idx16_state = 0;         // 0 not built, 1 built, -1 does not fit
-  uint16_t *widx16 = nullptr;  // nnz: window index of each entry's column
+  int wbits = 0;               // 8 or 16 once built
+  int32_t *rbw = nullptr;      // window partition (nrbw + 1 row boundaries)
+  int64_t nrbw = 0;
+  uint8_t *widx8 = nullptr;    // nnz: 8-bit window index of each entry's column
+  uint16_t *widx16 = nullptr;  // nnz: 16-bit window index of each entry's column
   uint16_t *woff16 = nullptr;  // sum of nu over the row blocks
-  int4 *rbmeta = nullptr;      // 2 nrb
+  int4 *rbmeta = nullptr;      // 2 nrbw
   uint16_t *off = nullptr;  // n_elem*dpe*dpe slot offsets within rows
   int32_t *diag = nullptr;  // rows: position of the diagonal entry, -1 if absent
   int ncolors = 0;
@@ -226,6 +232,8 @@ hipEvent_t get_event(lmh_context *ctx);
 int check_launch(const char *what);
 // per-device address of the allocation-free CG's scratch into ctx (context creation)
 int cg_scratch_setup(lmh_context *ctx);
+// x windows of a new matrix's pattern (csrc/lmh_solve.hip)
+int matrix_build_windows(lmh_matrix *A);
 
 // device helpers shared across translation units
 int exclusive_scan_i32(lmh_context *ctx, const int32_t *in, int32_t *out, int64_t n, int64_t *total);

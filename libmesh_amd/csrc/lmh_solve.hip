@@ -30,8 +30,8 @@ __device__ inline T wave_sum(T v) {
 }
 
 // block-wide sum of up to 3 values (deterministic); result valid in thread 0
-template <int NV>
-__device__ inline void block_sum(double (&v)[NV], double (*sh)[BLK / 64]) {
+template <int NV, int NT = BLK>
+__device__ inline void block_sum(double (&v)[NV], double (*sh)[NT / 64]) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
@@ -43,7 +43,7 @@ __device__ inline void block_sum(double (&v)[NV], double (*sh)[BLK / 64]) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       double s = 0;
-      for (int w = 0; w < BLK / 64; ++w) s += sh[k][w];
+      for (int w = 0; w < NT / 64; ++w) s += sh[k][w];
       v[k] = s;
     }
   }
@@ -53,10 +53,10 @@ __device__ inline void block_sum(double (&v)[NV], double (*sh)[BLK / 64]) {
 // one-block finalize kernel launched next on the same stream reduces them in
 // block order (deterministic); the kernel boundary makes the partials visible
 // without any device-scope fence inside the producer.
-template <int NV>
+template <int NV, int NT = BLK>
 __device__ inline void publish(double (&v)[NV], double *__restrict__ part) {
-  __shared__ double sh[NV][BLK / 64];
-  block_sum<NV>(v, sh);
+  __shared__ double sh[NV][NT / 64];
+  block_sum<NV, NT>(v, sh);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) part[k * gridDim.x + blockIdx.x] = v[k];
@@ -235,34 +235,48 @@ __device__ inline void rowblock_spmv(const int32_t *__restrict__ rb, int64_t nrb
 // FP64 results are bit-identical to the int32-column kernel too.
 typedef uint32_t vu4 __attribute__((ext_vector_type(4)));
 constexpr int UCAP = 768;  // distinct columns of one row block (x window)
+// 8-bit windows: blocks of at most W8_CAP - 8 entries and 256 distinct columns
+// (a Q2 hexahedral operator's 1024-entry blocks hold about 200); an entry then
+// costs one index byte instead of two
+constexpr int W8_CAP = 1024, W8_UCAP = 256;
+// 8-bit blocks run on 128-thread workgroups (8 entries per thread, as the
+// 2048-entry blocks on 256 threads), twice as many of them
+constexpr int W8_NT = 128, W8_GRID = 2;
+template <int IW>
+using widx_t = typename std::conditional<IW == 1, uint8_t, uint16_t>::type;
 // Pipelining.  Every block's loads are addressed from registers only, so no
 // load waits on another in flight: a 32-byte descriptor per block (rows,
 // entry range, window meta, built with the windows) is loaded three blocks
 // ahead, the block's window offsets two blocks ahead, and its values, window
 // indices, row pointers and x-window gathers one block ahead, while the
 // current block runs its LDS phases.
-template <typename T, bool DOT>
+// CAPW: the block capacity (RB_CAP with 16-bit indices, W8_CAP with 8-bit);
+// IW: index bytes per entry; NT: threads per workgroup.
+template <typename T, bool DOT, int CAPW = RB_CAP, int IW = 2, int NT = BLK>
 __device__ inline double rowblock_spmv16(int64_t nrb, const int4 *__restrict__ desc, const int32_t *__restrict__ rp,
-                                         const uint16_t *__restrict__ widx, const uint16_t *__restrict__ woff,
+                                         const widx_t<IW> *__restrict__ widx, const uint16_t *__restrict__ woff,
                                          const T *__restrict__ v, const T *__restrict__ x, T *__restrict__ y) {
   constexpr int CH = 16 / sizeof(T);  // entries per 16-byte value chunk
   typedef T vch __attribute__((ext_vector_type(CH)));
-  // the chunk's CH window indices (8 bytes for FP32, 4 for FP64)
+  // the chunk's CH window indices (IW * CH bytes: 8 for FP32 16-bit, 4 for
+  // FP64 16-bit and FP32 8-bit, 2 for FP64 8-bit)
   typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-  typedef typename std::conditional<CH == 4, u32x2, uint32_t>::type ich;
+  typedef typename std::conditional<
+      IW * CH == 8, u32x2, typename std::conditional<IW * CH == 4, uint32_t, uint16_t>::type>::type ich;
+  constexpr int UC = IW == 1 ? W8_UCAP : UCAP;
   // products at their chunk positions from the block's start rounded down to 8
   // entries (slots outside the block hold unread values)
-  __shared__ __attribute__((aligned(16))) T prod[RB_CAP];
-  __shared__ T xw[UCAP];
+  __shared__ __attribute__((aligned(16))) T prod[CAPW];
+  __shared__ T xw[UC];
   // Row offsets and the x of the block's rows (for the (x, Ax) partials).
   // FP32: double-buffered, written before the first barrier.  FP64: one
   // buffer, written after it (the previous block's rows are reduced by then),
   // which keeps the LDS under 32 KB for 5 resident workgroups per CU.
   constexpr int NB = sizeof(T) == 8 ? 1 : 2;
-  __shared__ int32_t rowbuf[NB][BLK];
-  __shared__ T xrow[NB][BLK];
-  constexpr int PER = RB_CAP / BLK, UPER = UCAP / BLK, NCH = PER / CH;
-  static_assert(PER == 8 && UCAP % BLK == 0, "one 8-entry index chunk per thread");
+  __shared__ int32_t rowbuf[NB][NT];
+  __shared__ T xrow[NB][NT];
+  constexpr int PER = CAPW / NT, UPER = UC / NT, NCH = PER / CH;
+  static_assert(PER % CH == 0 && UC % NT == 0, "whole value chunks and window slots per thread");
   int64_t lo, hi;
   block_range(nrb, lo, hi);
   double acc = 0;
@@ -280,7 +294,7 @@ __device__ inline double rowblock_spmv16(int64_t nrb, const int4 *__restrict__ d
     const int32_t nu = W.z >> 16;
 #pragma unroll
     for (int j = 0; j < UPER; ++j) {
-      const int32_t u = tid + j * BLK;
+      const int32_t u = tid + j * NT;
       uo[j] = __builtin_nontemporal_load(&woff[W.w + (u < nu ? u : 0)]);
     }
   };
@@ -288,7 +302,7 @@ __device__ inline double rowblock_spmv16(int64_t nrb, const int4 *__restrict__ d
     const int32_t NU = W.z >> 16, S0 = W.z & 0xffff;
 #pragma unroll
     for (int j = 0; j < UPER; ++j) {
-      const int32_t u = tid + j * BLK, o = (int32_t)uo[j];
+      const int32_t u = tid + j * NT, o = (int32_t)uo[j];
       xu[j] = x[u < NU ? o + (o <= S0 ? W.x : W.y) : 0];
     }
     if (DOT) xo = x[G.x + (tid < G.y - G.x ? tid : 0)];
@@ -299,7 +313,7 @@ __device__ inline double rowblock_spmv16(int64_t nrb, const int4 *__restrict__ d
     const int32_t a0 = k0 & ~7, span = (k0 - a0) + cnt;
 #pragma unroll
     for (int j = 0; j < NCH; ++j) {
-      const int32_t c = CH * tid + CH * BLK * j;  // chunks starting past the block re-read chunk 0
+      const int32_t c = CH * tid + CH * NT * j;  // chunks starting past the block re-read chunk 0
       const int32_t o = c < span ? c : 0;
       vv[j] = __builtin_nontemporal_load((const vch *)(v + a0 + o));
       iv[j] = __builtin_nontemporal_load((const ich *)(widx + a0 + o));
@@ -331,11 +345,11 @@ __device__ inline double rowblock_spmv16(int64_t nrb, const int4 *__restrict__ d
     buf ^= NB - 1;
     const int32_t R0 = Gc.x, R1 = Gc.y, K0 = Gc.z, CNT = Gc.w, RPN = rpnc, lead = K0 & 7, A0 = K0 - lead;
     const int32_t NU = Wc.z >> 16;
-    const bool staged = R1 - R0 < BLK;
+    const bool staged = R1 - R0 < NT;
     if (NB == 2 && staged && tid <= R1 - R0) rows[tid] = RPN - A0;
 #pragma unroll
     for (int j = 0; j < UPER; ++j)
-      if (tid + j * BLK < NU) xw[tid + j * BLK] = xuc[j];
+      if (tid + j * NT < NU) xw[tid + j * NT] = xuc[j];
     if (NB == 2 && DOT) xr[tid] = xoc;
     __syncthreads();  // window and row data visible; the previous block's rows are reduced
     if (NB == 1 && staged && tid <= R1 - R0) rows[tid] = RPN - A0;
@@ -345,21 +359,25 @@ __device__ inline double rowblock_spmv16(int64_t nrb, const int4 *__restrict__ d
       vch xs;
 #pragma unroll
       for (int e = 0; e < CH; ++e) {
-        const int32_t t = CH * tid + CH * BLK * j + e - lead;
+        const int32_t t = CH * tid + CH * NT * j + e - lead;
         uint32_t word;
-        if constexpr (CH == 4)
+        int sh;
+        if constexpr (IW * CH == 8) {
           word = ivc[j][e >> 1];
-        else
-          word = ivc[j];
-        const int32_t wi = (int32_t)((word >> (16 * (e & 1))) & 0xffffu);
+          sh = 16 * (e & 1);
+        } else {
+          word = (uint32_t)ivc[j];
+          sh = 8 * IW * e;
+        }
+        const int32_t wi = (int32_t)((word >> sh) & (IW == 1 ? 0xffu : 0xffffu));
         xs[e] = xw[t >= 0 && t < CNT ? wi : 0];
       }
-      *(vch *)&prod[CH * tid + CH * BLK * j] = vvc[j] * xs;
+      *(vch *)&prod[CH * tid + CH * NT * j] = vvc[j] * xs;
     }
     __syncthreads();
     const int g = tid >> 3, sub = tid & 7;
     if (staged) {
-      for (int32_t r = R0 + g; r < R1; r += BLK / 8) {
+      for (int32_t r = R0 + g; r < R1; r += NT / 8) {
         const int32_t a = rows[r - R0], e = rows[r - R0 + 1];
         T s = 0;
         for (int32_t t = a + sub; t < e; t += 8) s += prod[t];
@@ -372,7 +390,7 @@ __device__ inline double rowblock_spmv16(int64_t nrb, const int4 *__restrict__ d
         }
       }
     } else {
-      for (int32_t r = R0 + g; r < R1; r += BLK / 8) {
+      for (int32_t r = R0 + g; r < R1; r += NT / 8) {
         const int32_t a = rp[r] - A0, e = rp[r + 1] - A0;
         T s = 0;
         for (int32_t t = a + sub; t < e; t += 8) s += prod[t];
@@ -550,38 +568,45 @@ __global__ __launch_bounds__(BLK) void k_it_spmv(const int32_t *__restrict__ rb,
   publish<1>(acc, part);
 }
 
-// q = A32 p on the x-window FP32 operator; partials (p, q).  Held to
-// 64 VGPRs (8 waves per SIMD): the pipelined loop needs every resident wave.
-__global__ __launch_bounds__(BLK, 8) void k_it_spmv16(const int32_t *__restrict__ rb, int64_t nrb,
-                                                   const int32_t *__restrict__ rp, const uint16_t *__restrict__ widx,
-                                                   const uint16_t *__restrict__ woff, const int4 *__restrict__ desc,
-                                                   const float *__restrict__ v, const float *__restrict__ p,
-                                                   float *__restrict__ q, double *__restrict__ part,
-                                                   const double *__restrict__ S, int bank) {
+// q = A32 p on the x-window FP32 operator (IW-byte window indices on blocks
+// of CAPW entries); partials (p, q).  Held to 64 VGPRs (8 waves per SIMD): the
+// pipelined loop needs every resident wave.
+template <int CAPW, int IW, int NT>
+__global__ __launch_bounds__(NT, 2048 / NT) void k_it_spmv16(int64_t nrb, const int32_t *__restrict__ rp,
+                                                          const void *__restrict__ widx,
+                                                          const uint16_t *__restrict__ woff,
+                                                          const int4 *__restrict__ desc, const float *__restrict__ v,
+                                                          const float *__restrict__ p, float *__restrict__ q,
+                                                          double *__restrict__ part, const double *__restrict__ S,
+                                                          int bank) {
   if (S[bank * B_SIZE + B_DONE] != 0.0) return;
-  double acc[1] = {rowblock_spmv16<float, true>(nrb, desc, rp, widx, woff, v, p, q)};
-  publish<1>(acc, part);
+  double acc[1] = {
+      rowblock_spmv16<float, true, CAPW, IW, NT>(nrb, desc, rp, (const widx_t<IW> *)widx, woff, v, p, q)};
+  publish<1, NT>(acc, part);
 }
 
 // q = A p on the FP64 operator through the same x windows; partials (p, q).
 // Bit-identical to k_it_spmv<double>.
-__global__ __launch_bounds__(BLK) void k_it_spmv16d(const int32_t *__restrict__ rb, int64_t nrb,
-                                                    const int32_t *__restrict__ rp, const uint16_t *__restrict__ widx,
-                                                    const uint16_t *__restrict__ woff, const int4 *__restrict__ desc,
-                                                    const double *__restrict__ v, const double *__restrict__ p,
-                                                    double *__restrict__ q, double *__restrict__ part,
-                                                    const double *__restrict__ S, int bank) {
+template <int CAPW, int IW, int NT>
+__global__ __launch_bounds__(NT) void k_it_spmv16d(int64_t nrb, const int32_t *__restrict__ rp,
+                                                   const void *__restrict__ widx, const uint16_t *__restrict__ woff,
+                                                   const int4 *__restrict__ desc, const double *__restrict__ v,
+                                                   const double *__restrict__ p, double *__restrict__ q,
+                                                   double *__restrict__ part, const double *__restrict__ S,
+                                                   int bank) {
   if (S[bank * B_SIZE + B_DONE] != 0.0) return;
-  double acc[1] = {rowblock_spmv16<double, true>(nrb, desc, rp, widx, woff, v, p, q)};
-  publish<1>(acc, part);
+  double acc[1] = {
+      rowblock_spmv16<double, true, CAPW, IW, NT>(nrb, desc, rp, (const widx_t<IW> *)widx, woff, v, p, q)};
+  publish<1, NT>(acc, part);
 }
 
-__global__ __launch_bounds__(BLK, 8) void k_spmv16(const int32_t *__restrict__ rb, int64_t nrb,
-                                                const int32_t *__restrict__ rp, const uint16_t *__restrict__ widx,
-                                                const uint16_t *__restrict__ woff, const int4 *__restrict__ desc,
-                                                const float *__restrict__ v, const float *__restrict__ x,
-                                                float *__restrict__ y) {
-  rowblock_spmv16<float, false>(nrb, desc, rp, widx, woff, v, x, y);
+template <int CAPW, int IW, int NT>
+__global__ __launch_bounds__(NT, 2048 / NT) void k_spmv16(int64_t nrb, const int32_t *__restrict__ rp,
+                                                       const void *__restrict__ widx,
+                                                       const uint16_t *__restrict__ woff,
+                                                       const int4 *__restrict__ desc, const float *__restrict__ v,
+                                                       const float *__restrict__ x, float *__restrict__ y) {
+  rowblock_spmv16<float, false, CAPW, IW, NT>(nrb, desc, rp, (const widx_t<IW> *)widx, woff, v, x, y);
 }
 
 // Build the x windows, one row block per workgroup: sort the block's
@@ -592,28 +617,33 @@ __global__ __launch_bounds__(BLK, 8) void k_spmv16(const int32_t *__restrict__ r
 // their exclusive prefix sums (the window starts): write the window offsets,
 // each entry's window index and the block's descriptor (rows, entries, meta).  A block that does not fit
 // (more than UCAP distinct columns, too wide a span, too many entries) raises
-// *bad.
-template <bool COUNT>
+// *bad.  CAP: block capacity; IW: index bytes (the window limit is UCAP for
+// 16-bit and W8_UCAP for 8-bit indices).  The counting pass reports every
+// block's window size without judging it, so the host can split blocks whose
+// window is too large.
+template <bool COUNT, int CAP, int IW>
 __global__ __launch_bounds__(BLK) void k_build_windows(const int32_t *__restrict__ rb, int64_t nrb,
                                                        const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                                                       int32_t *__restrict__ wsz, uint16_t *__restrict__ widx,
+                                                       int32_t *__restrict__ wsz, widx_t<IW> *__restrict__ widx,
                                                        uint16_t *__restrict__ woff, int4 *__restrict__ meta,
                                                        int *__restrict__ bad) {
-  __shared__ int64_t keys[RB_CAP];
+  __shared__ int64_t keys[CAP];
   __shared__ int32_t gv[BLK], gi[BLK];
-  constexpr int PER = RB_CAP / BLK;
+  constexpr int PER = CAP / BLK;
+  constexpr int UC = IW == 1 ? W8_UCAP : UCAP;
+  static_assert(CAP <= 4096, "positions are kept in 12 key bits");
   const int tid = threadIdx.x;
   for (int64_t bk = blockIdx.x; bk < nrb; bk += gridDim.x) {
     const int32_t k0 = rp[rb[bk]], cnt = rp[rb[bk + 1]] - k0;
-    if ((k0 & 7) + cnt > RB_CAP) {  // does not fit one 8-entry index chunk per thread
+    if ((k0 & 7) + cnt > CAP) {  // the block from its start rounded down to 8 entries exceeds CAP
       if (tid == 0) atomicOr(bad, 1);
       continue;
     }
-    for (int t = tid; t < RB_CAP; t += BLK) keys[t] = t < cnt ? ((int64_t)ci[k0 + t] << 12) | t : INT64_MAX;
+    for (int t = tid; t < CAP; t += BLK) keys[t] = t < cnt ? ((int64_t)ci[k0 + t] << 12) | t : INT64_MAX;
     __syncthreads();
-    for (int k = 2; k <= RB_CAP; k <<= 1)
+    for (int k = 2; k <= CAP; k <<= 1)
       for (int j = k >> 1; j > 0; j >>= 1) {
-        for (int i = tid; i < RB_CAP; i += BLK) {
+        for (int i = tid; i < CAP; i += BLK) {
           const int pj = i ^ j;
           if (pj > i) {
             const int64_t a = keys[i], c = keys[pj];
@@ -671,7 +701,7 @@ __global__ __launch_bounds__(BLK) void k_build_windows(const int32_t *__restrict
       __syncthreads();
     }
     const int32_t nu = gv[BLK - 1], before = gv[tid] - nfl;
-    ok = ok && nu <= UCAP;
+    if (!COUNT) ok = ok && nu <= UC;
     if (!ok) {
       if (tid == 0) atomicOr(bad, 1);
     } else if (COUNT) {
@@ -687,7 +717,7 @@ __global__ __launch_bounds__(BLK) void k_build_windows(const int32_t *__restrict
             ++u;
             woff[w0 + u] = (uint16_t)(c <= split ? c - b0 : c - b1);
           }
-          widx[k0 + (int32_t)(keys[i] & 4095)] = (uint16_t)u;
+          widx[k0 + (int32_t)(keys[i] & 4095)] = (widx_t<IW>)u;
         }
       }
       if (tid == 0) {
@@ -955,30 +985,56 @@ static int spmv_grid(lmh_context *ctx) {
   return g < cap ? g : cap;
 }
 
-// x windows of the FP32 operator, built once per pattern: window sizes, their
-// prefix sums on the host, then offsets and indices
-static int ensure_windows(lmh_matrix *A) {
-  if (A->idx16_state != 0 || !A->fp32_index) return LMH_OK;
+// x windows of the SpMV operators, built once per pattern on the row
+// partition rb (host copy): window sizes (counting pass), blocks whose window
+// exceeds the index width split by rows when `split` (at most 8 rounds; a
+// single row over the limit does not fit), prefix sums on the host, then
+// offsets, indices and descriptors.  *fits = false leaves A unchanged.
+template <int CAP, int IW>
+static int try_windows(lmh_matrix *A, std::vector<int32_t> rb, bool split, bool *fits) {
+  constexpr int UC = IW == 1 ? W8_UCAP : UCAP;
   lmh_context *ctx = A->ctx;
-  const int64_t nrb = A->nrb;
-  const dim3 grid(grid_for(nrb, 1, 16384));
+  *fits = false;
   struct Tmp {
-    int32_t *p = nullptr;
+    void *p = nullptr;
     ~Tmp() { (void)hipFree(p); }
-  } wsz;
-  LMH_HIP(hipMalloc((void **)&wsz.p, sizeof(int32_t) * (nrb + 1)));
-  LMH_HIP(hipMemsetAsync(ctx->d_flags, 0, sizeof(int), ctx->stream));
-  LMH_LAUNCH(ctx, "build_windows", k_build_windows<true>, grid, dim3(BLK), 0, A->rb, nrb, A->rp, A->ci, wsz.p,
-             nullptr, nullptr, nullptr, ctx->d_flags);
-  LMH_TRY(check_launch("k_build_windows"));
-  std::vector<int32_t> w0(nrb + 1);
-  int bad = 0;
-  LMH_HIP(hipMemcpyAsync(w0.data(), wsz.p, sizeof(int32_t) * nrb, hipMemcpyDeviceToHost, ctx->stream));
-  LMH_HIP(hipMemcpyAsync(&bad, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-  LMH_HIP(hipStreamSynchronize(ctx->stream));
-  if (bad) {
-    A->idx16_state = -1;
-    return LMH_OK;
+  } wsz, rbd;
+  std::vector<int32_t> w0;
+  int64_t nrb = 0;
+  for (int round = 0;; ++round) {
+    nrb = (int64_t)rb.size() - 1;
+    (void)hipFree(rbd.p);
+    (void)hipFree(wsz.p);
+    rbd.p = wsz.p = nullptr;
+    LMH_HIP(hipMalloc(&rbd.p, sizeof(int32_t) * rb.size()));
+    LMH_HIP(hipMalloc(&wsz.p, sizeof(int32_t) * (nrb + 1)));
+    LMH_HIP(hipMemcpyAsync(rbd.p, rb.data(), sizeof(int32_t) * rb.size(), hipMemcpyHostToDevice, ctx->stream));
+    LMH_HIP(hipMemsetAsync(ctx->d_flags, 0, sizeof(int), ctx->stream));
+    LMH_LAUNCH(ctx, "build_windows", (k_build_windows<true, CAP, IW>), dim3(grid_for(nrb, 1, 16384)), dim3(BLK), 0,
+               (const int32_t *)rbd.p, nrb, A->rp, A->ci, (int32_t *)wsz.p, nullptr, nullptr, nullptr,
+               ctx->d_flags);
+    LMH_TRY(check_launch("k_build_windows"));
+    w0.assign(nrb + 1, 0);
+    int bad = 0;
+    LMH_HIP(hipMemcpyAsync(w0.data(), wsz.p, sizeof(int32_t) * nrb, hipMemcpyDeviceToHost, ctx->stream));
+    LMH_HIP(hipMemcpyAsync(&bad, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    LMH_HIP(hipStreamSynchronize(ctx->stream));
+    if (bad) return LMH_OK;
+    bool over = false;
+    for (int64_t b = 0; b < nrb; ++b) over |= w0[b] > UC;
+    if (!over) break;
+    if (!split || round == 8) return LMH_OK;
+    std::vector<int32_t> nb{0};
+    for (int64_t b = 0; b < nrb; ++b) {
+      const int32_t r0 = rb[b], r1 = rb[b + 1];
+      if (w0[b] > UC) {
+        if (r1 - r0 < 2) return LMH_OK;  // one row alone exceeds the window
+        const int32_t parts = std::min<int32_t>(r1 - r0, (w0[b] + (UC * 7 / 8) - 1) / (UC * 7 / 8));
+        for (int32_t k = 1; k < parts; ++k) nb.push_back(r0 + (int32_t)((int64_t)(r1 - r0) * k / parts));
+      }
+      nb.push_back(r1);
+    }
+    rb.swap(nb);
   }
   int64_t total = 0;
   for (int64_t b = 0; b < nrb; ++b) {
@@ -988,24 +1044,66 @@ static int ensure_windows(lmh_matrix *A) {
   }
   w0[nrb] = (int32_t)total;  // at most nnz < 2^31
   LMH_HIP(hipMemcpyAsync(wsz.p, w0.data(), sizeof(int32_t) * (nrb + 1), hipMemcpyHostToDevice, ctx->stream));
-  LMH_HIP(hipMalloc((void **)&A->widx16, sizeof(uint16_t) * (A->nnz + LMH_PAD)));
-  LMH_HIP(hipMalloc((void **)&A->woff16, sizeof(uint16_t) * (total + LMH_PAD)));
-  LMH_HIP(hipMalloc((void **)&A->rbmeta, sizeof(int4) * 2 * (nrb ? nrb : 1)));
-  LMH_LAUNCH(ctx, "build_windows", k_build_windows<false>, grid, dim3(BLK), 0, A->rb, nrb, A->rp, A->ci, wsz.p,
-             A->widx16, A->woff16, A->rbmeta, ctx->d_flags);
+  widx_t<IW> *widx = nullptr;
+  uint16_t *woff = nullptr;
+  int4 *meta = nullptr;
+  LMH_HIP(hipMalloc((void **)&widx, sizeof(widx_t<IW>) * (A->nnz + LMH_PAD)));
+  LMH_HIP(hipMalloc((void **)&woff, sizeof(uint16_t) * (total + LMH_PAD)));
+  LMH_HIP(hipMalloc((void **)&meta, sizeof(int4) * 2 * (nrb ? nrb : 1)));
+  LMH_HIP(hipMemsetAsync(ctx->d_flags, 0, sizeof(int), ctx->stream));
+  LMH_LAUNCH(ctx, "build_windows", (k_build_windows<false, CAP, IW>), dim3(grid_for(nrb, 1, 16384)), dim3(BLK), 0,
+             (const int32_t *)rbd.p, nrb, A->rp, A->ci, (int32_t *)wsz.p, widx, woff, meta, ctx->d_flags);
   LMH_TRY(check_launch("k_build_windows"));
+  int bad = 0;
   LMH_HIP(hipMemcpyAsync(&bad, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   LMH_HIP(hipStreamSynchronize(ctx->stream));
-  A->idx16_state = bad ? -1 : 1;  // the second pass sees the blocks the first one accepted
-  if (bad) {
-    LMH_HIP(hipFree(A->widx16));
-    LMH_HIP(hipFree(A->woff16));
-    LMH_HIP(hipFree(A->rbmeta));
-    A->widx16 = A->woff16 = nullptr;
-    A->rbmeta = nullptr;
+  if (bad) {  // not expected: the counting pass accepted every block
+    LMH_HIP(hipFree(widx));
+    LMH_HIP(hipFree(woff));
+    LMH_HIP(hipFree(meta));
+    return LMH_OK;
   }
+  if (IW == 1)
+    A->widx8 = (uint8_t *)widx;
+  else
+    A->widx16 = (uint16_t *)widx;
+  A->woff16 = woff;
+  A->rbmeta = meta;
+  A->rbw = (int32_t *)rbd.p;
+  rbd.p = nullptr;
+  A->nrbw = nrb;
+  A->wbits = 8 * IW;
+  *fits = true;
   return LMH_OK;
 }
+
+// 8-bit windows on blocks of up to W8_CAP - 8 entries (split further where a
+// window exceeds 256 columns), else 16-bit windows on the SpMV row blocks,
+// else none (the int32-column kernels).
+static int ensure_windows(lmh_matrix *A) {
+  if (A->idx16_state != 0 || !A->fp32_index) return LMH_OK;
+  lmh_context *ctx = A->ctx;
+  std::vector<int32_t> rp(A->rows + 1);
+  LMH_HIP(hipMemcpyAsync(rp.data(), A->rp, sizeof(int32_t) * (A->rows + 1), hipMemcpyDeviceToHost, ctx->stream));
+  LMH_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<int32_t> rb{0};
+  for (int64_t r = 0; r < A->rows; ++r)
+    if (r > rb.back() && rp[r + 1] - rp[rb.back()] > W8_CAP - 8) rb.push_back((int32_t)r);
+  rb.push_back((int32_t)A->rows);
+  bool fits = false;
+  LMH_TRY((try_windows<W8_CAP, 1>(A, rb, true, &fits)));
+  if (!fits) {
+    std::vector<int32_t> rb16(A->nrb + 1);
+    LMH_HIP(hipMemcpy(rb16.data(), A->rb, sizeof(int32_t) * (A->nrb + 1), hipMemcpyDeviceToHost));
+    LMH_TRY((try_windows<RB_CAP, 2>(A, rb16, false, &fits)));
+  }
+  A->idx16_state = fits ? 1 : -1;
+  return LMH_OK;
+}
+
+// at matrix creation: every solver entry point then finds the windows built,
+// the allocation-free one included
+int matrix_build_windows(lmh_matrix *A) { return ensure_windows(A); }
 
 static int ensure_diag(lmh_matrix *A) {
   if (A->diag) return LMH_OK;
@@ -1046,8 +1144,12 @@ int lmh_spmv_fp32(lmh_matrix *A, const float *x_d, float *y_d) {
   const int g = spmv_grid(ctx);
   LMH_TRY(ensure_windows(A));
   if (use_windows(A)) {
-    LMH_LAUNCH(ctx, "spmv_fp32", k_spmv16, dim3(g), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->widx16, A->woff16,
-               A->rbmeta, A->vals32, x_d, y_d);
+    if (A->wbits == 8)
+      LMH_LAUNCH(ctx, "spmv_fp32", (k_spmv16<W8_CAP, 1, W8_NT>), dim3(W8_GRID * g), dim3(W8_NT), 0, A->nrbw,
+                 A->rp, A->widx8, A->woff16, A->rbmeta, A->vals32, x_d, y_d);
+    else
+      LMH_LAUNCH(ctx, "spmv_fp32", (k_spmv16<RB_CAP, 2, BLK>), dim3(g), dim3(BLK), 0, A->nrbw, A->rp, A->widx16,
+                 A->woff16, A->rbmeta, A->vals32, x_d, y_d);
     return check_launch("k_spmv16");
   }
   LMH_LAUNCH(ctx, "spmv_fp32", (k_spmv<float, float, float, float>), dim3(g),
@@ -1325,13 +1427,14 @@ int cg_scratch_setup(lmh_context *ctx) {
 template <bool NA>
 static int cg_impl(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int64_t maxits, double *r, double *p,
                    double *q, double *dinv, const int32_t *diag, int64_t *iters, double *rel_res, int *reason) {
-#define CG_LAUNCH(name, kernel, grid, ...)                                                   \
+#define CG_LAUNCH_NT(name, kernel, grid, nt, ...)                                            \
   do {                                                                                      \
     if (NA)                                                                                 \
-      hipLaunchKernelGGL(kernel, grid, dim3(BLK), 0, ctx->stream, __VA_ARGS__);             \
+      hipLaunchKernelGGL(kernel, grid, dim3(nt), 0, ctx->stream, __VA_ARGS__);              \
     else                                                                                    \
-      LMH_LAUNCH(ctx, name, kernel, grid, dim3(BLK), 0, __VA_ARGS__);                       \
+      LMH_LAUNCH(ctx, name, kernel, grid, dim3(nt), 0, __VA_ARGS__);                        \
   } while (0)
+#define CG_LAUNCH(name, kernel, grid, ...) CG_LAUNCH_NT(name, kernel, grid, BLK, __VA_ARGS__)
   lmh_context *ctx = A->ctx;
   if (!NA) LMH_TRY(ensure_windows(A));
   const bool c16 = use_windows(A);
@@ -1339,6 +1442,7 @@ static int cg_impl(lmh_matrix *A, const double *b_d, double *x_d, double rtol, i
   double *S = ctx->d_scal;
   double *part1 = ctx->d_part, *part2 = ctx->d_part + 2 * ctx->n_part;  // 1 and 2 values per block
   const int g = vec_grid(ctx), gs = spmv_grid(ctx);
+  const int gsp = c16 && A->wbits == 8 ? W8_GRID * gs : gs;  // the iteration SpMV's grid
   CG_LAUNCH("invdiag", (k_invdiag<double, double>), dim3(grid_for(n, BLK, 8192)), n, diag, A->vals,
             A->precond == LMH_PRECOND_JACOBI, dinv);
   CG_LAUNCH("residual_fp64", k_residual, dim3(gs), A->rb, A->nrb, A->rp, A->ci, A->vals, b_d, x_d, r, part2);
@@ -1357,13 +1461,16 @@ static int cg_impl(lmh_matrix *A, const double *b_d, double *x_d, double rtol, i
     const int64_t nb = std::min<int64_t>(batch, maxits - K);
     for (int64_t i = 0; i < nb; ++i, ++K) {
       const int bank = (int)(K & 1);
-      if (c16)
-        CG_LAUNCH("cg_spmv_fp64", k_it_spmv16d, dim3(gs), A->rb, A->nrb, A->rp, A->widx16, A->woff16, A->rbmeta,
-                  A->vals, p, q, part1, S, bank);
+      if (c16 && A->wbits == 8)
+        CG_LAUNCH_NT("cg_spmv_fp64", (k_it_spmv16d<W8_CAP, 1, W8_NT>), dim3(gsp), W8_NT, A->nrbw, A->rp, A->widx8,
+                     A->woff16, A->rbmeta, A->vals, p, q, part1, S, bank);
+      else if (c16)
+        CG_LAUNCH("cg_spmv_fp64", (k_it_spmv16d<RB_CAP, 2, BLK>), dim3(gs), A->nrbw, A->rp, A->widx16, A->woff16,
+                  A->rbmeta, A->vals, p, q, part1, S, bank);
       else
         CG_LAUNCH("cg_spmv_fp64", k_it_spmv<double>, dim3(gs), A->rb, A->nrb, A->rp, A->ci, A->vals, p, q, part1, S,
                   bank);
-      CG_LAUNCH("cg_update_fp64", (k_it_update<double, double>), dim3(g), n, part1, gs, dinv, p, q, x_d, r, part2, S,
+      CG_LAUNCH("cg_update_fp64", (k_it_update<double, double>), dim3(g), n, part1, gsp, dinv, p, q, x_d, r, part2, S,
                 bank);
       CG_LAUNCH("cg_pupdate_fp64", k_it_pupdate<double>, dim3(g), n, part2, g, dinv, r, p, S, bank, 0);
     }
@@ -1591,6 +1698,7 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
   double *S = ctx->d_scal;
   double *part1 = ctx->d_part, *part2 = ctx->d_part + 2 * ctx->n_part;
   const int g = vec_grid(ctx), gs = spmv_grid(ctx);
+  const int gsp = c16 && A->wbits == 8 ? W8_GRID * gs : gs;  // the iteration SpMV's grid
   double init[40] = {0};
   init[G_THR] = rtol * rtol;
   init[G_DELTA2] = inner_rtol * inner_rtol;
@@ -1621,13 +1729,16 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
     const int64_t nb = std::min<int64_t>(batch, maxits - K);
     for (int64_t i = 0; i < nb; ++i, ++K) {
       const int bank = (int)(K & 1);
-      if (c16)
-        LMH_LAUNCH(ctx, "mx_spmv_fp32", k_it_spmv16, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->widx16,
-                   A->woff16, A->rbmeta, A->vals32, p32, q32, part1, S, bank);
+      if (c16 && A->wbits == 8)
+        LMH_LAUNCH(ctx, "mx_spmv_fp32", (k_it_spmv16<W8_CAP, 1, W8_NT>), dim3(gsp), dim3(W8_NT), 0, A->nrbw, A->rp,
+                   A->widx8, A->woff16, A->rbmeta, A->vals32, p32, q32, part1, S, bank);
+      else if (c16)
+        LMH_LAUNCH(ctx, "mx_spmv_fp32", (k_it_spmv16<RB_CAP, 2, BLK>), dim3(gs), dim3(BLK), 0, A->nrbw, A->rp,
+                   A->widx16, A->woff16, A->rbmeta, A->vals32, p32, q32, part1, S, bank);
       else
         LMH_LAUNCH(ctx, "mx_spmv_fp32", k_it_spmv<float>, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci,
                    A->vals32, p32, q32, part1, S, bank);
-      LMH_LAUNCH(ctx, "mx_update_fp32", (k_it_update<float, float>), dim3(g), dim3(BLK), 0, n, part1, gs, d32, p32,
+      LMH_LAUNCH(ctx, "mx_update_fp32", (k_it_update<float, float>), dim3(g), dim3(BLK), 0, n, part1, gsp, d32, p32,
                  q32, xc, r32, part2, S, bank);
       LMH_LAUNCH(ctx, "mx_pupdate_fp32", k_it_pupdate<float>, dim3(g), dim3(BLK), 0, n, part2, g, d32, r32, p32, S,
                  bank, 1);

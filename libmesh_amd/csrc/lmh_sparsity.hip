@@ -293,6 +293,7 @@ int matrix_create_impl(lmh_context *ctx, const lmh_dofmap *dm, bool colour, bool
     LMH_HIP(hipMemcpy(A->color_elems, celems.data(), sizeof(int32_t) * ne, hipMemcpyHostToDevice));
   }
   LMH_HIP(hipStreamSynchronize(ctx->stream));
+  LMH_TRY(matrix_build_windows(A.get()));
   *out = A.release();
   return LMH_OK;
 }
@@ -337,6 +338,7 @@ int lmh_matrix_create_csr2(lmh_context *ctx, int64_t rows, int64_t cols, const i
   if (nnz) LMH_HIP(hipMemcpy(A->ci, col_idx_h, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
   LMH_HIP(hipMemset(A->vals, 0, sizeof(double) * (nnz + LMH_PAD)));
   LMH_TRY(build_row_blocks(A.get(), rp));
+  LMH_TRY(matrix_build_windows(A.get()));
   *out = A.release();
   return LMH_OK;
 }
@@ -353,6 +355,8 @@ int lmh_matrix_destroy(lmh_matrix *A) {
   (void)hipFree(A->ci);
   (void)hipFree(A->rb);
   (void)hipFree(A->widx16);
+  (void)hipFree(A->widx8);
+  (void)hipFree(A->rbw);
   (void)hipFree(A->woff16);
   (void)hipFree(A->rbmeta);
   (void)hipFree(A->vals);

@@ -434,11 +434,13 @@ def test_identity_preconditioner(ctx):
 
 
 @pytest.mark.parametrize("t,n", [(L.HEX27, 8), (L.TET10, 3), (L.HEX8, 12), (L.HEX27, 64)])
-def test_fp64_cg_windows_bit_identical(ctx, t, n):
-    """The FP64 CG SpMV through the x windows (16-bit window indices, the
-    operator's FP64 values) forms the same products and sums them in the same
-    order as the int32-column kernel: the solve is bit-identical, iterates and
-    iteration count, with the windows on and off."""
+def test_fp64_cg_windows_match(ctx, t, n):
+    """The FP64 CG SpMV through the x windows (8- or 16-bit window indices, the
+    operator's FP64 values) forms the same products and sums every row in the
+    same order as the int32-column kernel; only the (p, Ap) partials group rows
+    by the window partition (1024-entry blocks for 8-bit indices), so the
+    solves agree to rounding: the same iteration count within one, and
+    iterates (both at relative residual 1e-10) within 1e-10 of each other."""
     p = L.Problem(n, t)
     p.perturb(SEED_PERTURB, 0.2 if t in (L.HEX8, L.HEX27) else 0.1)
     kappa = L.lognormal(p.n_elem, SEED_KAPPA)
@@ -455,5 +457,5 @@ def test_fp64_cg_windows_bit_identical(ctx, t, n):
         its, res, reason = g.cg(b, x, 1e-10, 20000)
         assert reason == L.CONVERGED_RTOL
         out.append((its, res, x.download()))
-    assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
-    assert np.array_equal(out[0][2], out[1][2])
+    assert abs(out[0][0] - out[1][0]) <= 1
+    assert np.linalg.norm(out[0][2] - out[1][2]) <= 1e-10 * np.linalg.norm(out[1][2])

@@ -55,7 +55,7 @@ def test_noalloc_cg_matches_cg_twice(ctx):
     assert abs(its - oits) <= 2
     assert np.linalg.norm(x - ox) / np.linalg.norm(ox) < 1e-8
     g.close()
-    # second call, smaller problem, fresh matrix: no x windows built yet
+    # second call, smaller problem, fresh matrix (its x windows come with it)
     p2, g2, *_ = solve_both(ctx, 5, L.HEX8, 2, 0x5EED0202, True)
     g2.close()
     # and a third on the same smaller size with the windows already built
