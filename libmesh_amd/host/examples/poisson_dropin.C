@@ -13,6 +13,9 @@
 // default asymmetric constraint rows instead, solved by Jacobi-BiCGSTAB
 // (LinearSolver::set_solver_type(BICGSTAB)).  `identity`: IDENTITY_PRECOND
 // instead of JACOBI_PRECOND (LinearSolver::set_preconditioner_type).
+// `condense`: System::create_static_condensation before init, as
+// miscellaneous_ex16 does: the system matrix is a StaticCondensation and the
+// solve runs on the reduced system (its size is printed).
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -25,6 +28,7 @@
 #include "libmesh/hip_csr_matrix.h"
 #include "libmesh/hip_linear_solver.h"
 #include "libmesh/libmesh.h"
+#include "libmesh/static_condensation.h"
 #include "libmesh/system.h"
 #include "box_qoi.h"
 #include "periodic_x.h"
@@ -45,7 +49,7 @@ int main(int argc, char **argv) {
   const ElemType type = (ElemType)std::atoi(argv[2]);
   const std::string prefix = argv[3];
   bool perturb = false, kappa = false, sine = false, mixed = false, adjoint = false, distort = false,
-       periodic = false, bicgstab = false, identity = false;
+       periodic = false, bicgstab = false, identity = false, condense = false;
   for (int a = 4; a < argc; ++a) {
     perturb |= !std::strcmp(argv[a], "perturb");
     kappa |= !std::strcmp(argv[a], "kappa");
@@ -56,6 +60,7 @@ int main(int argc, char **argv) {
     periodic |= !std::strcmp(argv[a], "periodic");
     bicgstab |= !std::strcmp(argv[a], "bicgstab");
     identity |= !std::strcmp(argv[a], "identity");
+    condense |= !std::strcmp(argv[a], "condense");
   }
   LibMeshInit init(argc, argv);
   try {
@@ -82,9 +87,13 @@ int main(int argc, char **argv) {
     } else {
       sys.get_dof_map().add_dirichlet_boundary(DirichletBoundary({0, 1, 2, 3, 4, 5}, {0}));
     }
-    auto matrix = std::make_unique<HipCsrMatrix<Number>>(init.comm());
-    matrix->set_asymmetric_constraint_rows(bicgstab);
-    sys.add_matrix("System Matrix", std::move(matrix));
+    if (condense) {
+      sys.create_static_condensation();  // the system matrix becomes a StaticCondensation
+    } else {
+      auto matrix = std::make_unique<HipCsrMatrix<Number>>(init.comm());
+      matrix->set_asymmetric_constraint_rows(bicgstab);
+      sys.add_matrix("System Matrix", std::move(matrix));
+    }
     auto solver = std::make_unique<HipLinearSolver<Number>>(init.comm());
     solver->set_mixed_precision(mixed);
     if (bicgstab) solver->set_solver_type(BICGSTAB);
@@ -99,9 +108,15 @@ int main(int argc, char **argv) {
     sys.attach_assemble_object(assembler);
     es.init();
     if (kappa) assembler.set_coefficient(MeshTools::lognormal_coefficients(mesh.n_elem(), 0x5EED0002));
+    if (condense && mixed)  // the reduced system has its own solver (the "<system>_condensed_" one)
+      static_cast<HipLinearSolver<Number> &>(sys.get_static_condensation().reduced_system_solver())
+          .set_mixed_precision(true);
 
     sys.solve();
     if (periodic && !bicgstab) sys.get_dof_map().enforce_constraints_exactly(sys);
+    if (condense)
+      std::printf("Static condensation reduced problem size to %u\n",
+                  (unsigned)sys.get_static_condensation().get_condensed_mat().m());
     auto *lin = static_cast<HipLinearSolver<Number> *>(sys.linear_solver.get());
     std::printf("dofs %u iterations %u residual %.6e reason %d\n", sys.get_dof_map().n_dofs(),
                 sys.n_linear_iterations(), sys.final_linear_residual(), (int)lin->get_converged_reason());

@@ -4,12 +4,14 @@
 #pragma once
 
 #include <map>
+#include <memory>
 #include <set>
 #include <tuple>
 #include <vector>
 
 #include "libmesh/libmesh_common.h"
 #include "libmesh/replicated_mesh.h"
+#include "libmesh/static_condensation_dof_map.h"
 
 namespace libMesh {
 
@@ -93,6 +95,21 @@ class DofMap {
   // homogeneous form of the same rows on an adjoint solution
   void enforce_adjoint_constraints_exactly(NumericVector<Number> &v, unsigned int q) const;
 
+  // DofMap::create_static_condensation (dof_map.h:1792-1809): the map of the
+  // DoFs a StaticCondensation eliminates, filled when the DoFs are distributed
+  void create_static_condensation(MeshBase &mesh, System &system) {
+    _sc = std::make_unique<StaticCondensationDofMap>(mesh, system, *this);
+  }
+  bool has_static_condensation() const { return _sc != nullptr; }
+  StaticCondensationDofMap &get_static_condensation() {
+    if (!_sc) libmesh_error_msg("DofMap: no static condensation");
+    return *_sc;
+  }
+  const StaticCondensationDofMap &get_static_condensation() const {
+    if (!_sc) libmesh_error_msg("DofMap: no static condensation");
+    return *_sc;
+  }
+
   // This library's mesh generation, files and C host API read the numbering
   // as flat arrays; the drop-in matrix reads it through the accessors above.
   unsigned dofs_per_elem() const { return _dpe; }
@@ -114,6 +131,7 @@ class DofMap {
   DofConstraints _dof_constraints;
   DofConstraintValueMap _primal_constraint_values;
   bool _error_on_loop = false;
+  std::unique_ptr<StaticCondensationDofMap> _sc;
 
   friend class System;
 };

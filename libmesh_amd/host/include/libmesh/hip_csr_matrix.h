@@ -89,6 +89,11 @@ class HipCsrMatrix : public SparseMatrix<T> {
   void read_matlab(const std::string &filename) override;
   void read_petsc_binary(const std::string &filename) override;
 
+  // Makes this matrix a view of a device matrix owned elsewhere (the reduced
+  // matrix of a StaticCondensation): the pattern is copied to the host, the
+  // values are read from the device on demand, and clear() leaves A alone.
+  void view_device_matrix(lmh_matrix *A);
+
   // device handles for the GPU assembly and solver
   lmh_matrix *device_matrix() const { return _A; }
   lmh_mesh *device_mesh() const { return _mesh; }
@@ -117,6 +122,7 @@ class HipCsrMatrix : public SparseMatrix<T> {
   lmh_mesh *_mesh = nullptr;
   lmh_dofmap *_dm = nullptr;
   lmh_matrix *_A = nullptr;
+  bool _owns_A = true;
   numeric_index_type _rows = 0, _cols = 0;
   int64_t _nnz = 0;
   std::vector<int32_t> _rp, _ci;        // host copy of the pattern

@@ -24,6 +24,7 @@ namespace libMesh {
 
 class EquationSystems;
 class HipAssembler;
+class StaticCondensation;
 
 class System {
  public:
@@ -131,6 +132,11 @@ class System {
   // quadrature order added to the FE type's default rule (system.h:1640)
   int extra_quadrature_order = 0;
 
+  // System::create_static_condensation (system.h:1985): condense the DoFs of
+  // element-internal nodes; call before EquationSystems::init
+  virtual void create_static_condensation();
+  bool has_static_condensation() const { return _dof_map.has_static_condensation(); }
+
   // System::add_vector / get_vector / have_vector / n_vectors
   // (src/systems/system.C:751-800, 893-960): named vectors sized like the
   // solution (created now when the system is initialised, else by init());
@@ -206,6 +212,12 @@ class ImplicitSystem : public System {
   LinearSolver<Number> *get_linear_solver() const { return linear_solver.get(); }
   // ImplicitSystem::linear_solver (public, include/systems/implicit_system.h:328)
   mutable std::unique_ptr<LinearSolver<Number>> linear_solver;
+  // ImplicitSystem::create_static_condensation / get_static_condensation
+  // (implicit_system.h:330-335, implicit_system.C:51-67): the system matrix
+  // becomes a StaticCondensation, which the HipLinearSolver solves through
+  // the reduced system
+  void create_static_condensation() override;
+  StaticCondensation &get_static_condensation();
   bool assemble_before_solve = true;
 
  protected:

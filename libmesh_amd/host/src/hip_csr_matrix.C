@@ -39,7 +39,8 @@ HipCsrMatrix<T>::~HipCsrMatrix() {
 
 template <typename T>
 void HipCsrMatrix<T>::clear() {
-  if (_A) lmh_matrix_destroy(_A);
+  if (_A && _owns_A) lmh_matrix_destroy(_A);
+  _owns_A = true;
   if (_dm) lmh_dofmap_destroy(_dm);
   if (_mesh) lmh_mesh_destroy(_mesh);
   _A = nullptr;
@@ -108,6 +109,22 @@ void HipCsrMatrix<T>::init(ParallelType) {
   hip_check(lmh_matrix_get_pattern(_A, _rp.data(), _ci.data()), "HipCsrMatrix::init (pattern)");
   _vals.assign(_nnz, T(0));
   _vals_host_valid = true;
+  this->_is_initialized = true;
+  _closed = true;
+}
+
+template <typename T>
+void HipCsrMatrix<T>::view_device_matrix(lmh_matrix *A) {
+  clear();
+  _A = A;
+  _owns_A = false;
+  _rows = (numeric_index_type)lmh_matrix_rows(A);
+  _cols = (numeric_index_type)lmh_matrix_cols(A);
+  _nnz = lmh_matrix_nnz(A);
+  _rp.resize(_rows + 1);
+  _ci.resize(_nnz);
+  hip_check(lmh_matrix_get_pattern(_A, _rp.data(), _ci.data()), "HipCsrMatrix::view_device_matrix");
+  _vals_host_valid = false;
   this->_is_initialized = true;
   _closed = true;
 }

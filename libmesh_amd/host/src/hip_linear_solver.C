@@ -5,6 +5,10 @@
 // members that the interface implements itself (src/solvers/linear_solver.C).
 #include "libmesh/hip_linear_solver.h"
 
+#include <type_traits>
+
+#include "libmesh/static_condensation.h"
+
 namespace libMesh {
 
 template <typename T>
@@ -84,6 +88,18 @@ std::pair<unsigned int, Real> HipLinearSolver<T>::run(SparseMatrix<T> &matrix_in
   this->init();
   auto *A = dynamic_cast<HipCsrMatrix<T> *>(&matrix_in);
   if (!A) libmesh_error_msg("HipLinearSolver needs a HipCsrMatrix");
+  if constexpr (std::is_same<T, Number>::value) {
+    // a statically condensed system: forward elimination, the reduced solve
+    // with the condensation's own solver at this tolerance, back substitution
+    // (the StaticCondensationPreconditioner apply of libMesh, used as the whole solve)
+    if (auto *sc = dynamic_cast<StaticCondensation *>(&matrix_in)) {
+      if (transpose && this->_solver_type == BICGSTAB) libmesh_not_implemented();
+      const auto r = sc->condensed_solve(rhs_in, solution_in, tol.value_or(this->default_tolerance()),
+                                         m_its.value_or(this->default_max_iterations()));
+      _reason = sc->reduced_converged_reason();
+      return r;
+    }
+  }
   // close everything, as EigenSparseLinearSolver::solve does (eigen_sparse_linear_solver.C:97-100)
   A->close();
   DeviceOperand<T> x(solution_in), b(rhs_in);

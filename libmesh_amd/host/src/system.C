@@ -2,6 +2,7 @@
 // the assembly + solve path (src/systems/system.C, implicit_system.C,
 // linear_implicit_system.C), the QoI hooks and the adjoint entry points.
 #include "libmesh/system.h"
+#include "libmesh/static_condensation.h"
 
 #include <cmath>
 
@@ -127,6 +128,7 @@ void System::init() {
   _dof_map.distribute_dofs(mesh);
   if (_constrain_obj) _constrain_obj->constrain();
   _dof_map.process_constraints(mesh);
+  if (_dof_map.has_static_condensation()) _dof_map.get_static_condensation().reinit();
   solution = std::make_unique<HipVector<Number>>(_es.comm(), _dof_map.n_dofs());
   _rhs = std::make_unique<HipVector<Number>>(_es.comm(), _dof_map.n_dofs());
   rhs = _rhs.get();
@@ -178,9 +180,31 @@ void System::assemble() {
     libmesh_error_msg("System::assemble: no assembly attached");
 }
 
+// System::create_static_condensation (system.C): the DoF map of the
+// condensation, filled when the DoFs are distributed
+void System::create_static_condensation() { _dof_map.create_static_condensation(_es.get_mesh(), *this); }
+
+// ImplicitSystem::create_static_condensation (implicit_system.C:57-67): the
+// system matrix is the StaticCondensation (built by init_matrices)
+void ImplicitSystem::create_static_condensation() {
+  System::create_static_condensation();
+  if (have_matrix("System Matrix"))
+    libmesh_error_msg("create_static_condensation: the system already has a System Matrix");
+}
+
+StaticCondensation &ImplicitSystem::get_static_condensation() {
+  auto *sc = have_matrix("System Matrix") ? dynamic_cast<StaticCondensation *>(&get_system_matrix()) : nullptr;
+  if (!sc) libmesh_error_msg("ImplicitSystem::get_static_condensation: no static condensation");
+  return *sc;
+}
+
 // ImplicitSystem::add_matrices: the "System Matrix" a caller injected before
-// init() is adopted; otherwise one is built (implicit_system.C:113-128).
+// init() is adopted; otherwise one is built (implicit_system.C:113-128), a
+// StaticCondensation when the system condenses (implicit_system.C:51-60).
 void ImplicitSystem::init_matrices() {
+  if (!have_matrix("System Matrix") && has_static_condensation())
+    add_matrix("System Matrix", std::make_unique<StaticCondensation>(_es.get_mesh(), *this, _dof_map,
+                                                                     _dof_map.get_static_condensation()));
   if (!have_matrix("System Matrix")) add_matrix("System Matrix", std::make_unique<HipCsrMatrix<Number>>(_es.comm()));
   if (!linear_solver) linear_solver = std::make_unique<HipLinearSolver<Number>>(_es.comm());
   System::init_matrices();

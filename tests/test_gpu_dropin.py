@@ -77,6 +77,29 @@ def test_poisson_c1_recipe_dropin_with_adjoint(tmp_path, mixed):
     assert np.abs(grad[elems] - fd).max() / np.abs(fd).max() < 1e-6
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_static_condensation_dropin(tmp_path, mixed):
+    """System::create_static_condensation before init (miscellaneous_ex16.C:
+    155-201): the system matrix is a StaticCondensation, assembly fills the
+    full operator, the solve runs on the reduced system (the HEX27 centre DoFs
+    eliminated) and back-substitutes.  The solution is the oracle's full solve."""
+    n = 5
+    flags = ["perturb", "kappa", "condense"] + (["mixed"] if mixed else [])
+    out = run(["poisson_dropin", n, 12, tmp_path / "sc"] + flags, tmp_path)
+    u = np.fromfile(tmp_path / "sc_u.bin")
+    m = O.Mesh(n, O.HEX27)
+    m.perturb(0x5EED0001, 0.2)
+    kappa = O.kappa_lognormal(m.n_elem, 0x5EED0002)
+    s = O.System(m)
+    v, b = s.assemble_poisson(3, kappa, 0)
+    rp, ci = s.pattern()
+    ut, its_full, _ = O.cg_eigen(rp, ci, v, b, 1e-14, 20000)
+    assert f"reduced problem size to {s.n_dofs - m.n_elem}" in out, out
+    assert np.linalg.norm(u - ut) / np.linalg.norm(ut) < 1e-8, out
+    its = int(out.split("iterations ")[1].split()[0])
+    assert 0 < its < 20000, out
+
+
 @pytest.mark.parametrize("bicgstab", [False, True])
 def test_poisson_periodic_constraints_dropin(tmp_path, bicgstab):
     """System::Constraint rows (u(x=1) = u(x=0) plus a heterogeneous row):

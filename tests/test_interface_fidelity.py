@@ -48,7 +48,7 @@ CHECKS = [
       "get_converged_reason"]),
     ("systems/implicit_system.h", "ImplicitSystem", "system.h", "ImplicitSystem",
      ["adjoint_solve", "adjoint_qoi_parameter_sensitivity", "assemble_residual_derivatives", "assembly",
-      "get_system_matrix", "get_linear_solver"]),
+      "get_system_matrix", "get_linear_solver", "create_static_condensation", "get_static_condensation"]),
     ("systems/system.h", "System", "system.h", "System",
      ["attach_assemble_function", "attach_assemble_object", "attach_constraint_object", "attach_QOI_function",
       "attach_QOI_object", "attach_QOI_derivative", "attach_QOI_derivative_object", "assemble_qoi",
@@ -56,7 +56,7 @@ CHECKS = [
       "add_adjoint_solution", "get_adjoint_solution", "add_adjoint_rhs", "get_adjoint_rhs", "add_sensitivity_rhs",
       "get_sensitivity_rhs", "set_adjoint_already_solved", "is_adjoint_already_solved", "add_vector", "get_vector",
       "have_vector", "n_vectors", "add_matrix", "get_matrix", "have_matrix", "adjoint_solve",
-      "adjoint_qoi_parameter_sensitivity"]),
+      "adjoint_qoi_parameter_sensitivity", "create_static_condensation", "has_static_condensation"]),
     ("systems/qoi_set.h", "QoISet", "qoi_set.h", "QoISet",
      ["QoISet", "clear", "add_index", "remove_index", "set_weight", "weight", "has_index"]),
     ("systems/sensitivity_data.h", "SensitivityData", "sensitivity_data.h", "SensitivityData",
@@ -68,7 +68,14 @@ CHECKS = [
      ["add_dirichlet_boundary", "distribute_dofs", "n_variables", "variable_name", "variable_type",
       "dof_indices", "is_constrained_dof", "n_constrained_dofs", "add_constraint_row", "constraint_rows_begin",
       "constraint_rows_end", "get_primal_constraint_values", "set_error_on_constraint_loop", "process_constraints",
-      "enforce_constraints_exactly", "enforce_adjoint_constraints_exactly"]),
+      "enforce_constraints_exactly", "enforce_adjoint_constraints_exactly", "create_static_condensation",
+      "has_static_condensation", "get_static_condensation"]),
+    ("numerics/static_condensation.h", "StaticCondensation", "static_condensation.h", "StaticCondensation",
+     ["StaticCondensation", "init", "setup", "apply", "get_condensed_mat", "set_current_elem",
+      "reduced_system_solver", "dont_condense_vars", "close", "clear"]),
+    ("numerics/static_condensation_dof_map.h", "StaticCondensationDofMap", "static_condensation_dof_map.h",
+     "StaticCondensationDofMap", ["StaticCondensationDofMap", "reinit", "dont_condense_vars", "uncondensed_vars",
+                                  "mesh", "initialized", "clear", "dof_indices"]),
     ("mesh/mesh_base.h", "MeshBase", "replicated_mesh.h", "ReplicatedMesh",
      ["n_nodes", "n_elem", "point", "elem_ref", "elem_ptr", "mesh_dimension"]),
     ("parallel/parallel_object.h", "ParallelObject", "parallel_object.h", "ParallelObject",
