@@ -51,6 +51,17 @@ def spmv_bytes(nnz, n, val_bytes):
     return (val_bytes + 4) * nnz + 4 * (n + 1) + 2 * val_bytes * n
 
 
+def spmv_symbol(L, g, fp64):
+    """The CG iteration's SpMV kernel for the matrix's column storage (x
+    windows with 8- or 16-bit indices, or int32 columns)."""
+    bits = L.hip().lmh_matrix_window_bits(g.matrix)
+    if bits == 8:
+        return f"lmh::k_it_spmv16{'d' if fp64 else ''}<1024, 1, 128>"
+    if bits == 16:
+        return f"lmh::k_it_spmv16{'d' if fp64 else ''}<2048, 2, 256>"
+    return f"lmh::k_it_spmv<{'double' if fp64 else 'float'}>"
+
+
 def scatter_bytes(nnz, n, n_elem, val_bytes):
     """Compulsory bytes of the assembly scatter: write each CSR value once, read
     col_idx and row_ptr, connectivity, unique coordinates, kappa, write rhs."""
@@ -750,7 +761,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel_symbol": "lmh::k_it_spmv16",
+            "kernel_symbol": spmv_symbol(L, g, fp64=False),
             "algorithmic_bytes": alg,
             "avg_launch_ms": round(mx["spmv_avg_ms"], 5),
             "avg_working_launch_ms": round(working_ms(mx), 5),
@@ -766,7 +777,7 @@ def main():
         "mixed_mfma32_rel_diff_vs_fp64_solution": err_mfma32_vs_fp64,
         "speedup_mixed_vs_fp64": round(mx["mdofs"] / results["fp64"]["mdofs"], 4),
         "mixed_rel_diff_vs_fp64_solution": err_mixed_vs_fp64,
-        "fp64_spmv_roofline": {"kernel_symbol": "lmh::k_it_spmv<double>", "achieved": round(ach64, 1),
+        "fp64_spmv_roofline": {"kernel_symbol": spmv_symbol(L, g, fp64=True), "achieved": round(ach64, 1),
                                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach64 / HBM_PEAK_GBS, 4),
                                "algorithmic_bytes": alg64,
                                "avg_working_launch_ms": round(working_ms(results["fp64"]), 5)},

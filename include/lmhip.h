@@ -330,17 +330,22 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
                     int64_t maxits, int64_t *iters, double *rel_res, int *reason, double *stats_h);
 /* Build the FP32 operator by rounding the FP64 values. */
 int lmh_matrix_make_fp32(lmh_matrix *A);
-/* Column storage of the solvers' SpMV: on (default) stores, per SpMV row
- * block, the block's distinct columns once (16-bit offsets against two bases)
- * and a 16-bit index into that list per entry (6 bytes per entry with the
- * FP32 value instead of 8, 10 with the FP64 value instead of 12; x is
- * gathered once per distinct column) whenever every block fits; off keeps
- * the int32 CSR columns.  Used by the FP32 inner iterations of
- * lmh_mixed_solve and by the iterations of lmh_cg_solve; the results are
- * bit-identical either way.  lmh_matrix_fp32_compression reports 1 when the
- * compressed columns are in use, 0 otherwise (builds them if needed). */
+/* Column storage of the solvers' SpMV: on (default) stores, per row block,
+ * the block's distinct columns once (16-bit offsets against two bases) and
+ * per entry the index of its column in that list: 8-bit on blocks of at most
+ * 1016 entries and 256 distinct columns (5 bytes per entry with the FP32
+ * value instead of 8, 9 with the FP64 value instead of 12), else 16-bit on
+ * the 2048-entry blocks, whenever every block fits; x is gathered once per
+ * distinct column.  Off keeps the int32 CSR columns.  Built with the matrix;
+ * used by the FP32 inner iterations of lmh_mixed_solve and by the iterations
+ * of lmh_cg_solve.  Row sums are bit-identical either way; the CG's (p, Ap)
+ * partials follow the block partition, so solves agree to rounding.
+ * lmh_matrix_fp32_compression reports 1 when the compressed columns are in
+ * use, 0 otherwise; lmh_matrix_window_bits their index width (8 or 16, 0 when
+ * not in use). */
 int lmh_matrix_set_fp32_compression(lmh_matrix *A, int on);
 int lmh_matrix_fp32_compression(lmh_matrix *A);
+int lmh_matrix_window_bits(lmh_matrix *A);
 /* Preconditioner of lmh_cg_solve, lmh_bicgstab_solve and lmh_mixed_solve on
  * this matrix, with libMesh's PreconditionerType values: LMH_PRECOND_JACOBI
  * (default; Eigen DiagonalPreconditioner, 1/a_ii or 1 where the diagonal is

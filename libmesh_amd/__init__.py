@@ -107,6 +107,7 @@ HIP_API = {
     "lmh_matrix_make_fp32": (_I, [_P]),
     "lmh_matrix_set_fp32_compression": (_I, [_P, _I]),
     "lmh_matrix_fp32_compression": (_I, [_P]),
+    "lmh_matrix_window_bits": (_I, [_P]),
     "lmh_matrix_set_preconditioner": (_I, [_P, _I]),
     "lmh_bicgstab_solve_transpose": (_I, [_P, _P, _P, _D, _I64, C.POINTER(_I64), C.POINTER(_D), C.POINTER(_I)]),
     "lmh_dofmap_condensed_dofs": (_I, [_P, C.POINTER(_I64), _P]),

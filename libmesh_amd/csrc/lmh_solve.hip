@@ -1811,4 +1811,10 @@ int lmh_matrix_fp32_compression(lmh_matrix *A) {
   return use_windows(A) ? 1 : 0;
 }
 
+int lmh_matrix_window_bits(lmh_matrix *A) {
+  if (!A) return -1;
+  if (ensure_windows(A) != LMH_OK) return -1;
+  return use_windows(A) ? A->wbits : 0;
+}
+
 }  // extern "C"
