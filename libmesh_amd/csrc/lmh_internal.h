@@ -278,6 +278,17 @@ __device__ inline void block_range(int64_t n, int64_t &lo, int64_t &hi) {
   hi = lo + per < n ? lo + per : n;
 }
 
+// the same with lo a multiple of W (the share rounded up to W items)
+template <int W>
+__device__ inline void block_range_aligned(int64_t n, int64_t &lo, int64_t &hi) {
+  const int lb = logical_block();
+  int64_t per = (n + gridDim.x - 1) / gridDim.x;
+  per = (per + W - 1) / W * W;
+  lo = (int64_t)lb * per;
+  if (lo > n) lo = n;
+  hi = lo + per < n ? lo + per : n;
+}
+
 // grid for an XCD-aware 1-D launch
 inline int grid_for(int64_t work, int per_block, int max_blocks) {
   int64_t g = (work + per_block - 1) / per_block;

@@ -241,7 +241,10 @@ constexpr int UCAP = 768;  // distinct columns of one row block (x window)
 constexpr int W8_CAP = 1024, W8_UCAP = 256;
 // 8-bit blocks run on 128-thread workgroups (8 entries per thread, as the
 // 2048-entry blocks on 256 threads), twice as many of them
-constexpr int W8_NT = 128, W8_GRID = 2;
+#ifndef LMH_W8_GRID
+#define LMH_W8_GRID 2
+#endif
+constexpr int W8_NT = 128, W8_GRID = LMH_W8_GRID;
 template <int IW>
 using widx_t = typename std::conditional<IW == 1, uint8_t, uint16_t>::type;
 // Pipelining.  Every block's loads are addressed from registers only, so no
@@ -736,7 +739,7 @@ __global__ __launch_bounds__(BLK) void k_it_update(int64_t n, const double *__re
                                                    const VT *__restrict__ dinv, const VT *__restrict__ p,
                                                    const VT *__restrict__ q, XT *__restrict__ x,
                                                    VT *__restrict__ r, double *__restrict__ part,
-                                                   double *__restrict__ S, int bank) {
+                                                   double *__restrict__ S, int bank, int xvec) {
   double *Sb = S + bank * B_SIZE;
   if (Sb[B_DONE] != 0.0) return;
   double t[1];
@@ -750,10 +753,40 @@ __global__ __launch_bounds__(BLK) void k_it_update(int64_t n, const double *__re
     return;
   }
   const VT a = (VT)alpha;
+  // 16-byte chunks of W entries (dinv, p, q, r start 16-byte aligned; x too
+  // when xvec), then the workgroup's remaining entries one by one
+  constexpr int W = 16 / sizeof(VT);
+  typedef VT vt __attribute__((ext_vector_type(W)));
+  typedef XT xt __attribute__((ext_vector_type(W)));
   int64_t lo, hi;
-  block_range(n, lo, hi);
+  block_range_aligned<W>(n, lo, hi);
+  const int64_t hv = lo + ((hi - lo) / W) * W;
   double acc[2] = {0, 0};
-  for (int64_t i = lo + threadIdx.x; i < hi; i += BLK) {
+  for (int64_t i = lo + W * threadIdx.x; i < hv; i += W * BLK) {
+    const vt pv = *(const vt *)(p + i), qv = *(const vt *)(q + i), dv = *(const vt *)(dinv + i);
+    vt rv = *(const vt *)(r + i);
+    xt xv;
+    if (xvec)
+      xv = *(const xt *)(x + i);
+    else
+#pragma unroll
+      for (int k = 0; k < W; ++k) xv[k] = x[i + k];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      xv[k] += (XT)(alpha * (double)pv[k]);
+      const VT rk = rv[k] - a * qv[k];
+      rv[k] = rk;
+      acc[0] += (double)rk * (double)rk;
+      acc[1] += (double)rk * (double)(dv[k] * rk);
+    }
+    *(vt *)(r + i) = rv;
+    if (xvec)
+      *(xt *)(x + i) = xv;
+    else
+#pragma unroll
+      for (int k = 0; k < W; ++k) x[i + k] = xv[k];
+  }
+  for (int64_t i = hv + threadIdx.x; i < hi; i += BLK) {
     x[i] += (XT)(alpha * (double)p[i]);
     const VT rv = r[i] - a * q[i];
     r[i] = rv;
@@ -805,9 +838,14 @@ __global__ __launch_bounds__(BLK) void k_it_pupdate(int64_t n, const double *__r
   }
   if (done) return;
   const VT beta = (VT)(rz / Sb[B_RZ]);
+  constexpr int W = 16 / sizeof(VT);
+  typedef VT vt __attribute__((ext_vector_type(W)));
   int64_t lo, hi;
-  block_range(n, lo, hi);
-  for (int64_t i = lo + threadIdx.x; i < hi; i += BLK) p[i] = dinv[i] * r[i] + beta * p[i];
+  block_range_aligned<W>(n, lo, hi);
+  const int64_t hv = lo + ((hi - lo) / W) * W;
+  for (int64_t i = lo + W * threadIdx.x; i < hv; i += W * BLK)
+    *(vt *)(p + i) = *(const vt *)(dinv + i) * *(const vt *)(r + i) + beta * *(const vt *)(p + i);
+  for (int64_t i = hv + threadIdx.x; i < hi; i += BLK) p[i] = dinv[i] * r[i] + beta * p[i];
 }
 
 // FP64 CG start: after k_residual (r = b - A x), p = dinv r; partials (r, p).
@@ -1105,6 +1143,10 @@ static int ensure_windows(lmh_matrix *A) {
 // the allocation-free one included
 int matrix_build_windows(lmh_matrix *A) { return ensure_windows(A); }
 
+// stride of the solvers' work vectors: a multiple of 64 entries, so every
+// vector starts 256-byte aligned (the vector kernels read 16-byte chunks)
+static int64_t work_stride(int64_t n) { return (n + 63) & ~(int64_t)63; }
+
 static int ensure_diag(lmh_matrix *A) {
   if (A->diag) return LMH_OK;
   LMH_HIP(hipMalloc((void **)&A->diag, sizeof(int32_t) * std::max<int64_t>(1, A->rows)));
@@ -1116,8 +1158,8 @@ static int ensure_diag(lmh_matrix *A) {
 static bool use_windows(const lmh_matrix *A) { return A->fp32_index && A->idx16_state == 1; }
 
 static int ensure_work(lmh_matrix *A, bool mixed) {
-  if (!A->w64) LMH_HIP(hipMalloc((void **)&A->w64, sizeof(double) * 4 * A->rows));
-  if (mixed && !A->w32) LMH_HIP(hipMalloc((void **)&A->w32, sizeof(float) * 5 * A->rows));
+  if (!A->w64) LMH_HIP(hipMalloc((void **)&A->w64, sizeof(double) * 4 * work_stride(A->rows)));
+  if (mixed && !A->w32) LMH_HIP(hipMalloc((void **)&A->w32, sizeof(float) * 5 * work_stride(A->rows)));
   return LMH_OK;
 }
 
@@ -1443,6 +1485,7 @@ static int cg_impl(lmh_matrix *A, const double *b_d, double *x_d, double rtol, i
   double *part1 = ctx->d_part, *part2 = ctx->d_part + 2 * ctx->n_part;  // 1 and 2 values per block
   const int g = vec_grid(ctx), gs = spmv_grid(ctx);
   const int gsp = c16 && A->wbits == 8 ? W8_GRID * gs : gs;  // the iteration SpMV's grid
+  const int xvec = ((uintptr_t)x_d & 15) == 0;               // the caller's x takes 16-byte chunks
   CG_LAUNCH("invdiag", (k_invdiag<double, double>), dim3(grid_for(n, BLK, 8192)), n, diag, A->vals,
             A->precond == LMH_PRECOND_JACOBI, dinv);
   CG_LAUNCH("residual_fp64", k_residual, dim3(gs), A->rb, A->nrb, A->rp, A->ci, A->vals, b_d, x_d, r, part2);
@@ -1471,7 +1514,7 @@ static int cg_impl(lmh_matrix *A, const double *b_d, double *x_d, double rtol, i
         CG_LAUNCH("cg_spmv_fp64", k_it_spmv<double>, dim3(gs), A->rb, A->nrb, A->rp, A->ci, A->vals, p, q, part1, S,
                   bank);
       CG_LAUNCH("cg_update_fp64", (k_it_update<double, double>), dim3(g), n, part1, gsp, dinv, p, q, x_d, r, part2, S,
-                bank);
+                bank, xvec);
       CG_LAUNCH("cg_pupdate_fp64", k_it_pupdate<double>, dim3(g), n, part2, g, dinv, r, p, S, bank, 0);
     }
     LMH_HIP(hipMemcpyAsync(snap + 32 * slot, S, sizeof(double) * 2 * B_SIZE, hipMemcpyDeviceToHost, ctx->stream));
@@ -1513,7 +1556,9 @@ int lmh_cg_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int
   LMH_TRY(ensure_diag(A));
   const int64_t n = A->rows;
   double *r = A->w64;
-  return cg_impl<false>(A, b_d, x_d, rtol, maxits, r, r + n, r + 2 * n, r + 3 * n, A->diag, iters, rel_res, reason);
+  const int64_t ns = work_stride(n);
+  return cg_impl<false>(A, b_d, x_d, rtol, maxits, r, r + ns, r + 2 * ns, r + 3 * ns, A->diag, iters, rel_res,
+                        reason);
 }
 
 int lmh_cg_solve_noalloc(lmh_matrix *A, const double *b_d, double *x_d, double rtol, int64_t maxits,
@@ -1694,7 +1739,8 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
   const int64_t n = A->rows;
   // xc: FP32 solution correction accumulated since the last refresh, folded
   // into the FP64 x before each refresh
-  float *r32 = A->w32, *p32 = r32 + n, *q32 = p32 + n, *d32 = q32 + n, *xc = d32 + n;
+  const int64_t ns = work_stride(n);
+  float *r32 = A->w32, *p32 = r32 + ns, *q32 = p32 + ns, *d32 = q32 + ns, *xc = d32 + ns;
   double *S = ctx->d_scal;
   double *part1 = ctx->d_part, *part2 = ctx->d_part + 2 * ctx->n_part;
   const int g = vec_grid(ctx), gs = spmv_grid(ctx);
@@ -1739,7 +1785,7 @@ int lmh_mixed_solve(lmh_matrix *A, const double *b_d, double *x_d, double rtol, 
         LMH_LAUNCH(ctx, "mx_spmv_fp32", k_it_spmv<float>, dim3(gs), dim3(BLK), 0, A->rb, A->nrb, A->rp, A->ci,
                    A->vals32, p32, q32, part1, S, bank);
       LMH_LAUNCH(ctx, "mx_update_fp32", (k_it_update<float, float>), dim3(g), dim3(BLK), 0, n, part1, gsp, d32, p32,
-                 q32, xc, r32, part2, S, bank);
+                 q32, xc, r32, part2, S, bank, 1);
       LMH_LAUNCH(ctx, "mx_pupdate_fp32", k_it_pupdate<float>, dim3(g), dim3(BLK), 0, n, part2, g, d32, r32, p32, S,
                  bank, 1);
     }
