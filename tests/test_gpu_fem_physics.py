@@ -2,7 +2,9 @@
 fem_physics_dropin.hip): the reaction-diffusion -div(kappa grad u) + sigma u^3 = 1
 written as a device element_time_derivative, assembled through
 FEMSystem::assembly and solved by NewtonSolver, checked against the oracle's
-FEMSystem restatement (orc_fem_reaction_diffusion) at 1e-12."""
+FEMSystem restatement (orc_fem_reaction_diffusion) at 1e-12; and a
+three-variable linear elasticity physics checked against the oracle's
+elasticity assembly (systems_of_equations_ex6's operator)."""
 import os
 import subprocess
 
@@ -106,3 +108,47 @@ def test_newton_dropin(tmp_path, n, t, perturb):
     # the nonlinearity is visible: the sigma = 0 (Poisson) solution differs
     lin = cpu_newton(s, NPTS[t], 0.0, kappa)
     assert np.linalg.norm(lin - ref) > 1e-2 * np.linalg.norm(ref)
+
+
+CLAMP = 1 << 4  # boundary id 4: the x = 0 face
+
+
+@pytest.mark.parametrize("n,t,perturb", [(3, O.HEX8, True), (2, O.HEX27, True), (2, O.TET10, False)])
+def test_elasticity_physics_matches_oracle(tmp_path, n, t, perturb):
+    """Three variables (var-major element DoFs): the residual K u - F and the
+    Jacobian K of the user elasticity physics against the oracle's assembly."""
+    m = O.Mesh(n, t)
+    if perturb:
+        m.perturb(0x5EED0001, 0.2)
+    s = O.System(m, 3, CLAMP)
+    cons = s.constrained().astype(bool)
+    u = np.random.default_rng(13).standard_normal(s.n_dofs) * 0.1
+    u[cons] = 0.0
+    u.tofile(tmp_path / "u_in.bin")
+    run([n, t, tmp_path / "el", "eval", "elasticity", f"u={tmp_path / 'u_in.bin'}"] + (["perturb"] if perturb else []),
+        tmp_path)
+    R = np.fromfile(tmp_path / "el_R.bin")
+    K = np.fromfile(tmp_path / "el_K.bin")
+    Kv, F = s.assemble_elasticity(NPTS[t], 1.0, 0.3, -1.0)
+    rp, ci = s.pattern()
+    A = sp.csr_matrix((Kv, ci, rp), shape=(s.n_dofs, s.n_dofs))
+    ref = A @ u - F
+    ref[cons] = 0.0
+    assert K.shape == Kv.shape
+    assert np.max(np.abs(K - Kv)) <= 1e-12 * np.max(np.abs(Kv))
+    assert np.linalg.norm(R - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_elasticity_physics_newton(tmp_path):
+    """The linear elasticity physics through NewtonSolver: one Newton step to
+    the oracle's direct solution."""
+    n, t = 3, O.HEX27
+    m = O.Mesh(n, t)
+    m.perturb(0x5EED0001, 0.2)
+    s = O.System(m, 3, CLAMP)
+    out = run([n, t, tmp_path / "en", "newton", "elasticity", "perturb", "rtol=1e-12"], tmp_path)
+    u = np.fromfile(tmp_path / "en_u.bin")
+    Kv, F = s.assemble_elasticity(3, 1.0, 0.3, -1.0)
+    rp, ci = s.pattern()
+    ref = spla.spsolve(sp.csr_matrix((Kv, ci, rp), shape=(s.n_dofs, s.n_dofs)).tocsc(), F)
+    assert np.linalg.norm(u - ref) <= 1e-9 * np.linalg.norm(ref), out
