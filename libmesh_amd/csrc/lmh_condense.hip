@@ -156,6 +156,76 @@ __global__ __launch_bounds__(256) void k_sc_values_wave(int64_t n_red, const int
   }
 }
 
+// The same values with reduced row i held in LDS: its columns and running
+// values are staged once, then for each condensed column c of row i
+// (ascending, wave-uniform) the lanes walk row c (the condensed DoF's
+// element, 27 entries for Q2) and find each column in the staged row by a
+// binary search in LDS instead of in global memory.  Every entry receives the
+// same subtractions in the same order as in k_sc_values_wave, so the values
+// are bit-identical.  LDS: max_row int32 + max_row doubles per wavefront.
+__device__ inline void sc_wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(256) void k_sc_values_lds(int64_t n_red, const int32_t *__restrict__ rfull,
+                                                       const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                       const double *__restrict__ v, const int32_t *__restrict__ ridx,
+                                                       const int32_t *__restrict__ dpos,
+                                                       const int32_t *__restrict__ srp, double *__restrict__ sv,
+                                                       int max_row) {
+  extern __shared__ double sc_sm[];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double *acc = sc_sm + (size_t)w * max_row;
+  int32_t *cols = (int32_t *)(sc_sm + (size_t)(blockDim.x >> 6) * max_row) + (size_t)w * max_row;
+  const uint64_t below = (1ull << lane) - 1;
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + w; r < n_red; r += nw) {
+    const int32_t i = rfull[r], k0 = rp[i], k1 = rp[i + 1], len = k1 - k0;
+    for (int32_t t = lane; t < len; t += 64) {
+      cols[t] = ci[k0 + t];
+      acc[t] = v[k0 + t];
+    }
+    sc_wave_lds_sync();
+    for (int32_t cb = k0; cb < k1; cb += 64) {
+      const int32_t kc = cb + lane;
+      const int32_t cc = kc < k1 ? ci[kc] : 0;
+      const int32_t qc = kc < k1 ? ridx[cc] : 0;
+      uint64_t m = __ballot(kc < k1 && qc < 0 && v[kc] != 0.0);
+      while (m) {
+        const int b = __builtin_ctzll(m);
+        m &= m - 1;
+        const int32_t c = __shfl(cc, b), q = -1 - __shfl(qc, b);
+        const double f = v[cb + b] / v[dpos[q]];
+        for (int32_t mm = rp[c] + lane; mm < rp[c + 1]; mm += 64) {
+          const int32_t j = ci[mm];
+          if (ridx[j] < 0) continue;
+          int32_t lo = 0, hi = len;
+          while (lo < hi) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (cols[mid] < j)
+              lo = mid + 1;
+            else
+              hi = mid;
+          }
+          if (lo < len && cols[lo] == j) acc[lo] -= f * v[mm];
+        }
+        sc_wave_lds_sync();  // this condensed column's updates land before the next one's
+      }
+    }
+    int32_t out = srp[r];
+    for (int32_t t0 = 0; t0 < len; t0 += 64) {
+      const int32_t t = t0 + lane;
+      const bool keep = t < len && ridx[cols[t]] >= 0;
+      const uint64_t km = __ballot(keep);
+      if (keep) sv[out + __popcll(km & below)] = acc[t];
+      out += __popcll(km);
+    }
+    sc_wave_lds_sync();  // the staged row is read before the next row overwrites it
+  }
+}
+
 // g_r = b_i - sum_{c in row i, condensed} K_ic b_c / K_cc; one wavefront per
 // reduced row, lanes over the row's entries (coalesced), then a fixed-order
 // shuffle reduction.
@@ -224,6 +294,11 @@ static int sc_setup(lmh_condensation *sc) {
     LMH_LAUNCH(ctx, "sc_values_check", k_sc_values, dim3(grid_for(sc->n_red, 256, 16384)), dim3(256), 0, sc->n_red,
                sc->rfull, A->rp, A->ci, A->vals, sc->ridx, sc->dpos, R->rp, R->ci, R->vals, ctx->d_flags);
     LMH_TRY(check_launch("k_sc_values"));
+  } else if ((size_t)A->max_row * 12 * 4 <= 48 * 1024) {
+    LMH_LAUNCH(ctx, "sc_values", k_sc_values_lds, dim3(grid_for(sc->n_red, 4, 65536)), dim3(256),
+               (size_t)A->max_row * 12 * 4, sc->n_red, sc->rfull, A->rp, A->ci, A->vals, sc->ridx, sc->dpos, R->rp,
+               R->vals, A->max_row);
+    LMH_TRY(check_launch("k_sc_values_lds"));
   } else {
     LMH_LAUNCH(ctx, "sc_values", k_sc_values_wave, dim3(grid_for(sc->n_red, 4, 65536)), dim3(256), 0, sc->n_red,
                sc->rfull, A->rp, A->ci, A->vals, sc->ridx, sc->dpos, R->rp, R->vals);

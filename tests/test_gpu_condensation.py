@@ -55,8 +55,9 @@ def test_condensed_operator_and_solve(ctx, n):
     assert np.array_equal(rp, osc.S.indptr) and np.array_equal(ci, osc.S.indices)
     assert rel(sc.values(), osc.S.data) < 1e-12
     v_first = sc.values()
-    sc.setup()  # values again through the wavefront-per-row kernel
-    assert rel(sc.values(), osc.S.data) < 1e-12 and rel(sc.values(), v_first) < 1e-14
+    sc.setup()  # values again through the LDS-staged wavefront-per-row kernel
+    # the same subtractions in the same order per entry as the first (checking) pass
+    assert np.array_equal(sc.values(), v_first)
     gr = L.DeviceArray(ctx, sc.n_reduced)
     sc.forward(b, gr)
     og = osc.forward(orhs)
