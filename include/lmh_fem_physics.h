@@ -38,8 +38,9 @@ namespace lmh_fem {
 constexpr int kWave = 64;
 constexpr int kMaxNodes = 27, kMaxQp = 64, kMaxVars = 3;
 
-// FEMContext of one element on the device.  Arrays live in LDS (FE values)
-// and in the element's slots of Fe / Ke (outputs).
+// FEMContext of one element on the device.  The element's FE values live in
+// LDS (JxW, xyz, dphi, u and grad u at the points), phi in the reference
+// table in global memory, the outputs in the element's slots of Fe / Ke.
 struct Context {
   int64_t elem;
   int nodes, vars, qps, dofs, lane_id;
@@ -72,11 +73,12 @@ struct Context {
   }
 };
 
-// LDS (doubles) of one block for n nodes, nq points, nv variables
+// LDS of one block for n nodes, nq points, nv variables: the element's own
+// data only (the reference tables w, phi and dref are read from global memory,
+// where every element's reads hit the cache), so several workgroups fit a CU
 inline size_t lds_bytes(int n, int nq, int nv) {
   const size_t nd = (size_t)nv * n;
-  const size_t doubles = nq + (size_t)n * nq + (size_t)n * nq * 3  // w, phi, dref
-                         + (size_t)n * 3 + nd                       // X, u_e
+  const size_t doubles = (size_t)n * 3 + nd                         // X, u_e
                          + nq + (size_t)nq * 3 + (size_t)nq * 9     // JxW, xyz, inverse map
                          + (size_t)n * nq * 3                       // physical dphi
                          + (size_t)nv * nq * 4;                     // u, grad u at the points
@@ -90,10 +92,8 @@ __global__ __launch_bounds__(kWave) void k_element_loop(Physics phys, lmh_fem_da
   extern __shared__ double sm[];
   const int lane = threadIdx.x;
   const int n = d.n_nodes, nq = d.n_qp, nv = d.n_vars, nd = d.n_dofs;
-  double *w = sm;
-  double *phi = w + nq;
-  double *dref = phi + n * nq;
-  double *X = dref + n * nq * 3;
+  const double *__restrict__ w = d.w, *__restrict__ phi = d.phi, *__restrict__ dref = d.dref;
+  double *X = sm;
   double *ue = X + n * 3;
   double *jxw = ue + nd;
   double *xq = jxw + nq;
@@ -102,10 +102,6 @@ __global__ __launch_bounds__(kWave) void k_element_loop(Physics phys, lmh_fem_da
   double *uq = dphi + n * nq * 3;
   double *gq = uq + nv * nq;
   unsigned char *cons = (unsigned char *)(gq + nv * nq * 3);
-  for (int t = lane; t < nq; t += kWave) w[t] = d.w[t];
-  for (int t = lane; t < n * nq; t += kWave) phi[t] = d.phi[t];
-  for (int t = lane; t < n * nq * 3; t += kWave) dref[t] = d.dref[t];
-  __syncthreads();
   for (int64_t e = blockIdx.x; e < d.n_elem; e += gridDim.x) {
     for (int a = lane; a < n; a += kWave) {
       const int64_t nd3 = 3 * (int64_t)d.conn[e * n + a];
