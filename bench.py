@@ -689,28 +689,24 @@ def main():
     condensed = bench_condensed(args, L, ctx, kappa, log)
     log(f"condensed: {condensed}")
 
-    # assembly kernel timings (FP64 path, one profiled assembly)
-    ctx.profile_reset()
-    ctx.profile(True)
-    g.assemble_poisson(3, kd, 0, b, L.FP64, scatter)
-    ctx.sync()
-    ctx.profile(False)
+    # assembly kernel timings: mean per assembly over ASM_REPS profiled
+    # assemblies (FP64 path, then the FP32 MFMA contraction + FP32 scatter,
+    # which writes the FP32 operator only)
+    ASM_REPS = 10
     asm_k = {}
-    for k in ("element_hex27_mfma_fp64", "scatter_gather_fp64", "scatter_colored_fp64"):
-        ms, c = ctx.profile_query(k)
-        if c:
-            asm_k[k] = ms * c
-    rhs_h = b.download()  # the FP64 right-hand side, compared with the oracle's below
-    # FP32 MFMA contraction + FP32 scatter (writes the FP32 operator only)
-    ctx.profile_reset()
-    ctx.profile(True)
-    g.assemble_poisson(3, kd, 0, b, L.FP32, scatter)
-    ctx.sync()
-    ctx.profile(False)
-    for k in ("element_hex27_mfma_fp32", "scatter_gather_fp32", "scatter_colored_fp32"):
-        ms, c = ctx.profile_query(k)
-        if c:
-            asm_k[k] = ms * c
+    for prec, tag in ((L.FP64, "fp64"), (L.FP32, "fp32")):
+        ctx.profile_reset()
+        ctx.profile(True)
+        for _ in range(ASM_REPS):
+            g.assemble_poisson(3, kd, 0, b, prec, scatter)
+        ctx.sync()
+        ctx.profile(False)
+        for k in ("element_hex27_mfma_", "scatter_gather_", "scatter_colored_"):
+            ms, c = ctx.profile_query(k + tag)
+            if c:
+                asm_k[k + tag] = ms * c / ASM_REPS
+        if prec == L.FP64:
+            rhs_h = b.download()  # the FP64 right-hand side, compared with the oracle's below
     x64 = results["fp64"].pop("x")
     xmx = results["mixed"].pop("x")
     err_mixed_vs_fp64 = float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))
