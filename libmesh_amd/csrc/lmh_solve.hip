@@ -532,10 +532,24 @@ __device__ inline void reduce_all(const double *__restrict__ part, int G, double
   __shared__ double sh[NV][BLK / 64];
   __shared__ double res[NV];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // eight independent loads per thread in flight (a consumer reads up to
+  // 8192 partials; one dependent load per step costs microseconds of latency)
+  constexpr int U = 8;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    double s = 0;
-    for (int b = threadIdx.x; b < G; b += BLK) s += part[k * G + b];
+    const double *pk = part + k * G;
+    double a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) a[u] = 0;
+    int b = threadIdx.x;
+    for (; b + (U - 1) * BLK < G; b += U * BLK) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) a[u] += pk[b + u * BLK];
+    }
+#pragma unroll
+    for (int u = 0; u < U - 1; ++u)
+      if (b + u * BLK < G) a[u] += pk[b + u * BLK];
+    double s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     s = wave_sum(s);
     if (lane == 0) sh[k][wv] = s;
   }
