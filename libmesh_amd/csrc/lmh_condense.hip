@@ -34,6 +34,12 @@ struct lmh_condensation {
   int32_t *dpos = nullptr;   // n_cond: position of K_cc in A's row c
   double *g = nullptr, *y = nullptr;  // reduced right-hand side and solution
   bool pattern_checked = false;       // k_sc_values found no fill-in (then k_sc_values_wave is used)
+  // Position map (pattern only, built after the fill check): for reduced row r
+  // and each structurally condensed column c of its full row i (ascending),
+  // the full-row-i position of every entry of row c, -1 for condensed or
+  // absent columns.  pmoff[r] .. pmoff[r + 1] are row r's entries.
+  int64_t *pmoff = nullptr;
+  int16_t *pmap = nullptr;
 };
 
 namespace lmh {
@@ -280,6 +286,188 @@ __global__ __launch_bounds__(256) void k_sc_backward(int64_t n_cond, const int32
   }
 }
 
+// Inclusive prefix sum over the wavefront's lanes.
+__device__ inline int32_t sc_wave_scan(int32_t x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int32_t y = __shfl_up(x, d);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+
+// Position-map sizes: per reduced row, the entries of all rows c of its
+// structurally condensed columns.  One wavefront per reduced row.
+__global__ __launch_bounds__(256) void k_sc_map_count(int64_t n_red, const int32_t *__restrict__ rfull,
+                                                      const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                      const int32_t *__restrict__ ridx, int64_t *__restrict__ cnt) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); r < n_red; r += nw) {
+    const int32_t i = rfull[r], k1 = rp[i + 1];
+    int64_t tot = 0;
+    for (int32_t cb = rp[i]; cb < k1; cb += 64) {
+      const int32_t kc = cb + lane;
+      const int32_t cc = kc < k1 ? ci[kc] : 0;
+      const int32_t lenc = kc < k1 && ridx[cc] < 0 ? rp[cc + 1] - rp[cc] : 0;
+      tot += __shfl(sc_wave_scan(lenc), 63);
+    }
+    if (lane == 0) cnt[r] = tot;
+  }
+}
+
+// Position-map entries (same walk as k_sc_values_lds, positions instead of
+// updates).  LDS: max_row int32 per wavefront.
+__global__ __launch_bounds__(256) void k_sc_map_fill(int64_t n_red, const int32_t *__restrict__ rfull,
+                                                     const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                     const int32_t *__restrict__ ridx, const int64_t *__restrict__ pmoff,
+                                                     int16_t *__restrict__ pmap, int max_row) {
+  extern __shared__ int32_t sc_cols[];
+  const int lane = threadIdx.x & 63;
+  int32_t *cols = sc_cols + (size_t)(threadIdx.x >> 6) * max_row;
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); r < n_red; r += nw) {
+    const int32_t i = rfull[r], k0 = rp[i], k1 = rp[i + 1], len = k1 - k0;
+    for (int32_t t = lane; t < len; t += 64) cols[t] = ci[k0 + t];
+    sc_wave_lds_sync();
+    int64_t base = pmoff[r];
+    for (int32_t cb = k0; cb < k1; cb += 64) {
+      const int32_t kc = cb + lane;
+      const int32_t cc = kc < k1 ? ci[kc] : 0;
+      const bool isc = kc < k1 && ridx[cc] < 0;
+      const int32_t lenc = isc ? rp[cc + 1] - rp[cc] : 0;
+      const int32_t inc = sc_wave_scan(lenc), moff = inc - lenc;
+      uint64_t m = __ballot(isc);
+      while (m) {
+        const int b = __builtin_ctzll(m);
+        m &= m - 1;
+        const int32_t c = __shfl(cc, b), mo = __shfl(moff, b);
+        for (int32_t mm = rp[c] + lane; mm < rp[c + 1]; mm += 64) {
+          const int32_t j = ci[mm];
+          int32_t pos = -1;
+          if (ridx[j] >= 0) {
+            int32_t lo = 0, hi = len;
+            while (lo < hi) {
+              const int32_t mid = (lo + hi) >> 1;
+              if (cols[mid] < j)
+                lo = mid + 1;
+              else
+                hi = mid;
+            }
+            if (lo < len && cols[lo] == j) pos = lo;
+          }
+          pmap[base + mo + (mm - rp[c])] = (int16_t)pos;
+        }
+      }
+      base += __shfl(inc, 63);
+    }
+    sc_wave_lds_sync();  // cols are read before the next row overwrites them
+  }
+}
+
+// Reduced values through the position map: the same subtractions in the same
+// order as k_sc_values_lds (condensed columns of row i ascending, those with a
+// zero K_ic skipped, f_c = K_ic / K_cc), so the values are bit-identical; the
+// factors of a 64-entry chunk are formed by all lanes at once and each row c
+// is walked with coalesced loads of its values and map entries, with no
+// searches.  LDS: max_row doubles per wavefront.
+__global__ __launch_bounds__(256) void k_sc_values_map(int64_t n_red, const int32_t *__restrict__ rfull,
+                                                       const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                       const double *__restrict__ v, const int32_t *__restrict__ ridx,
+                                                       const int32_t *__restrict__ dpos,
+                                                       const int64_t *__restrict__ pmoff,
+                                                       const int16_t *__restrict__ pmap,
+                                                       const int32_t *__restrict__ srp, double *__restrict__ sv,
+                                                       int max_row) {
+  extern __shared__ double sc_sm[];
+  const int lane = threadIdx.x & 63;
+  double *acc = sc_sm + (size_t)(threadIdx.x >> 6) * max_row;
+  const uint64_t below = (1ull << lane) - 1;
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); r < n_red; r += nw) {
+    const int32_t i = rfull[r], k0 = rp[i], k1 = rp[i + 1], len = k1 - k0;
+    for (int32_t t = lane; t < len; t += 64) acc[t] = v[k0 + t];
+    sc_wave_lds_sync();
+    int64_t base = pmoff[r];
+    for (int32_t cb = k0; cb < k1; cb += 64) {
+      const int32_t kc = cb + lane;
+      const int32_t cc = kc < k1 ? ci[kc] : 0;
+      const int32_t qc = kc < k1 ? ridx[cc] : 0;
+      const bool isc = kc < k1 && qc < 0;
+      const int32_t c0 = isc ? rp[cc] : 0, lenc = isc ? rp[cc + 1] - c0 : 0;
+      const double vic = isc ? v[kc] : 0.0;
+      const double fl = vic != 0.0 ? vic / v[dpos[-1 - qc]] : 0.0;
+      const int32_t inc = sc_wave_scan(lenc), moff = inc - lenc;
+      uint64_t m = __ballot(vic != 0.0);
+      while (m) {
+        const int b = __builtin_ctzll(m);
+        m &= m - 1;
+        const double f = __shfl(fl, b);
+        const int32_t cs = __shfl(c0, b), L = __shfl(lenc, b);
+        const int16_t *mp = pmap + base + __shfl(moff, b);
+        for (int32_t t = lane; t < L; t += 64) {
+          const int32_t pos = mp[t];
+          if (pos >= 0) acc[pos] -= f * v[cs + t];
+        }
+        sc_wave_lds_sync();  // this condensed column's updates land before the next one's
+      }
+      base += __shfl(inc, 63);
+    }
+    int32_t out = srp[r];
+    for (int32_t t0 = 0; t0 < len; t0 += 64) {
+      const int32_t t = t0 + lane;
+      const bool keep = t < len && ridx[ci[k0 + t]] >= 0;
+      const uint64_t km = __ballot(keep);
+      if (keep) sv[out + __popcll(km & below)] = acc[t];
+      out += __popcll(km);
+    }
+    sc_wave_lds_sync();  // the staged row is read before the next row overwrites it
+  }
+}
+
+// Position map, built once the pattern has passed the fill check; skipped
+// (the refresh keeps k_sc_values_lds) when rows are too long for int16
+// positions or the map would exceed 4 GiB.
+static int sc_build_map(lmh_condensation *sc) {
+  lmh_matrix *A = sc->A;
+  lmh_context *ctx = A->ctx;
+  // the refresh stages max_row doubles per wavefront (4 per workgroup)
+  if (!sc->n_cond || A->max_row >= 32768 || (size_t)A->max_row * 8 * 4 > 48 * 1024) return LMH_OK;
+  const int64_t nr = sc->n_red;
+  if (hipMalloc((void **)&sc->pmoff, sizeof(int64_t) * (nr + 1)) != hipSuccess) {
+    (void)hipGetLastError();
+    sc->pmoff = nullptr;
+    return LMH_OK;  // no room: the refresh keeps the search kernel
+  }
+  LMH_LAUNCH(ctx, "sc_map_count", k_sc_map_count, dim3(grid_for(nr, 4, 65536)), dim3(256), 0, nr, sc->rfull, A->rp,
+             A->ci, sc->ridx, sc->pmoff);
+  std::vector<int64_t> h(nr + 1);
+  LMH_HIP(hipMemcpyAsync(h.data(), sc->pmoff, sizeof(int64_t) * nr, hipMemcpyDeviceToHost, ctx->stream));
+  LMH_HIP(hipStreamSynchronize(ctx->stream));
+  int64_t tot = 0;
+  for (int64_t r = 0; r < nr; ++r) {
+    const int64_t c = h[r];
+    h[r] = tot;
+    tot += c;
+  }
+  h[nr] = tot;
+  if (tot * (int64_t)sizeof(int16_t) > (int64_t(4) << 30) ||
+      hipMalloc((void **)&sc->pmap, sizeof(int16_t) * std::max<int64_t>(1, tot)) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(sc->pmoff);
+    sc->pmoff = nullptr;
+    sc->pmap = nullptr;
+    return LMH_OK;
+  }
+  LMH_HIP(hipMemcpyAsync(sc->pmoff, h.data(), sizeof(int64_t) * (nr + 1), hipMemcpyHostToDevice, ctx->stream));
+  LMH_LAUNCH(ctx, "sc_map_fill", k_sc_map_fill, dim3(grid_for(nr, 4, 65536)), dim3(256), (size_t)A->max_row * 4 * 4,
+             nr, sc->rfull, A->rp, A->ci, sc->ridx, sc->pmoff, sc->pmap, A->max_row);
+  LMH_TRY(check_launch("k_sc_map_fill"));
+  LMH_HIP(hipStreamSynchronize(ctx->stream));  // h is read by the copy above
+  return LMH_OK;
+}
+
 static int sc_setup(lmh_condensation *sc) {
   lmh_matrix *A = sc->A, *R = sc->R;
   lmh_context *ctx = A->ctx;
@@ -294,6 +482,11 @@ static int sc_setup(lmh_condensation *sc) {
     LMH_LAUNCH(ctx, "sc_values_check", k_sc_values, dim3(grid_for(sc->n_red, 256, 16384)), dim3(256), 0, sc->n_red,
                sc->rfull, A->rp, A->ci, A->vals, sc->ridx, sc->dpos, R->rp, R->ci, R->vals, ctx->d_flags);
     LMH_TRY(check_launch("k_sc_values"));
+  } else if (sc->pmap) {
+    LMH_LAUNCH(ctx, "sc_values", k_sc_values_map, dim3(grid_for(sc->n_red, 4, 65536)), dim3(256),
+               (size_t)A->max_row * 8 * 4, sc->n_red, sc->rfull, A->rp, A->ci, A->vals, sc->ridx, sc->dpos, sc->pmoff,
+               sc->pmap, R->rp, R->vals, A->max_row);
+    LMH_TRY(check_launch("k_sc_values_map"));
   } else if ((size_t)A->max_row * 12 * 4 <= 48 * 1024) {
     LMH_LAUNCH(ctx, "sc_values", k_sc_values_lds, dim3(grid_for(sc->n_red, 4, 65536)), dim3(256),
                (size_t)A->max_row * 12 * 4, sc->n_red, sc->rfull, A->rp, A->ci, A->vals, sc->ridx, sc->dpos, R->rp,
@@ -314,7 +507,10 @@ static int sc_setup(lmh_condensation *sc) {
   LMH_CHECK(!(flags & SC_ZEROPIVOT), LMH_ERR_DEGENERATE, "zero diagonal on a condensed DoF");
   LMH_CHECK(!(flags & SC_FILL), LMH_ERR_UNSUPPORTED,
             "condensation needs an entry outside the matrix pattern (the condensed DoF is not element-interior)");
-  sc->pattern_checked = true;
+  if (!sc->pattern_checked) {
+    sc->pattern_checked = true;
+    LMH_TRY(sc_build_map(sc));
+  }
   return LMH_OK;
 }
 
@@ -450,6 +646,8 @@ int lmh_condensation_destroy(lmh_condensation *sc) {
   (void)hipFree(sc->dpos);
   (void)hipFree(sc->g);
   (void)hipFree(sc->y);
+  (void)hipFree(sc->pmoff);
+  (void)hipFree(sc->pmap);
   delete sc;
   return LMH_OK;
 }

@@ -187,3 +187,25 @@ def test_structural_zero_coupling_and_structural_fill(ctx):
     with pytest.raises(L.LmhError, match="outside the matrix pattern"):
         L.StaticCondensation(prob, np.array([0], dtype=np.int32))
     L.check(L.hip().lmh_matrix_destroy(prob.matrix))
+
+
+def test_refresh_skips_zero_couplings(ctx):
+    """A reduced row whose structural coupling to a condensed DoF holds an exact
+    zero (K_20 = 0): the first pass and every refresh skip that condensed
+    column and give the dense Schur complement, bit-identically to each other."""
+    n = 7
+    rng = np.random.default_rng(5)
+    M = rng.standard_normal((n, n))
+    K = M @ M.T + n * np.eye(n)
+    K[2, 0] = K[0, 2] = 0.0
+    K[0, 1] = K[1, 0] = 0.0
+    prob, rp, ci = _csr_problem(ctx, K, np.ones((n, n), dtype=bool))
+    sc = L.StaticCondensation(prob, np.array([0, 1], dtype=np.int32))
+    S = K[2:, 2:] - K[2:, :2] @ np.linalg.inv(K[:2, :2]) @ K[:2, 2:]
+    v_first = sc.values()
+    assert rel(v_first, S.ravel()) < 1e-13
+    for _ in range(2):
+        sc.setup()
+        assert np.array_equal(sc.values(), v_first)
+    sc.close()
+    L.check(L.hip().lmh_matrix_destroy(prob.matrix))
