@@ -436,7 +436,7 @@ int lmh_dofmap_condensed_dofs(const lmh_dofmap *dm, int64_t *n, int32_t *dofs_h)
  * Also builds the device position map that later refreshes use: 2 bytes per
  * entry of each condensed row, counted once per reduced row that couples to it
  * (27 x 26 x 2 B per condensed DoF for HEX27, 368 MB for 64^3); without room
- * for it (or for rows over 32,767 entries) refreshes search the pattern instead. */
+ * for it (or for rows over 384 entries) refreshes search the pattern instead. */
 int lmh_condensation_create(lmh_matrix *A, int64_t n_cond, const int32_t *cond_dofs_h, lmh_condensation **out);
 /* Recomputes the reduced values after A's values changed (StaticCondensation::setup/close). */
 int lmh_condensation_setup(lmh_condensation *sc);

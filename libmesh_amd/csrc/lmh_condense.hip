@@ -366,12 +366,24 @@ __global__ __launch_bounds__(256) void k_sc_map_fill(int64_t n_red, const int32_
   }
 }
 
+// Inclusive prefix sum over the 16 lanes of a lane group.
+__device__ inline int32_t sc_group_scan(int32_t x) {
+  const int sl = threadIdx.x & 15;
+#pragma unroll
+  for (int d = 1; d < 16; d <<= 1) {
+    const int32_t y = __shfl_up(x, d, 16);
+    if (sl >= d) x += y;
+  }
+  return x;
+}
+
 // Reduced values through the position map: the same subtractions in the same
 // order as k_sc_values_lds (condensed columns of row i ascending, those with a
-// zero K_ic skipped, f_c = K_ic / K_cc), so the values are bit-identical; the
-// factors of a 64-entry chunk are formed by all lanes at once and each row c
-// is walked with coalesced loads of its values and map entries, with no
-// searches.  LDS: max_row doubles per wavefront.
+// zero K_ic skipped, f_c = K_ic / K_cc), so the values are bit-identical.
+// Each row c is walked with coalesced loads of its values and map entries, no
+// searches.  A wavefront runs four reduced rows, one per 16-lane group, so
+// four rows' chains of dependent loads are in flight at once.  LDS: max_row
+// doubles per row (16 per workgroup).
 __global__ __launch_bounds__(256) void k_sc_values_map(int64_t n_red, const int32_t *__restrict__ rfull,
                                                        const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
                                                        const double *__restrict__ v, const int32_t *__restrict__ ridx,
@@ -381,46 +393,48 @@ __global__ __launch_bounds__(256) void k_sc_values_map(int64_t n_red, const int3
                                                        const int32_t *__restrict__ srp, double *__restrict__ sv,
                                                        int max_row) {
   extern __shared__ double sc_sm[];
-  const int lane = threadIdx.x & 63;
-  double *acc = sc_sm + (size_t)(threadIdx.x >> 6) * max_row;
-  const uint64_t below = (1ull << lane) - 1;
-  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
-  for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); r < n_red; r += nw) {
+  const int lane = threadIdx.x & 63, sl = lane & 15, gb = lane & 48;
+  const int slot = (threadIdx.x >> 6) * 4 + (lane >> 4);
+  double *acc = sc_sm + (size_t)slot * max_row;
+  const uint32_t below = (1u << sl) - 1;
+  auto gballot = [&](bool p) { return (uint32_t)(__ballot(p) >> gb) & 0xffffu; };
+  const int64_t ng = (int64_t)gridDim.x * (blockDim.x >> 4);
+  for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 4) + slot; r < n_red; r += ng) {
     const int32_t i = rfull[r], k0 = rp[i], k1 = rp[i + 1], len = k1 - k0;
-    for (int32_t t = lane; t < len; t += 64) acc[t] = v[k0 + t];
+    for (int32_t t = sl; t < len; t += 16) acc[t] = v[k0 + t];
     sc_wave_lds_sync();
     int64_t base = pmoff[r];
-    for (int32_t cb = k0; cb < k1; cb += 64) {
-      const int32_t kc = cb + lane;
+    for (int32_t cb = k0; cb < k1; cb += 16) {
+      const int32_t kc = cb + sl;
       const int32_t cc = kc < k1 ? ci[kc] : 0;
       const int32_t qc = kc < k1 ? ridx[cc] : 0;
       const bool isc = kc < k1 && qc < 0;
       const int32_t c0 = isc ? rp[cc] : 0, lenc = isc ? rp[cc + 1] - c0 : 0;
       const double vic = isc ? v[kc] : 0.0;
       const double fl = vic != 0.0 ? vic / v[dpos[-1 - qc]] : 0.0;
-      const int32_t inc = sc_wave_scan(lenc), moff = inc - lenc;
-      uint64_t m = __ballot(vic != 0.0);
+      const int32_t inc = sc_group_scan(lenc), moff = inc - lenc;
+      uint32_t m = gballot(vic != 0.0);
       while (m) {
-        const int b = __builtin_ctzll(m);
+        const int b = __builtin_ctz(m);
         m &= m - 1;
-        const double f = __shfl(fl, b);
-        const int32_t cs = __shfl(c0, b), L = __shfl(lenc, b);
-        const int16_t *mp = pmap + base + __shfl(moff, b);
-        for (int32_t t = lane; t < L; t += 64) {
+        const double f = __shfl(fl, b, 16);
+        const int32_t cs = __shfl(c0, b, 16), L = __shfl(lenc, b, 16);
+        const int16_t *mp = pmap + base + __shfl(moff, b, 16);
+        for (int32_t t = sl; t < L; t += 16) {
           const int32_t pos = mp[t];
           if (pos >= 0) acc[pos] -= f * v[cs + t];
         }
         sc_wave_lds_sync();  // this condensed column's updates land before the next one's
       }
-      base += __shfl(inc, 63);
+      base += __shfl(inc, 15, 16);
     }
     int32_t out = srp[r];
-    for (int32_t t0 = 0; t0 < len; t0 += 64) {
-      const int32_t t = t0 + lane;
+    for (int32_t t0 = 0; t0 < len; t0 += 16) {
+      const int32_t t = t0 + sl;
       const bool keep = t < len && ridx[ci[k0 + t]] >= 0;
-      const uint64_t km = __ballot(keep);
-      if (keep) sv[out + __popcll(km & below)] = acc[t];
-      out += __popcll(km);
+      const uint32_t km = gballot(keep);
+      if (keep) sv[out + __popc(km & below)] = acc[t];
+      out += __popc(km);
     }
     sc_wave_lds_sync();  // the staged row is read before the next row overwrites it
   }
@@ -432,8 +446,8 @@ __global__ __launch_bounds__(256) void k_sc_values_map(int64_t n_red, const int3
 static int sc_build_map(lmh_condensation *sc) {
   lmh_matrix *A = sc->A;
   lmh_context *ctx = A->ctx;
-  // the refresh stages max_row doubles per wavefront (4 per workgroup)
-  if (!sc->n_cond || A->max_row >= 32768 || (size_t)A->max_row * 8 * 4 > 48 * 1024) return LMH_OK;
+  // the refresh stages max_row doubles per reduced row (16 per workgroup)
+  if (!sc->n_cond || A->max_row >= 32768 || (size_t)A->max_row * 8 * 16 > 48 * 1024) return LMH_OK;
   const int64_t nr = sc->n_red;
   if (hipMalloc((void **)&sc->pmoff, sizeof(int64_t) * (nr + 1)) != hipSuccess) {
     (void)hipGetLastError();
@@ -483,8 +497,8 @@ static int sc_setup(lmh_condensation *sc) {
                sc->rfull, A->rp, A->ci, A->vals, sc->ridx, sc->dpos, R->rp, R->ci, R->vals, ctx->d_flags);
     LMH_TRY(check_launch("k_sc_values"));
   } else if (sc->pmap) {
-    LMH_LAUNCH(ctx, "sc_values", k_sc_values_map, dim3(grid_for(sc->n_red, 4, 65536)), dim3(256),
-               (size_t)A->max_row * 8 * 4, sc->n_red, sc->rfull, A->rp, A->ci, A->vals, sc->ridx, sc->dpos, sc->pmoff,
+    LMH_LAUNCH(ctx, "sc_values", k_sc_values_map, dim3(grid_for(sc->n_red, 16, 65536)), dim3(256),
+               (size_t)A->max_row * 8 * 16, sc->n_red, sc->rfull, A->rp, A->ci, A->vals, sc->ridx, sc->dpos, sc->pmoff,
                sc->pmap, R->rp, R->vals, A->max_row);
     LMH_TRY(check_launch("k_sc_values_map"));
   } else if ((size_t)A->max_row * 12 * 4 <= 48 * 1024) {
