@@ -189,11 +189,13 @@ def test_structural_zero_coupling_and_structural_fill(ctx):
     L.check(L.hip().lmh_matrix_destroy(prob.matrix))
 
 
-def test_refresh_skips_zero_couplings(ctx):
+@pytest.mark.parametrize("n", [7, 400])
+def test_refresh_skips_zero_couplings(ctx, n):
     """A reduced row whose structural coupling to a condensed DoF holds an exact
     zero (K_20 = 0): the first pass and every refresh skip that condensed
-    column and give the dense Schur complement, bit-identically to each other."""
-    n = 7
+    column and give the dense Schur complement, bit-identically to each other.
+    n = 7 refreshes through the position map; rows of 400 entries exceed its
+    limit (384), so n = 400 refreshes through the LDS search kernel."""
     rng = np.random.default_rng(5)
     M = rng.standard_normal((n, n))
     K = M @ M.T + n * np.eye(n)
